@@ -1,0 +1,290 @@
+// Small kernels around the fused stage: initial primitive->conserved
+// conversion, primitive recovery for download, CFL partials from stored
+// primitives, the single-workgroup reduce/finalize that advances the
+// device-resident time loop, and the per-operator kernels behind the
+// reference's plugin entry points (boundary fill, edge reconstruction, edge
+// flux) used for operator-level parity.
+#pragma once
+
+#include "stage_kernel.hpp"
+
+namespace cato {
+
+struct FieldGeom {
+  int ni, nj, nh, pitch;
+  int bc[4];
+  double gamma, cfl;
+  const double *dxi, *dyj;
+};
+
+// equation_of_state.f90:297-324 on the interior.
+__global__ void k_prim_to_cons(FieldGeom g, const double *__restrict__ rho, const double *__restrict__ u,
+                               const double *__restrict__ v, const double *__restrict__ p, double *__restrict__ U0,
+                               double *__restrict__ U1, double *__restrict__ U2, double *__restrict__ U3) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y;
+  if (i >= g.ni) return;
+  size_t k = (size_t)j * g.pitch + i;
+  double gm1 = g.gamma - 1.0;
+  double one_over = 1.0 / gm1;  // field / real is field * (1/real) (field_2d.f90:1169-1190)
+  double r = rho[k], uu = u[k], vv = v[k], pp = p[k];
+  U0[k] = r;
+  U1[k] = uu * r;
+  U2[k] = vv * r;
+  double t1 = pp * one_over;
+  double t2 = r * 0.5;
+  double t5 = op_add(uu * uu, vv * vv);
+  U3[k] = op_add(t1, t2 * t5);
+}
+
+// fluid_t%calculate_derived_quantities for download (u, v, p, cs).
+__global__ void k_derived(FieldGeom g, const double *__restrict__ U0, const double *__restrict__ U1,
+                          const double *__restrict__ U2, const double *__restrict__ U3, double *__restrict__ u,
+                          double *__restrict__ v, double *__restrict__ p, double *__restrict__ cs) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y;
+  if (i >= g.ni) return;
+  size_t k = (size_t)j * g.pitch + i;
+  double c;
+  Prim<double> q = cons_to_prim(U0[k], U1[k], U2[k], U3[k], g.gamma, g.gamma - 1.0, &c);
+  u[k] = q.u;
+  v[k] = q.v;
+  p[k] = q.p;
+  cs[k] = c;
+}
+
+// Sound speed of stored primitives (equation_of_state.f90:250-251).
+__global__ void k_sound_speed(FieldGeom g, const double *__restrict__ rho, const double *__restrict__ p,
+                              double *__restrict__ cs) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y;
+  if (i >= g.ni) return;
+  size_t k = (size_t)j * g.pitch + i;
+  cs[k] = sqrtr(op_div(p[k], rho[k]) * g.gamma);
+}
+
+// CFL partial minima over the interior (get_timestep, mod_fluid.f90:554-558).
+// from_prims: u, v unfiltered and cs from (rho, p) as after fluid_t%initialize;
+// otherwise the primitives are recovered from the conserved state.
+__global__ __launch_bounds__(kThreads) void k_cfl_partial(FieldGeom g, int from_prims, const double *__restrict__ a0,
+                                                          const double *__restrict__ a1, const double *__restrict__ a2,
+                                                          const double *__restrict__ a3, double *__restrict__ block_min,
+                                                          int *__restrict__ block_err) {
+  __shared__ double rm[kThreads / 64];
+  __shared__ int re[kThreads / 64];
+  size_t n = (size_t)g.ni * g.nj;
+  double m = __builtin_inf();
+  int err = 0;
+  for (size_t t = (size_t)blockIdx.x * kThreads + threadIdx.x; t < n; t += (size_t)gridDim.x * kThreads) {
+    int j = (int)(t / g.ni), i = (int)(t - (size_t)j * g.ni);
+    size_t k = (size_t)j * g.pitch + i;
+    double u, v, cs, rho, p;
+    if (from_prims) {
+      rho = a0[k];
+      u = a1[k];
+      v = a2[k];
+      p = a3[k];
+      cs = sqrtr(op_div(p, rho) * g.gamma);
+    } else {
+      Prim<double> q = cons_to_prim(a0[k], a1[k], a2[k], a3[k], g.gamma, g.gamma - 1.0, &cs);
+      rho = q.rho;
+      u = q.u;
+      v = q.v;
+      p = q.p;
+    }
+    if (rho != rho || u != u || v != v || p != p) err |= 1;
+    if (rho < 0.0) err |= 2;
+    if (p < 0.0) err |= 4;
+    double dx = absd(g.dxi[i + g.nh]), dy = absd(g.dyj[j + g.nh]);
+    double val = g.cfl / (((absd(u) + cs) / dx) + ((absd(v) + cs) / dy));
+    m = (val < m) ? val : m;
+  }
+  double wm = wave_min(m);
+  int we = wave_or(err);
+  if ((threadIdx.x & 63) == 0) {
+    rm[threadIdx.x >> 6] = wm;
+    re[threadIdx.x >> 6] = we;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double mm = rm[0];
+    int ee = re[0];
+    for (int k = 1; k < kThreads / 64; ++k) {
+      mm = (rm[k] < mm) ? rm[k] : mm;
+      ee |= re[k];
+    }
+    block_min[blockIdx.x] = mm;
+    block_err[blockIdx.x] = ee;
+  }
+}
+
+// Error code of error.f90:6-9 from the OR of sanity flags.
+__device__ inline int err_code(int flags) {
+  if (flags & 1) return 3;  // NANS_FOUND
+  if (flags & 2) return 1;  // NEG_DENSITY
+  if (flags & 4) return 2;  // NEG_PRESSURE
+  return 0;
+}
+
+// Reduce per-workgroup partials into out[0] = min dt, out[1] = flags; clears flags.
+// mode 0: plain reduction into out; mode 1: initialise the loop (first dt);
+// mode 2: end of a step of the main.f90 time loop.
+__global__ __launch_bounds__(1024) void k_reduce_finalize(int nb, double *__restrict__ block_min,
+                                                          int *__restrict__ block_err, double *__restrict__ out,
+                                                          LoopState *__restrict__ L, int mode) {
+  __shared__ double sm[16];
+  __shared__ int se[16];
+  if (mode == 2 && L->done) return;
+  double m = __builtin_inf();
+  int e = 0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    double x = block_min[b];
+    m = (x < m) ? x : m;
+    e |= block_err[b];
+    block_err[b] = 0;
+  }
+  m = wave_min(m);
+  e = wave_or(e);
+  if ((threadIdx.x & 63) == 0) {
+    sm[threadIdx.x >> 6] = m;
+    se[threadIdx.x >> 6] = e;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  int nw = (blockDim.x + 63) / 64;
+  for (int k = 1; k < nw; ++k) {
+    m = (sm[k] < m) ? sm[k] : m;
+    e |= se[k];
+  }
+  m = (sm[0] < m) ? sm[0] : m;
+  e |= se[0];
+  if (out) {
+    out[0] = m;
+    out[1] = (double)e;
+  }
+  if (mode == 1) {
+    L->dt = (L->constant_dt > 0.0) ? L->constant_dt : m;
+    if (!(L->time < L->max_time && L->iteration < L->max_iterations)) L->done = 1;
+  } else if (mode == 2) {
+    // main.f90:167-196 (master%time += dt; contour cadence; dt clip on the driver time)
+    const double EPS = Lim<double>::eps;
+    L->master_time = L->master_time + L->dt;
+    double new_time = L->master_time;
+    if (absd(L->time - L->next_output_time) < EPS) L->next_output_time = L->next_output_time + L->contour_dt;
+    if (new_time > L->next_output_time) {
+      double delta_t = absd(L->next_output_time - L->time);
+      new_time = L->time + delta_t;
+    }
+    L->time = new_time;
+    L->iteration = L->iteration + 1;
+    int code = err_code(e);
+    if (code) {
+      L->error = code;
+      L->done = 1;
+    }
+    L->dt = (L->constant_dt > 0.0) ? L->constant_dt : m;
+    if (!(L->time < L->max_time && L->iteration < L->max_iterations)) L->done = 1;
+  }
+}
+
+__global__ void k_set_scalar(double *dst, double value) { *dst = value; }
+
+// --------------------------- per-operator kernels --------------------------
+// Halo'd column-major arrays: element (i, j), i,j in [1-nh, n+nh], at
+// (i+nh-1) + (j+nh-1)*ld.
+
+// apply_primitive_bc: every ghost cell takes the value of its mapped interior cell.
+__global__ void k_apply_bc(FieldGeom g, int ld, double *rho, double *u, double *v, double *p) {
+  int nx = g.ni + 2 * g.nh, ny = g.nj + 2 * g.nh;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nx * ny) return;
+  int ii = t % nx, jj = t / nx;
+  int gi = ii + 1 - g.nh, gj = jj + 1 - g.nh;
+  if (gi >= 1 && gi <= g.ni && gj >= 1 && gj <= g.nj) return;
+  bool fu, fv;
+  int si = map_axis(gi, g.ni, g.bc[0], g.bc[1], fu);
+  int sj = map_axis(gj, g.nj, g.bc[2], g.bc[3], fv);
+  size_t d = (size_t)ii + (size_t)jj * ld;
+  size_t s = (size_t)(si + g.nh - 1) + (size_t)(sj + g.nh - 1) * ld;
+  rho[d] = rho[s];
+  u[d] = fu ? -u[s] : u[s];
+  v[d] = fv ? -v[s] : v[s];
+  p[d] = p[s];
+}
+
+// muscl_interpolation_t%interpolate_edge_values on a halo'd field q:
+// edges (2, 0:ni, 0:nj) for both directions.
+template <typename R, int LIMC>
+__global__ void k_interpolate(FieldGeom g, int ld, const double *__restrict__ q, double *__restrict__ ie,
+                              double *__restrict__ je) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = (g.ni + 1) * (g.nj + 1);
+  if (t >= 2 * n) return;
+  int dir = t >= n;
+  int k = t - dir * n;
+  int i = k % (g.ni + 1), j = k / (g.ni + 1);
+  // local copy of the stencil neighbourhood as R
+  R buf[7][3];
+  for (int b = -1; b <= 1; ++b)
+    for (int s = -3; s <= 3; ++s) {
+      int gi = dir == 0 ? i + s : i + b;
+      int gj = dir == 0 ? j + b : j + s;
+      R val = R(0);
+      if (gi >= 1 - g.nh && gi <= g.ni + g.nh && gj >= 1 - g.nh && gj <= g.nj + g.nh)
+        val = (R)q[(size_t)(gi + g.nh - 1) + (size_t)(gj + g.nh - 1) * ld];
+      buf[s + 3][b + 1] = val;
+    }
+  R L, Rv;
+  // normal stride 3 (rows of buf), transverse stride 1
+  recon_var<R, LIMC>(&buf[3][1], 3, 1, L, Rv);
+  double *dst = dir == 0 ? ie : je;
+  dst[2 * (size_t)k + 0] = (double)L;
+  dst[2 * (size_t)k + 1] = (double)Rv;
+}
+
+// get_<dir>flux on given edge states; p is the halo'd cell pressure.
+template <typename R, int FLUX>
+__global__ void k_edge_flux(FieldGeom g, int ld, const double *__restrict__ edges, const double *__restrict__ p,
+                            int low_mach_fix, double *__restrict__ iflux, double *__restrict__ jflux) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int nI = (g.ni + 1) * g.nj, nJ = g.ni * (g.nj + 1);
+  if (t >= nI + nJ) return;
+  int dir = t >= nI;
+  int i, j;
+  if (!dir) {
+    i = t % (g.ni + 1);
+    j = t / (g.ni + 1) + 1;
+  } else {
+    int k = t - nI;
+    i = k % g.ni + 1;
+    j = k / g.ni;
+  }
+  size_t ne = (size_t)2 * (g.ni + 1) * (g.nj + 1);
+  size_t e = 2 * ((size_t)i + (size_t)(g.ni + 1) * j);
+  const double *b = edges + (dir ? ne : 0);
+  EdgeState<R> s;
+  s.rL = (R)b[0 * 2 * ne + e]; s.rR = (R)b[0 * 2 * ne + e + 1];
+  s.uL = (R)b[1 * 2 * ne + e]; s.uR = (R)b[1 * 2 * ne + e + 1];
+  s.vL = (R)b[2 * 2 * ne + e]; s.vR = (R)b[2 * 2 * ne + e + 1];
+  s.pL = (R)b[3 * 2 * ne + e]; s.pR = (R)b[3 * 2 * ne + e + 1];
+  auto P = [&](int a, int c) -> R {
+    int gi = dir == 0 ? i + a : i + c;
+    int gj = dir == 0 ? j + c : j + a;
+    return (R)p[(size_t)(gi + g.nh - 1) + (size_t)(gj + g.nh - 1) * ld];
+  };
+  PressNbr<R> pn;
+  pn.p00 = P(0, 0); pn.p10 = P(1, 0); pn.p20 = P(2, 0); pn.pm10 = P(-1, 0);
+  pn.p0p = P(0, 1); pn.p0m = P(0, -1); pn.p1p = P(1, 1); pn.p1m = P(1, -1);
+  R f[4];
+  if (!dir) {
+    edge_flux<R, FLUX, 0>((R)g.gamma, low_mach_fix, s, pn, f);
+    double *d = iflux + 4 * ((size_t)i + (size_t)(g.ni + 1) * (j - 1));
+    for (int v = 0; v < 4; ++v) d[v] = (double)f[v];
+  } else {
+    edge_flux<R, FLUX, 1>((R)g.gamma, low_mach_fix, s, pn, f);
+    double *d = jflux + 4 * ((size_t)(i - 1) + (size_t)g.ni * j);
+    for (int v = 0; v < 4; ++v) d[v] = (double)f[v];
+  }
+}
+
+}  // namespace cato

@@ -1,0 +1,82 @@
+// Shared enumerations and launch-argument structures of the CATO HIP library.
+#pragma once
+
+#include "../../include/cato_hip.h"
+
+namespace cato {
+
+enum FluxId {
+  FLUX_AUSMPW_PLUS = CATO_FLUX_AUSMPW_PLUS,
+  FLUX_SLAU = CATO_FLUX_SLAU,
+  FLUX_SLAU2 = CATO_FLUX_SLAU2,
+  FLUX_SD_SLAU = CATO_FLUX_SD_SLAU,
+  FLUX_SD_SLAU2 = CATO_FLUX_SD_SLAU2,
+  FLUX_ROE = CATO_FLUX_ROE,
+  FLUX_AUSM_PLUS = CATO_FLUX_AUSM_PLUS,
+  FLUX_HLLC = CATO_FLUX_HLLC,
+  FLUX_COUNT = 8
+};
+
+enum LimId {
+  LIM_MINMOD = CATO_LIMITER_MINMOD,
+  LIM_SUPERBEE = CATO_LIMITER_SUPERBEE,
+  LIM_VAN_LEER = CATO_LIMITER_VAN_LEER,
+  LIM_MLP3 = CATO_LIMITER_MLP3,
+  LIM_MLP5 = CATO_LIMITER_MLP5,
+  LIM_COUNT = 5
+};
+
+enum BcId { BC_PERIODIC = CATO_BC_PERIODIC, BC_ZERO_GRADIENT = CATO_BC_ZERO_GRADIENT, BC_SYMMETRY = CATO_BC_SYMMETRY };
+
+// How a stage kernel combines the residual L(U_k) with the stored states.
+enum StageMode {
+  MODE_FIRST = 0,    // out = U_k + L*(c*dt)                 (SSP stage 1; c = 1)
+  MODE_COMBINE = 1,  // out = ((U_0*a) + (U_k*b)) + L*(c*dt) (SSP stages >= 2)
+  MODE_BASE = 2,     // out = U_0 + L*(dt*c)                 (rk(k,n) low-storage stages)
+  MODE_RESIDUAL = 3  // out = L                              (flux_solver_t%solve)
+};
+
+// Device-resident scalars of one solve's time loop (main.f90:138-198).
+struct LoopState {
+  double dt;                // dt of the next step
+  double time;              // driver time (with the contour-interval clip)
+  double master_time;       // unclipped accumulated time (master_puppeteer.f90:206)
+  double next_output_time;  // contour cadence that drives the clip
+  double contour_dt;
+  double max_time;
+  double constant_dt;       // > 0: use_constant_delta_t
+  long long iteration;
+  long long max_iterations;
+  int done;
+  int error;
+  int pad[2];
+};
+
+// Per-field pointers of one conserved state, interior only, pitched rows.
+struct State4 {
+  double *f[4];
+};
+
+struct StageArgs {
+  int ni, nj, nh, pitch;     // interior cells, halo width, row pitch (elements)
+  int mode;
+  int prim_from_array;       // 1: primitives come from prim[] instead of U_k
+  int last_stage;            // 1: write CFL partials and sanity flags for the new state
+  int low_mach_fix;
+  int bc[4];                 // +x, -x, +y, -y
+  int tiles_x, tiles_y;      // tile grid
+  double gamma, cfl;
+  double a, b, c;            // stage coefficients
+  const double *uk[4];       // U_k (primitives source)
+  const double *u0[4];       // U_0 (MODE_COMBINE / MODE_BASE)
+  const double *prim[4];     // rho, u, v, p when prim_from_array
+  double *out[4];
+  const double *dxi, *lxi;   // per halo'd i: x spacing and x-edge length
+  const double *dyj, *lyj;   // per halo'd j
+  const LoopState *loop;     // dt source and done flag (may be null -> dt_value)
+  double dt_value;
+  double *block_min;         // per-block CFL minimum (last stage)
+  int *block_err;            // per-block sanity flags
+};
+
+}  // namespace cato

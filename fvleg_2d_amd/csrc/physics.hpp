@@ -1,0 +1,552 @@
+// Device physics for the CATO hot path on gfx950: field-operator arithmetic,
+// primitive recovery, MUSCL/TVD2 and MLP3/MLP5 edge reconstruction, and the
+// AUSM+/AUSMPW+/SLAU-family/Roe/HLLC edge fluxes.
+//
+// Every routine keeps the reference's operation order (citations inline) so
+// that, compiled with -ffp-contract=off and IEEE division/sqrt, the FP64
+// instantiation reproduces the reference arithmetic bit for bit.  The same
+// templates instantiated with float give the reduced-precision flux form.
+//
+// Fortran semantics restated here:
+//   MAX(a1,a2,..) keeps the running value unless a later argument compares
+//   greater (MIN: less); SIGN(1,x) = copysign(1,x); x**2 = x*x; products and
+//   sums associate left to right.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+#include "cato_types.hpp"
+
+namespace cato {
+
+template <typename R> struct Lim;
+template <> struct Lim<double> {
+  static constexpr double eps = 2.220446049250313080847e-16;  // epsilon(1.0_rk)
+  static constexpr double tiny = 2.2250738585072013830902e-308; // tiny(1.0_rk)
+};
+template <> struct Lim<float> {
+  // The reduced-precision form keeps the reference's absolute thresholds
+  // (rounded to float); tiny() becomes the float minimum normal.
+  static constexpr float eps = 2.220446049250313080847e-16f;
+  static constexpr float tiny = 1.17549435082228750797e-38f;
+};
+
+template <typename R> __host__ __device__ inline R fmax2(R a, R b) { return (b > a) ? b : a; }
+template <typename R> __host__ __device__ inline R fmin2(R a, R b) { return (b < a) ? b : a; }
+template <typename R> __host__ __device__ inline R sq(R x) { return x * x; }
+
+__host__ __device__ inline double absd(double x) { return __builtin_fabs(x); }
+__host__ __device__ inline float absd(float x) { return __builtin_fabsf(x); }
+__host__ __device__ inline double sqrtr(double x) { return __builtin_sqrt(x); }
+__host__ __device__ inline float sqrtr(float x) { return __builtin_sqrtf(x); }
+__host__ __device__ inline double copysignr(double a, double b) { return __builtin_copysign(a, b); }
+__host__ __device__ inline float copysignr(float a, float b) { return __builtin_copysignf(a, b); }
+
+// field_2d_cpu_operators.f90:84-176: + and - flush |r| < epsilon to zero; field/field
+// division returns 0 when |numerator| < tiny.
+__device__ inline double op_add(double a, double b) { double r = a + b; return (absd(r) < Lim<double>::eps) ? 0.0 : r; }
+__device__ inline double op_sub(double a, double b) { double r = a - b; return (absd(r) < Lim<double>::eps) ? 0.0 : r; }
+__device__ inline double op_div(double a, double b) { return (absd(a) < Lim<double>::tiny) ? 0.0 : a / b; }
+
+// Primitive state of one cell.
+template <typename R> struct Prim { R rho, u, v, p; };
+
+// fluid_t%calculate_derived_quantities for one cell (mod_fluid.f90:564-591 with
+// equation_of_state.f90:286-292 and 250-251): returns rho,u,v,p (u,v Mach-filtered)
+// and the sound speed.
+__device__ inline Prim<double> cons_to_prim(double r, double ru, double rv, double rE, double gamma, double gm1,
+                                            double *cs_out) {
+  double u = op_div(ru, r);
+  double v = op_div(rv, r);
+  double t1 = r * gm1;
+  double t2 = op_div(rE, r);
+  double t5 = op_add(u * u, v * v);
+  double t6 = t5 * 0.5;
+  double t7 = op_sub(t2, t6);
+  double p = t1 * t7;
+  double cs = sqrtr(op_div(p, r) * gamma);
+  // Mach filter (mod_fluid.f90:580-588): |u/cs| < 1e-13 => u = 0.  When |u| >= 2e-13*cs
+  // the quotient is certainly >= 1e-13, so the division is skipped (identical result).
+  const double MF = 1e-13;
+  const double lim = 2.0 * MF * cs;
+  if (!(absd(v) >= lim) && absd(op_div(v, cs)) < MF) v = 0.0;
+  if (!(absd(u) >= lim) && absd(op_div(u, cs)) < MF) u = 0.0;
+  if (cs_out) *cs_out = cs;
+  return Prim<double>{r, u, v, p};
+}
+
+// ---------------------------------------------------------------------------
+// Reconstruction (edge states L/R at the interface between cells 0 and 1).
+// Stencil values along the edge normal: qm3=q(-2), qm=q(-1), q0, q1, q2, q3=q(+3).
+// ---------------------------------------------------------------------------
+template <typename R> __device__ inline R tvd_phi(int lim, R r) {
+  if (lim == LIM_MINMOD) return fmax2(R(0), fmin2(r, R(1)));  // muscl_tvd2.fypp:196
+  if (lim == LIM_SUPERBEE) return fmax2(fmax2(R(0), fmin2(R(2) * r, R(1))), fmin2(r, R(2)));  // :198
+  return (r + absd(r)) / (R(1) + absd(r));  // :200
+}
+
+// muscl_tvd2.fypp:177-220
+template <typename R, int LIMC>
+__device__ inline void recon_tvd2(R qm, R q0, R q1, R q2, R &L, R &Rv) {
+  R dm = q0 - qm, dp = q1 - q0, d3 = q2 - q1;
+  if (absd(dm) < Lim<R>::eps) dm = R(0);
+  if (absd(dp) < Lim<R>::eps) dp = R(0);
+  if (absd(d3) < Lim<R>::eps) d3 = R(0);
+  R r_L = dp / (dm + R(1e-16));
+  R r_R = dp / (d3 + R(1e-16));
+  R phi = tvd_phi<R>(LIMC, r_L);
+  L = q0 + R(0.5) * phi * dm;
+  phi = tvd_phi<R>(LIMC, r_R);
+  Rv = q1 - R(0.5) * phi * d3;
+}
+
+// muscl_mlp.fypp:196-279.  t_* are the transverse neighbours for tan(theta):
+// a0p = q(0,+1), a0m = q(0,-1), a1p = q(1,+1), a1m = q(1,-1) in (normal, transverse).
+template <typename R, int ORDER>
+__device__ inline void recon_mlp(R qm3, R qm, R q0, R q1, R q2, R q3, R a0p, R a0m, R a1p, R a1m, R &L, R &Rv) {
+  const R SMALL = R(1e-30);
+  R dm = q0 - qm, dp = q1 - q0, d3 = q2 - q1;
+  if (absd(dm) < Lim<R>::eps) dm = R(0);
+  if (absd(dp) < Lim<R>::eps) dp = R(0);
+  if (absd(d3) < Lim<R>::eps) d3 = R(0);
+  R r_R_p1 = dp / (d3 + SMALL);
+  R r_L = dp / (dm + SMALL);
+  R beta_L, beta_R;
+  if constexpr (ORDER == 5) {
+    R dm3 = qm - qm3, d5 = q3 - q2;
+    if (absd(dm3) < Lim<R>::eps) dm3 = R(0);
+    if (absd(d5) < Lim<R>::eps) d5 = R(0);
+    R r_L_m1 = dm / (dm3 + SMALL);
+    R r_L_p1 = d3 / (dp + SMALL);
+    R r_R_p2 = d3 / (d5 + SMALL);
+    R r_R = dm / (dp + SMALL);
+    beta_L = ((-(R(2) / (r_L_m1 + SMALL)) + R(11)) + R(24) * r_L - R(3) * r_L * r_L_p1) / R(30);
+    beta_R = ((-(R(2) / (r_R_p2 + SMALL)) + R(11)) + R(24) * r_R_p1 - R(3) * r_R_p1 * r_R) / R(30);
+  } else {
+    (void)qm3;
+    (void)q3;
+    beta_L = (R(1) + R(2) * r_L) / R(3);
+    beta_R = (R(1) + R(2) * r_R_p1) / R(3);
+  }
+  R tt = absd(a0p - a0m) / (absd(q1 - qm) + SMALL);
+  R tt1 = absd(a1p - a1m) / (absd(q2 - q0) + SMALL);
+  R aLt = (R(2) * fmax2(R(1), r_L) * (R(1) + fmax2(R(0), (tt1 / (r_R_p1 + SMALL))))) / (R(1) + tt);
+  R aRt = (R(2) * fmax2(R(1), r_R_p1) * (R(1) + fmax2(R(0), (tt / (r_L + SMALL))))) / (R(1) + tt1);
+  R aL = fmax2(R(1), fmin2(R(2), aLt));
+  R aR = fmax2(R(1), fmin2(R(2), aRt));
+  L = q0 + R(0.5) * fmax2(R(0), fmin2(fmin2(aL * r_L, aL), beta_L)) * dm;
+  Rv = q1 - R(0.5) * fmax2(R(0), fmin2(fmin2(aR * r_R_p1, aR), beta_R)) * d3;
+}
+
+// ---------------------------------------------------------------------------
+// Edge fluxes.  DIR 0: i-edge (normal (1,0), the cell's RIGHT edge);
+// DIR 1: j-edge (normal (0,1), TOP edge).  Rectilinear grids give exactly these
+// normals from quad_cell.f90:193-229.
+// ---------------------------------------------------------------------------
+template <typename R> struct EdgeState { R rL, uL, vL, pL, rR, uR, vR, pR; };
+
+// Neighbour cell pressures needed by the shock sensors, in (normal, transverse)
+// coordinates relative to the edge's left cell 0: p(a,b).
+template <typename R> struct PressNbr {
+  R p00, p10, p20, pm10;   // (0,0) (1,0) (2,0) (-1,0)
+  R p0p, p0m, p1p, p1m;    // (0,+1) (0,-1) (1,+1) (1,-1)
+};
+
+// SLAU family: slau_solver.fypp:355-490
+template <typename R, int VARIANT, int DIR>
+__device__ inline void flux_slau(R gamma, const EdgeState<R> &s, const PressNbr<R> &pn, R out[4]) {
+  const R n_x = DIR == 0 ? R(1) : R(0), n_y = DIR == 0 ? R(0) : R(1);
+  const R THRESHOLD = R(5e-3), C_SD1 = R(0.01);
+  const R eps = Lim<R>::eps;
+  R rho_L = s.rL, u_L = s.uL, v_L = s.vL, p_L = s.pL;
+  R rho_R = s.rR, u_R = s.uR, v_R = s.vR, p_R = s.pR;
+  R V_n_plus = u_L * n_x + v_L * n_y;
+  R V_n_minus = u_R * n_x + v_R * n_y;
+  R c_plus = sqrtr(gamma * p_L / rho_L);
+  R c_minus = sqrtr(gamma * p_R / rho_R);
+  R c_bar = R(0.5) * (c_plus + c_minus);
+  R V_n_bar = (rho_L * absd(V_n_plus) + rho_R * absd(V_n_minus)) / (rho_L + rho_R);
+  R vel2 = sq(u_L) + sq(v_L) + sq(u_R) + sq(v_R);
+  R M_hat = fmin2(R(1), sqrtr(R(0.5) * vel2) / c_bar);
+  R Chi = sq(R(1) - M_hat);
+  R M_bar = V_n_bar / c_bar;
+  R p_bar = R(0.5) * (p_L + p_R);
+  R rho_bar = R(0.5) * (rho_L + rho_R);
+  R M_plus = V_n_plus / c_bar;
+  R M_minus = V_n_minus / c_bar;
+  R Beta_plus = (absd(M_plus) < R(1)) ? R(0.25) * (R(2) - M_plus) * sq(M_plus + R(1))
+                                       : R(0.5) * (R(1) + copysignr(R(1), M_plus));
+  R Beta_minus = (absd(M_minus) < R(1)) ? R(0.25) * (R(2) + M_minus) * sq(M_minus - R(1))
+                                         : R(0.5) * (R(1) + copysignr(R(1), -M_minus));
+  R delta_p = p_R - p_L;
+  if (absd(delta_p) < eps || absd(delta_p) < THRESHOLD * p_bar) delta_p = R(0);
+  R delta_rho = rho_R - rho_L;
+  if (absd(delta_rho) < eps) delta_rho = R(0);
+  R delta_Beta = Beta_plus - Beta_minus;
+  if (absd(delta_Beta) < eps) delta_Beta = R(0);
+  R p_L_minus_R = p_L - p_R;
+  if (absd(p_L_minus_R) < eps) p_L_minus_R = R(0);
+  constexpr bool SD = (VARIANT == FLUX_SD_SLAU || VARIANT == FLUX_SD_SLAU2);
+  R theta = R(0);
+  if constexpr (SD) {
+    // slau_solver.fypp:410-448, cell pressures in the edge's (normal, transverse) frame
+    R d[6];
+    if constexpr (DIR == 0) {
+      d[0] = absd(pn.p0p - pn.p00);   // p(i,j+1)   - p(i,j)
+      d[1] = absd(pn.pm10 - pn.p00);  // p(i-1,j)   - p(i,j)
+      d[2] = absd(pn.p0m - pn.p00);   // p(i,j-1)   - p(i,j)
+      d[3] = absd(pn.p1m - pn.p10);   // p(i+1,j-1) - p(i+1,j)
+      d[4] = absd(pn.p20 - pn.p10);   // p(i+2,j)   - p(i+1,j)
+      d[5] = absd(pn.p1p - pn.p10);   // p(i+1,j+1) - p(i+1,j)
+    } else {
+      d[0] = absd(pn.p0m - pn.p00);   // p(i-1,j)   - p(i,j)
+      d[1] = absd(pn.pm10 - pn.p00);  // p(i,j-1)   - p(i,j)
+      d[2] = absd(pn.p0p - pn.p00);   // p(i+1,j)   - p(i,j)
+      d[3] = absd(pn.p1p - pn.p10);   // p(i+1,j+1) - p(i,j+1)
+      d[4] = absd(pn.p20 - pn.p10);   // p(i,j+2)   - p(i,j+1)
+      d[5] = absd(pn.p1m - pn.p10);   // p(i-1,j+1) - p(i,j+1)
+    }
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      if (d[k] < eps || d[k] < THRESHOLD * p_bar) d[k] = R(0);
+    }
+    R mx6 = d[0];
+#pragma unroll
+    for (int k = 1; k < 6; ++k) mx6 = fmax2(mx6, d[k]);
+    R delta_p_max = fmax2(absd(delta_p), mx6);
+    theta = fmin2(R(1), sq(((C_SD1 * absd(delta_p) / p_bar) + C_SD1) / ((delta_p_max / p_bar) + C_SD1))) *
+            fmax2(R(0), R(1) - absd(M_bar));
+  } else {
+    (void)pn;
+  }
+  R p_tilde;
+  if constexpr (VARIANT == FLUX_SLAU || VARIANT == FLUX_SD_SLAU)
+    p_tilde = (p_bar + R(0.5) * delta_Beta * p_L_minus_R) + (R(1) - Chi) * (Beta_plus + Beta_minus - R(1)) * p_bar;
+  else
+    p_tilde = p_bar + (R(0.5) * delta_Beta * p_L_minus_R) +
+              sqrtr(R(0.5) * vel2) * (Beta_plus + Beta_minus - R(1)) * rho_bar * c_bar;
+  R g = -(fmax2(fmin2(M_plus, R(0)), R(-1)) * fmin2(fmax2(M_minus, R(0)), R(1)));
+  R base = R(0.5) * ((rho_L * V_n_plus) + (rho_R * V_n_minus) - absd(V_n_bar) * delta_rho) * (R(1) - g);
+  R mass_flux;
+  if constexpr (SD) mass_flux = base - ((theta / (R(2) * c_bar)) * delta_p);
+  else mass_flux = base - ((Chi / (R(2) * c_bar)) * delta_p);
+  R mfL = R(0.5) * (mass_flux + absd(mass_flux));
+  R mfR = R(0.5) * (mass_flux - absd(mass_flux));
+  R gg = gamma / (gamma - R(1));
+  R H_L = gg * (p_L / rho_L) + R(0.5) * (sq(u_L) + sq(v_L));
+  R H_R = gg * (p_R / rho_R) + R(0.5) * (sq(u_R) + sq(v_R));
+  out[0] = mfL + mfR;
+  out[1] = (mfL * u_L) + (mfR * u_R) + p_tilde * n_x;
+  out[2] = (mfL * v_L) + (mfR * v_R) + p_tilde * n_y;
+  out[3] = (mfL * H_L) + (mfR * H_R);
+}
+
+// AUSMPW+: ausmpw_plus_solver.fypp:341-458 and split functions 486-568 (alpha = 1/8)
+template <typename R> __device__ inline R pw_P_plus(R M) {
+  const R alpha = R(1) / R(8);
+  if (absd(M) > R(1)) return R(0.5) * (R(1) + copysignr(R(1), M));
+  return R(0.25) * sq(M + R(1)) * (R(2) - M) + alpha * M * sq(sq(M) - R(1));
+}
+template <typename R> __device__ inline R pw_P_minus(R M) {
+  const R alpha = R(1) / R(8);
+  if (absd(M) > R(1)) return R(0.5) * (R(1) - copysignr(R(1), M));
+  return R(0.25) * sq(M - R(1)) * (R(2) + M) - alpha * M * sq(sq(M) - R(1));
+}
+template <typename R> __device__ inline R pw_M_plus(R M) {
+  if (absd(M) > R(1)) return R(0.5) * (M + absd(M));
+  return R(0.25) * sq(M + R(1));
+}
+template <typename R> __device__ inline R pw_M_minus(R M) {
+  if (absd(M) > R(1)) return R(0.5) * (M - absd(M));
+  return -(R(0.25) * sq(M - R(1)));
+}
+
+template <typename R, int DIR>
+__device__ inline void flux_ausmpw(R gamma, int low_mach_fix, const EdgeState<R> &s, const PressNbr<R> &pn,
+                                   R out[4]) {
+  const R n_x = DIR == 0 ? R(1) : R(0), n_y = DIR == 0 ? R(0) : R(1);
+  const R LOW_MACH = R(0.1);  // globals.f90:32
+  R rho_L = s.rL, u_LHS = s.uL, v_LHS = s.vL, p_L = s.pL;
+  R rho_R = s.rR, u_RHS = s.uR, v_RHS = s.vR, p_R = s.pR;
+  R U_L = DIR == 0 ? u_LHS : v_LHS, U_R = DIR == 0 ? u_RHS : v_RHS;
+  R V_L = DIR == 0 ? v_LHS : u_LHS, V_R = DIR == 0 ? v_RHS : u_RHS;
+  R cs_L = sqrtr(gamma * p_L / rho_L);
+  R cs_R = sqrtr(gamma * p_R / rho_R);
+  R M_LHS = sqrtr(sq(u_LHS) + sq(v_LHS)) / cs_L;
+  R M_RHS = sqrtr(sq(u_RHS) + sq(v_RHS)) / cs_R;
+  if (low_mach_fix && (absd(M_LHS) < LOW_MACH || absd(M_RHS) < LOW_MACH)) {
+    R f_M = fmin2(R(1), fmax2(M_LHS, M_RHS));
+    R ulf = R(0.5) * (u_LHS + u_RHS) + f_M * R(0.5) * (u_LHS - u_RHS);
+    R urf = R(0.5) * (u_LHS + u_RHS) - f_M * R(0.5) * (u_LHS - u_RHS);
+    R vlf = R(0.5) * (v_LHS + v_RHS) + f_M * R(0.5) * (v_LHS - v_RHS);
+    R vrf = R(0.5) * (v_LHS + v_RHS) - f_M * R(0.5) * (v_LHS - v_RHS);
+    u_LHS = ulf; u_RHS = urf; v_LHS = vlf; v_RHS = vrf;
+  }
+  R gg = gamma / (gamma - R(1));
+  R H_L = gg * (p_L / rho_L) + R(0.5) * (sq(u_LHS) + sq(v_LHS));
+  R H_R = gg * (p_R / rho_R) + R(0.5) * (sq(u_RHS) + sq(v_RHS));
+  R H_normal = fmin2(H_L - R(0.5) * sq(V_L), H_R - R(0.5) * sq(V_R));
+  R c_s = sqrtr(R(2) * ((gamma - R(1)) / (gamma + R(1))) * H_normal);
+  R c_half = (R(0.5) * (U_L + U_R) < R(0)) ? sq(c_s) / fmax2(absd(U_R), c_s) : sq(c_s) / fmax2(absd(U_L), c_s);
+  R M_L = U_L / c_half;
+  R M_R = U_R / c_half;
+  R M_L_plus = pw_M_plus(M_L);
+  R M_R_minus = pw_M_minus(M_R);
+  R P_R_minus = pw_P_minus(M_R);
+  R P_L_plus = pw_P_plus(M_L);
+  R p_s = p_L * P_L_plus + p_R * P_R_minus;
+  R mt = fmin2((p_L / p_R), (p_R / p_L));
+  R w = R(1) - mt * mt * mt;
+  // get_f neighbours (ausmpw_plus_solver.fypp:433-441): i-dir p1=(0,-1),(1,-1) p2=(0,+1),(1,+1);
+  // j-dir p1=(i+1,j),(i+1,j+1) p2=(i-1,j),(i-1,j+1), i.e. transverse +1 then -1.
+  R p1L = DIR == 0 ? pn.p0m : pn.p0p, p1R = DIR == 0 ? pn.p1m : pn.p1p;
+  R p2L = DIR == 0 ? pn.p0p : pn.p0m, p2R = DIR == 0 ? pn.p1p : pn.p1m;
+  R f_L, f_R;
+  if (absd(p_s) < Lim<R>::tiny) {
+    f_L = R(0);
+    f_R = R(0);
+  } else {
+    R mn4 = fmin2(fmin2(fmin2(p1L, p1R), p2L), p2R);
+    R m1 = fmin2(R(1), mn4 / fmin2(p_L, p_R));
+    R min_term = sq(m1);
+    f_L = ((p_L / p_s) - R(1)) * min_term;
+    f_R = ((p_R / p_s) - R(1)) * min_term;
+  }
+  R Mb_L, Mb_R;
+  if (M_L_plus + M_R_minus < R(0)) {
+    Mb_L = M_L_plus * w * (R(1) + f_L);
+    Mb_R = M_R_minus + M_L_plus * ((R(1) - w) * (R(1) + f_L) - f_R);
+  } else {
+    Mb_L = M_L_plus + M_R_minus * ((R(1) - w) * (R(1) + f_R) - f_L);
+    Mb_R = M_R_minus * w * (R(1) + f_R);
+  }
+  R mfL = Mb_L * c_half * rho_L;
+  R mfR = Mb_R * c_half * rho_R;
+  out[0] = mfL + mfR;
+  out[1] = (mfL * u_LHS) + (mfR * u_RHS) + ((P_L_plus * n_x * p_L) + (P_R_minus * n_x * p_R));
+  out[2] = (mfL * v_LHS) + (mfR * v_RHS) + ((P_L_plus * n_y * p_L) + (P_R_minus * n_y * p_R));
+  out[3] = (mfL * H_L) + (mfR * H_R);
+}
+
+// Roe (reference variant): roe_solver.fypp:420-511, f_h 519-527
+template <typename R> __device__ inline R roe_f_h(R pL, R pR) {
+  R ratio = absd(pR - pL) / (fmin2(pR, pL) + R(1e-16));
+  return (ratio < R(1)) ? R(0) : R(1);
+}
+
+template <typename R, int DIR>
+__device__ inline void flux_roe(R gamma, const EdgeState<R> &s, const PressNbr<R> &pn, R out[4]) {
+  const R nx = DIR == 0 ? R(1) : R(0), ny = DIR == 0 ? R(0) : R(1);
+  R rl = absd(s.rL), ul = s.uL, vl = s.vL, pl = absd(s.pL);
+  R gg = gamma / (gamma - R(1));
+  R hl = gg * (pl / rl) + R(0.5) * (sq(ul) + sq(vl));
+  R rr = absd(s.rR), ur = s.uR, vr = s.vR, pr = absd(s.pR);
+  R hr = gg * (pr / rr) + R(0.5) * (sq(ur) + sq(vr));
+  R rav = sqrtr(rl * rr);
+  R gam1 = gamma - R(1);
+  R dd = rav / rl;
+  R dd1 = R(1) / (R(1) + dd);
+  R uav = (ul + dd * ur) * dd1;
+  R vav = (vl + dd * vr) * dd1;
+  R hav = (hl + dd * hr) * dd1;
+  R q2a = R(0.5) * (uav * uav + vav * vav);
+  R c2a = gam1 * (hav - q2a);
+  R cav = sqrtr(c2a);
+  R uv = uav * nx + vav * ny;
+  R du = (ur - ul) * nx + (vr - vl) * ny;
+  R du_trans = -((ur - ul) * ny) + (vr - vl) * nx;
+  R cs_R = sqrtr(gamma * pr / rr);
+  R cs_L = sqrtr(gamma * pl / rl);
+  R eabs1 = absd(fmin2(uv - cav, (ul * nx + vl * ny) - cs_L));
+  R eabs2 = absd(uv);
+  R eabs4 = absd(fmax2(uv + cav, (ur * nx + vr * ny) + cs_R));
+  R h1 = rav * cav * du;
+  R h2 = eabs1 * (pr - pl - h1) / (R(2) * c2a);
+  R h3 = eabs2 * (rr - rl - (pr - pl) / c2a);
+  R h4 = eabs2 * rav;
+  R h5 = eabs4 * (pr - pl + h1) / (R(2) * c2a);
+  R psi = absd(cs_R - cs_L) / fmax2(cs_R, cs_L);
+  R FR0 = rr * uv, FR1 = rr * ur * uv + nx * pr, FR2 = rr * vr * uv + ny * pr, FR3 = rr * hr * uv;
+  R FL0 = rl * uv, FL1 = rl * ul * uv + nx * pl, FL2 = rl * vl * uv + ny * pl, FL3 = rl * hl * uv;
+  R rm0 = (h2 + h3 + h5);
+  R rm1 = (h2 * (uav - cav * nx) + h3 * uav + h4 * (ur - ul - du * nx) + h5 * (uav + cav * nx));
+  R rm2 = (h2 * (vav - cav * ny) + h3 * vav + h4 * (vr - vl - du * ny) + h5 * (vav + cav * ny));
+  R rm3 = (h2 * (hav - cav * uv) + h3 * q2a + h4 * (uav * (ur - ul) + vav * (vr - vl) - uv * du) +
+           h5 * (hav + cav * uv));
+  out[0] = R(0.5) * (FR0 + FL0 - rm0);
+  out[1] = R(0.5) * (FR1 + FL1 - rm1);
+  out[2] = R(0.5) * (FR2 + FL2 - rm2);
+  out[3] = R(0.5) * (FR3 + FL3 - rm3);
+  // f_p (roe_solver.fypp:496-507); f_h is symmetric in its two pressures.
+  R fp;
+  if constexpr (DIR == 0) {
+    fp = roe_f_h(pn.p00, pn.p10);            // (i,j)   (i+1,j)
+    fp = fmax2(fp, roe_f_h(pn.p00, pn.p0p)); // (i,j)   (i,j+1)
+    fp = fmax2(fp, roe_f_h(pn.p00, pn.p0m)); // (i,j)   (i,j-1)
+    fp = fmax2(fp, roe_f_h(pn.p0p, pn.p1p)); // (i,j+1) (i+1,j+1)
+    fp = fmax2(fp, roe_f_h(pn.p0m, pn.p1m)); // (i,j-1) (i+1,j-1)
+  } else {
+    fp = roe_f_h(pn.p00, pn.p10);            // (i,j)   (i,j+1)
+    fp = fmax2(fp, roe_f_h(pn.p00, pn.p0m)); // (i,j)   (i-1,j)
+    fp = fmax2(fp, roe_f_h(pn.p00, pn.p0p)); // (i,j)   (i+1,j)
+    fp = fmax2(fp, roe_f_h(pn.p10, pn.p1m)); // (i,j+1) (i-1,j+1)
+    fp = fmax2(fp, roe_f_h(pn.p10, pn.p1p)); // (i,j+1) (i+1,j+1)
+  }
+  out[1] = out[1] - R(0.5) * fp * rav * cav * (-(du_trans * ny) + psi * du * nx);
+  out[2] = out[2] - R(0.5) * fp * rav * cav * (du_trans * nx + psi * du * ny);
+}
+
+// AUSM+ from interface_state (ausm_plus_solver.f90:499-643) with the AUSM+-u flags
+// (M_p = p_u = 0), beta = 1/8, alpha = 3/16, upwinded as in the commented solve (312-325).
+template <typename R, int DIR>
+__device__ inline void flux_ausm_plus(R gamma, const EdgeState<R> &s, R out[4]) {
+  const R n_x = DIR == 0 ? R(1) : R(0), n_y = DIR == 0 ? R(0) : R(1);
+  const R beta = R(1) / R(8), alpha = R(3) / R(16);
+  R rho_L = s.rL, u_L = s.uL, v_L = s.vL, p_L = s.pL;
+  R rho_R = s.rR, u_R = s.uR, v_R = s.vR, p_R = s.pR;
+  R gamma_factor = ((R(2) * (gamma - R(1))) / (gamma + R(1)));
+  R vel_L = u_L * n_x + v_L * n_y;
+  R vel_R = u_R * n_x + v_R * n_y;
+  R gg = gamma / (gamma - R(1));
+  R H_L = (p_L / rho_L) * gg + R(0.5) * (u_L * u_L + v_L * v_L);
+  R H_R = (p_R / rho_R) * gg + R(0.5) * (u_R * u_R + v_R * v_R);
+  R a_crit_L = sqrtr(gamma_factor * absd(H_L));
+  R a_crit_R = sqrtr(gamma_factor * absd(H_R));
+  R a_circ_L = sq(a_crit_L) / fmax2(a_crit_L, vel_L);
+  R a_circ_R = sq(a_crit_R) / fmax2(a_crit_R, -vel_R);
+  R a_half = fmin2(a_circ_L, a_circ_R);
+  R M_L = vel_L / a_half;
+  R M_R = vel_R / a_half;
+  R M2p_L = R(0.25) * sq(M_L + R(1)), M2m_L = -(R(0.25) * sq(M_L - R(1)));
+  R M2p_R = R(0.25) * sq(M_R + R(1)), M2m_R = -(R(0.25) * sq(M_R - R(1)));
+  R M_plus = (absd(M_L) >= R(1)) ? R(0.5) * (M_L + absd(M_L)) : M2p_L * (R(1) - R(16) * beta * M2m_L);
+  R M_minus = (absd(M_R) >= R(1)) ? R(0.5) * (M_R - absd(M_R)) : M2m_R * (R(1) + R(16) * beta * M2p_R);
+  R M_sum = M_plus + M_minus;
+  if (absd(M_sum) < R(1e-13)) M_sum = R(0);
+  R M_half = M_sum + R(0);
+  if (absd(M_half) < R(1e-13)) M_half = R(0);
+  R P_plus = (absd(M_L) >= R(1)) ? (R(0.5) * (M_L + absd(M_L))) / M_L
+                                  : M2p_L * ((R(2) - M_L) - R(16) * alpha * M_L * M2m_L);
+  R P_minus = (absd(M_R) >= R(1)) ? (R(0.5) * (M_R - absd(M_R))) / M_R
+                                   : M2m_R * ((R(-2) - M_R) + R(16) * alpha * M_R * M2p_R);
+  R p_half = P_plus * p_L + P_minus * p_R + R(0);
+  if (M_half > R(0)) {
+    R mass_flux = M_half * a_half * rho_L;
+    out[0] = mass_flux;
+    out[1] = mass_flux * u_L + (n_x * p_half);
+    out[2] = mass_flux * v_L + (n_y * p_half);
+    out[3] = mass_flux * H_L;
+  } else {
+    R mass_flux = M_half * a_half * rho_R;
+    out[0] = mass_flux;
+    out[1] = mass_flux * u_R + (n_x * p_half);
+    out[2] = mass_flux * v_R + (n_y * p_half);
+    out[3] = mass_flux * H_R;
+  }
+}
+
+// HLLC (Toro 2009, sec. 10.4-10.6): PVRS pressure, pressure-based wave speeds,
+// contact speed eq. 10.37, star states eq. 10.39.
+template <typename R, int DIR>
+__device__ inline void flux_hllc(R gamma, const EdgeState<R> &s, R out[4]) {
+  const R n_x = DIR == 0 ? R(1) : R(0), n_y = DIR == 0 ? R(0) : R(1);
+  R rL = s.rL, uL = s.uL, vL = s.vL, pL = s.pL;
+  R rR = s.rR, uR = s.uR, vR = s.vR, pR = s.pR;
+  R gm1 = gamma - R(1);
+  R qL = uL * n_x + vL * n_y;
+  R qR = uR * n_x + vR * n_y;
+  R cL = sqrtr(gamma * pL / rL);
+  R cR = sqrtr(gamma * pR / rR);
+  R rho_bar = R(0.5) * (rL + rR);
+  R cs_bar = R(0.5) * (cL + cR);
+  R p_pvrs = R(0.5) * (pL + pR) - R(0.5) * (qR - qL) * rho_bar * cs_bar;
+  R p_star = fmax2(R(0), p_pvrs);
+  R gfac = (gamma + R(1)) / (R(2) * gamma);
+  R wL = (p_star <= pL) ? R(1) : sqrtr(R(1) + gfac * (p_star / pL - R(1)));
+  R wR = (p_star <= pR) ? R(1) : sqrtr(R(1) + gfac * (p_star / pR - R(1)));
+  R SL = qL - cL * wL;
+  R SR = qR + cR * wR;
+  R mL = rL * (SL - qL);
+  R mR = rR * (SR - qR);
+  R S_star = (pR - pL + qL * mL - qR * mR) / (mL - mR);
+  R EL = pL / gm1 + R(0.5) * rL * (uL * uL + vL * vL);
+  R ER = pR / gm1 + R(0.5) * rR * (uR * uR + vR * vR);
+  if (R(0) <= SL) {
+    out[0] = rL * qL;
+    out[1] = rL * uL * qL + pL * n_x;
+    out[2] = rL * vL * qL + pL * n_y;
+    out[3] = qL * (EL + pL);
+  } else if (SR <= R(0)) {
+    out[0] = rR * qR;
+    out[1] = rR * uR * qR + pR * n_x;
+    out[2] = rR * vR * qR + pR * n_y;
+    out[3] = qR * (ER + pR);
+  } else if (S_star >= R(0)) {
+    R fac = mL / (SL - S_star);
+    R Us1 = fac * (uL + (S_star - qL) * n_x);
+    R Us2 = fac * (vL + (S_star - qL) * n_y);
+    R Us3 = fac * (EL / rL + (S_star - qL) * (S_star + pL / mL));
+    out[0] = rL * qL + SL * (fac - rL);
+    out[1] = (rL * uL * qL + pL * n_x) + SL * (Us1 - rL * uL);
+    out[2] = (rL * vL * qL + pL * n_y) + SL * (Us2 - rL * vL);
+    out[3] = qL * (EL + pL) + SL * (Us3 - EL);
+  } else {
+    R fac = mR / (SR - S_star);
+    R Us1 = fac * (uR + (S_star - qR) * n_x);
+    R Us2 = fac * (vR + (S_star - qR) * n_y);
+    R Us3 = fac * (ER / rR + (S_star - qR) * (S_star + pR / mR));
+    out[0] = rR * qR + SR * (fac - rR);
+    out[1] = (rR * uR * qR + pR * n_x) + SR * (Us1 - rR * uR);
+    out[2] = (rR * vR * qR + pR * n_y) + SR * (Us2 - rR * vR);
+    out[3] = qR * (ER + pR) + SR * (Us3 - ER);
+  }
+}
+
+template <typename R, int FLUX, int DIR>
+__device__ inline void edge_flux(R gamma, int low_mach_fix, const EdgeState<R> &s, const PressNbr<R> &pn, R out[4]) {
+  if constexpr (FLUX == FLUX_AUSMPW_PLUS) flux_ausmpw<R, DIR>(gamma, low_mach_fix, s, pn, out);
+  else if constexpr (FLUX == FLUX_SLAU || FLUX == FLUX_SLAU2 || FLUX == FLUX_SD_SLAU || FLUX == FLUX_SD_SLAU2)
+    flux_slau<R, FLUX, DIR>(gamma, s, pn, out);
+  else if constexpr (FLUX == FLUX_ROE) flux_roe<R, DIR>(gamma, s, pn, out);
+  else if constexpr (FLUX == FLUX_AUSM_PLUS) flux_ausm_plus<R, DIR>(gamma, s, out);
+  else flux_hllc<R, DIR>(gamma, s, out);
+}
+
+template <int FLUX> constexpr bool needs_pressure_nbrs() {
+  return FLUX == FLUX_AUSMPW_PLUS || FLUX == FLUX_SD_SLAU || FLUX == FLUX_SD_SLAU2 || FLUX == FLUX_ROE;
+}
+
+// ---------------------------------------------------------------------------
+// Divergence for one conserved variable of one cell: flux_split_edges
+// (flux_solver.f90:252-277 and 395-425).  l1..l4 = bottom/right/top/left lengths.
+// ---------------------------------------------------------------------------
+__device__ inline double divergence(double Fi, double Fim, double Gj, double Gjm, double l1, double l2, double l3,
+                                    double l4, double vol, bool vol_pow2, double inv_vol) {
+  const double REL_TOL = 1e-14;
+  const double NEAR_ZERO = 5.0 * Lim<double>::tiny;
+  double mi = fmax2(fmax2(absd(Fi), absd(Fim)), 1.0);
+  double mj = fmax2(fmax2(absd(Gj), absd(Gjm)), 1.0);
+  double ri, rj;
+  if (mi == 1.0) {
+    ri = (Fi * l2) + (-(Fim * l4));  // x/1 and x*1 are exact
+  } else {
+    ri = ((Fi / mi * l2) + (-(Fim / mi * l4))) * mi;
+  }
+  if (mj == 1.0) {
+    rj = (Gj * l3) + (-(Gjm * l1));
+  } else {
+    rj = ((Gj / mj * l3) + (-(Gjm / mj * l1))) * mj;
+  }
+  if (absd(ri) < mi * REL_TOL) ri = 0.0;
+  if (absd(rj) < mj * REL_TOL) rj = 0.0;
+  double fl = ri + rj;
+  if (absd(fl) < NEAR_ZERO) fl = 0.0;
+  double d = -fl;
+  // x / 2^k is exactly x * 2^-k: skip the IEEE division for power-of-two volumes
+  return vol_pow2 ? d * inv_vol : d / vol;
+}
+
+}  // namespace cato

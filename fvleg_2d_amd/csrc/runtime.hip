@@ -1,0 +1,812 @@
+// Host runtime of the CATO HIP library: geometry, device-resident state,
+// the stage sequences of the SSP/RK integrators, the device-side time loop,
+// operator-level entry points and the C ABI of include/cato_hip.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "aux_kernels.hpp"
+#include "runtime.hpp"
+#include "stage_dispatch.hpp"
+
+namespace cato {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int hip_check(hipError_t e, const char *what) {
+  if (e == hipSuccess) return CATO_OK;
+  return fail(CATO_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCK(expr)                                     \
+  do {                                                  \
+    int _rc = hip_check((expr), #expr);                 \
+    if (_rc != CATO_OK) return _rc;                     \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Geometry: quad_cell.f90:69-229 per cell; the HIP path accepts rectilinear
+// blocks (node_x depends on i only, node_y on j only), for which the per-cell
+// geometry is exactly reproduced from four 1D arrays.
+// ---------------------------------------------------------------------------
+static inline double hmax2(double a, double b) { return (b > a) ? b : a; }
+static inline double hmin2(double a, double b) { return (b < a) ? b : a; }
+
+int build_geometry(int ni, int nj, int nh, const double *nx, const double *ny, Geometry1D &g) {
+  const int nnx = ni + 2 * nh + 1, nny = nj + 2 * nh + 1;
+  auto NX = [&](int a, int b) { return nx[(size_t)a + (size_t)b * nnx]; };
+  auto NY = [&](int a, int b) { return ny[(size_t)a + (size_t)b * nnx]; };
+  for (int b = 0; b < nny; ++b)
+    for (int a = 0; a < nnx; ++a) {
+      if (NX(a, b) != NX(a, 0) || NY(a, b) != NY(0, b))
+        return fail(CATO_ERR_UNSUPPORTED,
+                    "grid is not rectilinear (node_x must depend on i only and node_y on j only)");
+    }
+  const int ncx = ni + 2 * nh, ncy = nj + 2 * nh;
+  g.dxi.resize(ncx);
+  g.lxi.resize(ncx);
+  g.dyj.resize(ncy);
+  g.lyj.resize(ncy);
+  for (int a = 0; a < ncx; ++a) {
+    double d = NX(a + 1, 0) - NX(a, 0);
+    g.dxi[a] = d;
+    g.lxi[a] = std::sqrt(d * d + 0.0 * 0.0);
+    if (!(d > 0.0)) return fail(CATO_ERR_INVALID, "non-positive cell width in x");
+  }
+  for (int b = 0; b < ncy; ++b) {
+    double d = NY(0, b + 1) - NY(0, b);
+    g.dyj[b] = d;
+    g.lyj[b] = std::sqrt(0.0 * 0.0 + d * d);
+    if (!(d > 0.0)) return fail(CATO_ERR_INVALID, "non-positive cell width in y");
+  }
+  // Cross-check the separable form against the full quad-cell algorithm on
+  // every cell (normals must be exactly axis aligned for the kernel's fluxes).
+  for (int b = 0; b < ncy; ++b)
+    for (int a = 0; a < ncx; ++a) {
+      double x[4] = {NX(a, b), NX(a + 1, b), NX(a + 1, b + 1), NX(a, b + 1)};
+      double y[4] = {NY(a, b), NY(a + 1, b), NY(a + 1, b + 1), NY(a, b + 1)};
+      double vol = (x[1] - x[0]) * (y[3] - y[0]) - (x[3] - x[0]) * (y[1] - y[0]);
+      if (vol != g.dxi[a] * g.dyj[b]) return fail(CATO_ERR_UNSUPPORTED, "cell volume is not separable");
+      const int head[4] = {1, 2, 3, 0};
+      for (int e = 1; e <= 2; ++e) {  // RIGHT and TOP edges carry the i and j fluxes
+        double mx = (x[head[e]] + x[e]) / 2.0, my = (y[head[e]] + y[e]) / 2.0;
+        double ddx = x[head[e]] - mx, ddy = y[head[e]] - my;
+        double vx = (mx + ddy) - mx, vy = (my - ddx) - my;
+        double len = std::sqrt(vx * vx + vy * vy);
+        double ex = (e == 1) ? 1.0 : 0.0, ey = (e == 1) ? 0.0 : 1.0;
+        if (vx / len != ex || vy / len != ey)
+          return fail(CATO_ERR_UNSUPPORTED, "edge normals are not axis aligned on this grid");
+      }
+    }
+  return CATO_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+static int validate_config(const cato_config &c) {
+  if (c.flux < 0 || c.flux >= FLUX_COUNT) return fail(CATO_ERR_INVALID, "unknown flux solver");
+  if (c.limiter < 0 || c.limiter >= LIM_COUNT) return fail(CATO_ERR_INVALID, "unknown limiter");
+  if (c.integrator < CATO_RK_SSP2 || c.integrator > CATO_RK_2_5)
+    return fail(CATO_ERR_INVALID, "unknown time integration scheme (ssp_rk43 has no body in the reference)");
+  if (c.precision != CATO_PRECISION_FP64 && c.precision != CATO_PRECISION_FP32_FLUX)
+    return fail(CATO_ERR_INVALID, "unknown precision");
+  int need = (c.limiter == LIM_MLP5) ? 3 : 2;
+  if (c.n_ghost_layers != need)
+    return fail(CATO_ERR_INVALID,
+                "The number of required ghost cell layers doesn't match the edge interpolation order");
+  const int bcs[4] = {c.bc_plus_x, c.bc_minus_x, c.bc_plus_y, c.bc_minus_y};
+  for (int k = 0; k < 4; ++k)
+    if (bcs[k] < 0 || bcs[k] > 2) return fail(CATO_ERR_INVALID, "unsupported boundary condition type");
+  if ((c.bc_plus_x == BC_PERIODIC) != (c.bc_minus_x == BC_PERIODIC) ||
+      (c.bc_plus_y == BC_PERIODIC) != (c.bc_minus_y == BC_PERIODIC))
+    return fail(CATO_ERR_UNSUPPORTED, "a periodic boundary must be periodic on both sides of its axis");
+  if (!(c.gamma > 1.0)) return fail(CATO_ERR_INVALID, "polytropic_index must be > 1");
+  if (!(c.cfl > 0.0)) return fail(CATO_ERR_INVALID, "cfl must be > 0");
+  if (c.tile >= kNumTiles) return fail(CATO_ERR_INVALID, "unknown tile id");
+  return CATO_OK;
+}
+
+int default_tile(const cato_config &c, int ni, int nj) {
+  if (c.tile >= 0) return c.tile;
+  (void)ni;
+  (void)nj;
+  return 0;  // 32 x 8
+}
+
+int Solve::init(const cato_config &c, const cato_geom &g) {
+  int rc = validate_config(c);
+  if (rc) return rc;
+  if (g.ni < c.n_ghost_layers || g.nj < c.n_ghost_layers)
+    return fail(CATO_ERR_INVALID, "grid must have at least n_ghost_layers interior cells per direction");
+  if (!g.node_x || !g.node_y) return fail(CATO_ERR_INVALID, "null node coordinates");
+  cfg = c;
+  ni = g.ni;
+  nj = g.nj;
+  nh = c.n_ghost_layers;
+  pitch = (ni + 15) & ~15;
+  rc = build_geometry(ni, nj, nh, g.node_x, g.node_y, geo);
+  if (rc) return rc;
+  tile = default_tile(c, ni, nj);
+  tiles_x = (ni + kTiles[tile].tx - 1) / kTiles[tile].tx;
+  tiles_y = (nj + kTiles[tile].ty - 1) / kTiles[tile].ty;
+  launcher = stage_launcher(c.flux, c.limiter, c.precision, tile);
+  if (!launcher) return fail(CATO_ERR_INVALID, "no stage kernel for this configuration");
+  return CATO_OK;
+}
+
+size_t Solve::field_elems() const { return (size_t)pitch * nj; }
+
+size_t Solve::device_bytes() const {
+  size_t f = field_elems();
+  size_t nb = (size_t)tiles_x * tiles_y;
+  size_t geo_n = (size_t)2 * (ni + 2 * nh) + 2 * (nj + 2 * nh);
+  return (12 + 4) * f * sizeof(double) + geo_n * sizeof(double) + nb * (sizeof(double) + sizeof(int)) +
+         sizeof(LoopState) + 4 * sizeof(double) + 1024;
+}
+
+// Carve the device arrays of one solve out of `base` (device_bytes() long).
+void Solve::bind(char *base) {
+  size_t f = field_elems();
+  double *p = reinterpret_cast<double *>(base);
+  for (int s = 0; s < 3; ++s)
+    for (int v = 0; v < 4; ++v) {
+      U[s][v] = p;
+      p += f;
+    }
+  for (int v = 0; v < 4; ++v) {
+    prim[v] = p;
+    p += f;
+  }
+  dxi = p; p += ni + 2 * nh;
+  lxi = p; p += ni + 2 * nh;
+  dyj = p; p += nj + 2 * nh;
+  lyj = p; p += nj + 2 * nh;
+  block_min = p; p += (size_t)tiles_x * tiles_y;
+  result = p; p += 4;
+  char *c = reinterpret_cast<char *>(p);
+  c = reinterpret_cast<char *>(((uintptr_t)c + 255) & ~(uintptr_t)255);
+  loop = reinterpret_cast<LoopState *>(c);
+  c += sizeof(LoopState);
+  block_err = reinterpret_cast<int *>(c);
+}
+
+int Solve::upload_geometry(hipStream_t s) {
+  HIPCK(hipMemcpyAsync(dxi, geo.dxi.data(), geo.dxi.size() * 8, hipMemcpyHostToDevice, s));
+  HIPCK(hipMemcpyAsync(lxi, geo.lxi.data(), geo.lxi.size() * 8, hipMemcpyHostToDevice, s));
+  HIPCK(hipMemcpyAsync(dyj, geo.dyj.data(), geo.dyj.size() * 8, hipMemcpyHostToDevice, s));
+  HIPCK(hipMemcpyAsync(lyj, geo.lyj.data(), geo.lyj.size() * 8, hipMemcpyHostToDevice, s));
+  HIPCK(hipMemsetAsync(block_err, 0, (size_t)tiles_x * tiles_y * sizeof(int), s));
+  LoopState L{};
+  HIPCK(hipMemcpyAsync(loop, &L, sizeof(L), hipMemcpyHostToDevice, s));
+  return CATO_OK;
+}
+
+FieldGeom Solve::fgeom() const {
+  FieldGeom g;
+  g.ni = ni;
+  g.nj = nj;
+  g.nh = nh;
+  g.pitch = pitch;
+  g.bc[0] = cfg.bc_plus_x;
+  g.bc[1] = cfg.bc_minus_x;
+  g.bc[2] = cfg.bc_plus_y;
+  g.bc[3] = cfg.bc_minus_y;
+  g.gamma = cfg.gamma;
+  g.cfl = cfg.cfl;
+  g.dxi = dxi;
+  g.dyj = dyj;
+  return g;
+}
+
+StageArgs Solve::stage_args(int mode, double a, double b, double c, int in_buf, int u0_buf, int out_buf, bool last,
+                            const LoopState *lp, double dt_value) const {
+  StageArgs s{};
+  s.ni = ni;
+  s.nj = nj;
+  s.nh = nh;
+  s.pitch = pitch;
+  s.mode = mode;
+  s.prim_from_array = (in_buf < 0) ? 1 : 0;
+  s.last_stage = last ? 1 : 0;
+  s.low_mach_fix = cfg.apply_low_mach_fix;
+  s.bc[0] = cfg.bc_plus_x;
+  s.bc[1] = cfg.bc_minus_x;
+  s.bc[2] = cfg.bc_plus_y;
+  s.bc[3] = cfg.bc_minus_y;
+  s.tiles_x = tiles_x;
+  s.tiles_y = tiles_y;
+  s.gamma = cfg.gamma;
+  s.cfl = cfg.cfl;
+  s.a = a;
+  s.b = b;
+  s.c = c;
+  for (int v = 0; v < 4; ++v) {
+    s.uk[v] = in_buf >= 0 ? U[in_buf][v] : nullptr;
+    s.u0[v] = u0_buf >= 0 ? U[u0_buf][v] : nullptr;
+    s.prim[v] = prim[v];
+    s.out[v] = out_buf >= 0 ? U[out_buf][v] : prim[v];
+  }
+  s.dxi = dxi;
+  s.lxi = lxi;
+  s.dyj = dyj;
+  s.lyj = lyj;
+  s.loop = lp;
+  s.dt_value = dt_value;
+  s.block_min = block_min;
+  s.block_err = block_err;
+  return s;
+}
+
+// Stage sequence of one step: ssp_rk_2_2 (mod_fluid.f90:1115-1129),
+// ssp_rk_3_3 (773-802) and rk_multistage (957-966).  Buffers: cur holds U^n,
+// the two others hold intermediate stages; the last stage overwrites U^n in
+// place (it reads U^n only at its own cell).
+std::vector<StageArgs> Solve::step_stages(const LoopState *lp, double dt_value) const {
+  std::vector<StageArgs> st;
+  const int n = cur, A = (cur + 1) % 3, B = (cur + 2) % 3;
+  const int in0 = prims_valid ? -1 : n;
+  switch (cfg.integrator) {
+    case CATO_RK_SSP2:
+      st.push_back(stage_args(MODE_FIRST, 0, 0, 1.0, in0, -1, A, false, lp, dt_value));
+      st.push_back(stage_args(MODE_COMBINE, 0.5, 0.5, 0.5, A, n, n, true, lp, dt_value));
+      break;
+    case CATO_RK_SSP3:
+      st.push_back(stage_args(MODE_FIRST, 0, 0, 1.0, in0, -1, A, false, lp, dt_value));
+      st.push_back(stage_args(MODE_COMBINE, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0, A, n, B, false, lp, dt_value));
+      st.push_back(stage_args(MODE_COMBINE, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, B, n, n, true, lp, dt_value));
+      break;
+    default: {
+      static const double a13[3] = {0.1481, 0.4, 1.0}, a14[4] = {0.0833, 0.2069, 0.4265, 1.0},
+                          a15[5] = {0.0533, 0.1263, 0.2375, 0.4414, 1.0}, a23[3] = {0.1918, 0.4929, 1.0},
+                          a24[4] = {0.1084, 0.2602, 0.5052, 1.0}, a25[5] = {0.0695, 0.1602, 0.2898, 0.5060, 1.0};
+      const double *al;
+      int ns;
+      switch (cfg.integrator) {
+        case CATO_RK_1_3: al = a13; ns = 3; break;
+        case CATO_RK_1_4: al = a14; ns = 4; break;
+        case CATO_RK_1_5: al = a15; ns = 5; break;
+        case CATO_RK_2_3: al = a23; ns = 3; break;
+        case CATO_RK_2_4: al = a24; ns = 4; break;
+        default: al = a25; ns = 5; break;
+      }
+      int prev = in0;
+      for (int m = 0; m < ns; ++m) {
+        bool last = (m == ns - 1);
+        int out = last ? n : ((m % 2 == 0) ? A : B);
+        // first stage of the first step reads the stored primitives; U_0 is U^n either way
+        st.push_back(stage_args(MODE_BASE, 0, 0, al[m], prev, n, out, last, lp, dt_value));
+        prev = out;
+      }
+    }
+  }
+  return st;
+}
+
+}  // namespace cato
+
+using namespace cato;
+
+struct cato_ctx {
+  Solve s;
+  hipStream_t stream = nullptr;
+  char *mem = nullptr;
+  int device = 0;
+  bool timing = false;
+  double stage_ms = 0.0, other_ms = 0.0;
+  long long stage_n = 0, other_n = 0;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<std::pair<hipEvent_t, hipEvent_t>, int>> pending;  // (start, stop), is_stage
+};
+
+namespace {
+
+hipEvent_t take_event(cato_ctx *c) {
+  if (!c->ev_pool.empty()) {
+    hipEvent_t e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+void collect_timing(cato_ctx *c) {
+  for (auto &p : c->pending) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.first.first, p.first.second) == hipSuccess) {
+      if (p.second) {
+        c->stage_ms += ms;
+        c->stage_n++;
+      } else {
+        c->other_ms += ms;
+        c->other_n++;
+      }
+    }
+    c->ev_pool.push_back(p.first.first);
+    c->ev_pool.push_back(p.first.second);
+  }
+  c->pending.clear();
+}
+
+template <typename F> int timed(cato_ctx *c, bool is_stage, F &&launch) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (c->timing) {
+    a = take_event(c);
+    b = take_event(c);
+    if (a && b) (void)hipEventRecord(a, c->stream);
+  }
+  launch();
+  int rc = hip_check(hipGetLastError(), "kernel launch");
+  if (c->timing && a && b) {
+    (void)hipEventRecord(b, c->stream);
+    c->pending.push_back({{a, b}, is_stage ? 1 : 0});
+    if (c->pending.size() > 4096) {
+      (void)hipEventSynchronize(b);
+      collect_timing(c);
+    }
+  }
+  return rc;
+}
+
+int launch_stages(cato_ctx *c, const std::vector<StageArgs> &st) {
+  Solve &s = c->s;
+  dim3 grid((unsigned)(s.tiles_x * s.tiles_y));
+  for (const StageArgs &a : st) {
+    int rc = timed(c, true, [&] { s.launcher(grid, c->stream, a); });
+    if (rc) return rc;
+  }
+  return CATO_OK;
+}
+
+int launch_reduce(cato_ctx *c, int mode) {
+  Solve &s = c->s;
+  int nb = s.tiles_x * s.tiles_y;
+  return timed(c, false, [&] {
+    hipLaunchKernelGGL(k_reduce_finalize, dim3(1), dim3(1024), 0, c->stream, nb, s.block_min, s.block_err, s.result,
+                       s.loop, mode);
+  });
+}
+
+// Current CFL minimum and sanity flags of the stored state into result[0..1].
+int launch_cfl(cato_ctx *c, int loop_mode) {
+  Solve &s = c->s;
+  FieldGeom g = s.fgeom();
+  int nb = s.tiles_x * s.tiles_y;
+  const double *a0, *a1, *a2, *a3;
+  if (s.prims_valid) {
+    a0 = s.prim[0]; a1 = s.prim[1]; a2 = s.prim[2]; a3 = s.prim[3];
+  } else {
+    a0 = s.U[s.cur][0]; a1 = s.U[s.cur][1]; a2 = s.U[s.cur][2]; a3 = s.U[s.cur][3];
+  }
+  int from_prims = s.prims_valid ? 1 : 0;
+  int rc = timed(c, false, [&] {
+    hipLaunchKernelGGL(k_cfl_partial, dim3((unsigned)nb), dim3(kThreads), 0, c->stream, g, from_prims, a0, a1, a2, a3,
+                       s.block_min, s.block_err);
+  });
+  if (rc) return rc;
+  return launch_reduce(c, loop_mode);
+}
+
+int check_ctx(cato_ctx *c) { return c ? CATO_OK : fail(CATO_ERR_INVALID, "null context"); }
+
+// Host <-> device copies of interior fields with (ld, halo) host layout.
+int upload_field(cato_ctx *c, double *dst, const double *src, int ld, int halo) {
+  const Solve &s = c->s;
+  if (!src) return fail(CATO_ERR_INVALID, "null field pointer");
+  if (halo < 0 || ld < s.ni + 2 * halo) return fail(CATO_ERR_INVALID, "bad leading dimension / halo");
+  const double *p = src + (size_t)halo + (size_t)halo * ld;
+  return hip_check(hipMemcpy2DAsync(dst, (size_t)s.pitch * 8, p, (size_t)ld * 8, (size_t)s.ni * 8, (size_t)s.nj,
+                                    hipMemcpyHostToDevice, c->stream),
+                   "upload");
+}
+int download_field(cato_ctx *c, double *dst, const double *src, int ld, int halo) {
+  const Solve &s = c->s;
+  if (!dst) return CATO_OK;
+  if (halo < 0 || ld < s.ni + 2 * halo) return fail(CATO_ERR_INVALID, "bad leading dimension / halo");
+  double *p = dst + (size_t)halo + (size_t)halo * ld;
+  return hip_check(hipMemcpy2DAsync(p, (size_t)ld * 8, src, (size_t)s.pitch * 8, (size_t)s.ni * 8, (size_t)s.nj,
+                                    hipMemcpyDeviceToHost, c->stream),
+                   "download");
+}
+
+dim3 field_grid(const Solve &s) { return dim3((unsigned)((s.ni + 255) / 256), (unsigned)s.nj); }
+
+template <int LIMC> void launch_interp(dim3 g, hipStream_t st, int prec, FieldGeom fg, int ld, const double *q,
+                                              double *ie, double *je) {
+  if (prec == CATO_PRECISION_FP32_FLUX)
+    hipLaunchKernelGGL((k_interpolate<float, LIMC>), g, dim3(256), 0, st, fg, ld, q, ie, je);
+  else
+    hipLaunchKernelGGL((k_interpolate<double, LIMC>), g, dim3(256), 0, st, fg, ld, q, ie, je);
+}
+
+template <int F> void launch_flux(dim3 g, hipStream_t st, int prec, FieldGeom fg, int ld, const double *e,
+                                         const double *p, int lm, double *fi, double *fj) {
+  if (prec == CATO_PRECISION_FP32_FLUX)
+    hipLaunchKernelGGL((k_edge_flux<float, F>), g, dim3(256), 0, st, fg, ld, e, p, lm, fi, fj);
+  else
+    hipLaunchKernelGGL((k_edge_flux<double, F>), g, dim3(256), 0, st, fg, ld, e, p, lm, fi, fj);
+}
+
+
+}  // namespace
+
+extern "C" {
+
+const char *cato_last_error(void) { return g_last_error.c_str(); }
+int cato_abi_version(void) { return CATO_HIP_ABI_VERSION; }
+int cato_num_tiles(void) { return kNumTiles; }
+int cato_tile_shape(int tile, int *tx, int *ty) {
+  if (tile < 0 || tile >= kNumTiles) return fail(CATO_ERR_INVALID, "unknown tile id");
+  if (tx) *tx = kTiles[tile].tx;
+  if (ty) *ty = kTiles[tile].ty;
+  return CATO_OK;
+}
+
+int cato_ctx_create(const cato_config *cfg, const cato_geom *geom, cato_ctx **out) {
+  if (!cfg || !geom || !out) return fail(CATO_ERR_INVALID, "null argument");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(CATO_ERR_NO_DEVICE, "no HIP device");
+  cato_ctx *c = new cato_ctx();
+  int rc = c->s.init(*cfg, *geom);
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  (void)hipGetDevice(&c->device);
+  if ((rc = hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream"))) {
+    delete c;
+    return rc;
+  }
+  size_t bytes = c->s.device_bytes();
+  if ((rc = hip_check(hipMalloc(&c->mem, bytes), "device allocation"))) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return rc;
+  }
+  c->s.bind(c->mem);
+  if ((rc = c->s.upload_geometry(c->stream)) || (rc = hip_check(hipStreamSynchronize(c->stream), "sync"))) {
+    cato_ctx_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return CATO_OK;
+}
+
+int cato_ctx_destroy(cato_ctx *c) {
+  if (!c) return CATO_OK;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  for (auto &p : c->pending) {
+    (void)hipEventDestroy(p.first.first);
+    (void)hipEventDestroy(p.first.second);
+  }
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->mem) (void)hipFree(c->mem);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return CATO_OK;
+}
+
+int cato_set_primitive(cato_ctx *c, const double *rho, const double *u, const double *v, const double *p, int ld,
+                       int halo) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  Solve &s = c->s;
+  if ((rc = upload_field(c, s.prim[0], rho, ld, halo)) || (rc = upload_field(c, s.prim[1], u, ld, halo)) ||
+      (rc = upload_field(c, s.prim[2], v, ld, halo)) || (rc = upload_field(c, s.prim[3], p, ld, halo)))
+    return rc;
+  FieldGeom g = s.fgeom();
+  hipLaunchKernelGGL(k_prim_to_cons, field_grid(s), dim3(256), 0, c->stream, g, s.prim[0], s.prim[1], s.prim[2],
+                     s.prim[3], s.U[s.cur][0], s.U[s.cur][1], s.U[s.cur][2], s.U[s.cur][3]);
+  HIPCK(hipGetLastError());
+  s.prims_valid = true;
+  return hip_check(hipStreamSynchronize(c->stream), "set_primitive");
+}
+
+int cato_set_conserved(cato_ctx *c, const double *r, const double *ru, const double *rv, const double *rE, int ld,
+                       int halo) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  Solve &s = c->s;
+  double **U = s.U[s.cur];
+  if ((rc = upload_field(c, U[0], r, ld, halo)) || (rc = upload_field(c, U[1], ru, ld, halo)) ||
+      (rc = upload_field(c, U[2], rv, ld, halo)) || (rc = upload_field(c, U[3], rE, ld, halo)))
+    return rc;
+  s.prims_valid = false;
+  return hip_check(hipStreamSynchronize(c->stream), "set_conserved");
+}
+
+int cato_get_conserved(cato_ctx *c, double *r, double *ru, double *rv, double *rE, int ld, int halo) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  Solve &s = c->s;
+  double **U = s.U[s.cur];
+  if ((rc = download_field(c, r, U[0], ld, halo)) || (rc = download_field(c, ru, U[1], ld, halo)) ||
+      (rc = download_field(c, rv, U[2], ld, halo)) || (rc = download_field(c, rE, U[3], ld, halo)))
+    return rc;
+  return hip_check(hipStreamSynchronize(c->stream), "get_conserved");
+}
+
+int cato_get_primitive(cato_ctx *c, double *rho, double *u, double *v, double *p, double *cs, int ld, int halo) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  Solve &s = c->s;
+  FieldGeom g = s.fgeom();
+  // scratch: the two non-current state buffers
+  double **T = s.U[(s.cur + 1) % 3];
+  double **T2 = s.U[(s.cur + 2) % 3];
+  if (s.prims_valid) {
+    hipLaunchKernelGGL(k_sound_speed, field_grid(s), dim3(256), 0, c->stream, g, s.prim[0], s.prim[3], T2[0]);
+    HIPCK(hipGetLastError());
+    if ((rc = download_field(c, rho, s.prim[0], ld, halo)) || (rc = download_field(c, u, s.prim[1], ld, halo)) ||
+        (rc = download_field(c, v, s.prim[2], ld, halo)) || (rc = download_field(c, p, s.prim[3], ld, halo)) ||
+        (rc = download_field(c, cs, T2[0], ld, halo)))
+      return rc;
+  } else {
+    double **U = s.U[s.cur];
+    hipLaunchKernelGGL(k_derived, field_grid(s), dim3(256), 0, c->stream, g, U[0], U[1], U[2], U[3], T[0], T[1], T[2],
+                       T[3]);
+    HIPCK(hipGetLastError());
+    if ((rc = download_field(c, rho, U[0], ld, halo)) || (rc = download_field(c, u, T[0], ld, halo)) ||
+        (rc = download_field(c, v, T[1], ld, halo)) || (rc = download_field(c, p, T[2], ld, halo)) ||
+        (rc = download_field(c, cs, T[3], ld, halo)))
+      return rc;
+  }
+  return hip_check(hipStreamSynchronize(c->stream), "get_primitive");
+}
+
+int cato_compute_dt(cato_ctx *c, double *dt) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if ((rc = launch_cfl(c, 0))) return rc;
+  double r[2];
+  HIPCK(hipMemcpyAsync(r, c->s.result, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  if (dt) *dt = r[0];
+  return CATO_OK;
+}
+
+int cato_step(cato_ctx *c, double dt, int *error_code) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  Solve &s = c->s;
+  if (!(dt >= 2.2250738585072014e-308))
+    return fail(CATO_ERR_INVALID, "The timestep dt is < tiny(1.0_rk)");  // slau_solver.fypp:162-166
+  if ((rc = launch_stages(c, s.step_stages(nullptr, dt)))) return rc;
+  s.prims_valid = false;
+  if ((rc = launch_reduce(c, 0))) return rc;
+  double r[2];
+  HIPCK(hipMemcpyAsync(r, s.result, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  if (c->timing) collect_timing(c);
+  if (error_code) {
+    int f = (int)r[1];
+    *error_code = (f & 1) ? 3 : (f & 2) ? 1 : (f & 4) ? 2 : 0;
+  }
+  return CATO_OK;
+}
+
+static int run_impl(cato_ctx *c, double max_time, long long max_iter, double contour_dt, double constant_dt,
+                    bool poll, long long fixed_steps, long long *iters, double *time, int *err) {
+  Solve &s = c->s;
+  LoopState L{};
+  L.time = 0.0;
+  L.master_time = 0.0;
+  L.next_output_time = 0.0 + contour_dt;
+  L.contour_dt = contour_dt;
+  L.max_time = max_time;
+  L.constant_dt = constant_dt;
+  L.iteration = 0;
+  L.max_iterations = max_iter;
+  HIPCK(hipMemcpyAsync(s.loop, &L, sizeof(L), hipMemcpyHostToDevice, c->stream));
+  int rc = launch_cfl(c, 1);
+  if (rc) return rc;
+  long long launched = 0;
+  const long long chunk = 32;
+  LoopState H{};
+  while (true) {
+    long long n = poll ? chunk : fixed_steps;
+    for (long long k = 0; k < n; ++k) {
+      if ((rc = launch_stages(c, s.step_stages(s.loop, 0.0)))) return rc;
+      s.prims_valid = false;
+      if ((rc = launch_reduce(c, 2))) return rc;
+    }
+    launched += n;
+    if (!poll) break;
+    HIPCK(hipMemcpyAsync(&H, s.loop, sizeof(H), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    if (c->timing) collect_timing(c);
+    if (H.done) break;
+  }
+  if (!poll) return CATO_OK;
+  if (iters) *iters = H.iteration;
+  if (time) *time = H.time;
+  if (err) *err = H.error;
+  return CATO_OK;
+}
+
+int cato_run(cato_ctx *c, double max_time, long long max_iterations, double contour_interval_dt, double constant_dt,
+             long long *iterations, double *time, int *error_code) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (max_iterations <= 0) max_iterations = 0x7fffffffLL;
+  if (!(contour_interval_dt > 0.0)) contour_interval_dt = max_time;
+  return run_impl(c, max_time, max_iterations, contour_interval_dt, constant_dt, true, 0, iterations, time,
+                  error_code);
+}
+
+int cato_advance(cato_ctx *c, long long n) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (n <= 0) return CATO_OK;
+  return run_impl(c, 1e300, n, 1e300, 0.0, false, n, nullptr, nullptr, nullptr);
+}
+
+int cato_synchronize(cato_ctx *c) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  HIPCK(hipStreamSynchronize(c->stream));
+  if (c->timing) collect_timing(c);
+  return CATO_OK;
+}
+
+int cato_set_timing(cato_ctx *c, int enable) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  HIPCK(hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  c->timing = enable != 0;
+  c->stage_ms = c->other_ms = 0.0;
+  c->stage_n = c->other_n = 0;
+  return CATO_OK;
+}
+
+int cato_get_timing(cato_ctx *c, double *stage_ms, long long *stage_n, double *other_ms, long long *other_n) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  HIPCK(hipStreamSynchronize(c->stream));
+  collect_timing(c);
+  if (stage_ms) *stage_ms = c->stage_ms;
+  if (stage_n) *stage_n = c->stage_n;
+  if (other_ms) *other_ms = c->other_ms;
+  if (other_n) *other_n = c->other_n;
+  return CATO_OK;
+}
+
+// ---- operator-level entry points -------------------------------------------
+namespace {
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+}  // namespace
+
+int cato_apply_bc(cato_ctx *c, double *rho, double *u, double *v, double *p, int ld) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  Solve &s = c->s;
+  if (!rho || !u || !v || !p) return fail(CATO_ERR_INVALID, "null field pointer");
+  int nx = s.ni + 2 * s.nh, ny = s.nj + 2 * s.nh;
+  if (ld < nx) return fail(CATO_ERR_INVALID, "ld < ni + 2*nh");
+  size_t n = (size_t)ld * ny;
+  DevBuf d;
+  HIPCK(hipMalloc(&d.p, 4 * n * 8));
+  double *D = (double *)d.p;
+  double *H[4] = {rho, u, v, p};
+  for (int k = 0; k < 4; ++k) HIPCK(hipMemcpyAsync(D + k * n, H[k], n * 8, hipMemcpyHostToDevice, c->stream));
+  FieldGeom g = s.fgeom();
+  hipLaunchKernelGGL(k_apply_bc, dim3((unsigned)((nx * ny + 255) / 256)), dim3(256), 0, c->stream, g, ld, D, D + n,
+                     D + 2 * n, D + 3 * n);
+  HIPCK(hipGetLastError());
+  for (int k = 0; k < 4; ++k) HIPCK(hipMemcpyAsync(H[k], D + k * n, n * 8, hipMemcpyDeviceToHost, c->stream));
+  return hip_check(hipStreamSynchronize(c->stream), "apply_bc");
+}
+
+int cato_interpolate_edges(cato_ctx *c, const double *q, int ld, double *i_edges, double *j_edges) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  Solve &s = c->s;
+  if (!q || !i_edges || !j_edges) return fail(CATO_ERR_INVALID, "null pointer");
+  int nx = s.ni + 2 * s.nh, ny = s.nj + 2 * s.nh;
+  if (ld < nx) return fail(CATO_ERR_INVALID, "ld < ni + 2*nh");
+  size_t n = (size_t)ld * ny, ne = (size_t)2 * (s.ni + 1) * (s.nj + 1);
+  DevBuf d;
+  HIPCK(hipMalloc(&d.p, (n + 2 * ne) * 8));
+  double *Q = (double *)d.p, *IE = Q + n, *JE = IE + ne;
+  HIPCK(hipMemcpyAsync(Q, q, n * 8, hipMemcpyHostToDevice, c->stream));
+  FieldGeom g = s.fgeom();
+  dim3 grid((unsigned)((ne + 255) / 256));  // 2 * (ni+1)*(nj+1) threads
+  switch (s.cfg.limiter) {
+    case LIM_MINMOD: launch_interp<LIM_MINMOD>(grid, c->stream, s.cfg.precision, g, ld, Q, IE, JE); break;
+    case LIM_SUPERBEE: launch_interp<LIM_SUPERBEE>(grid, c->stream, s.cfg.precision, g, ld, Q, IE, JE); break;
+    case LIM_VAN_LEER: launch_interp<LIM_VAN_LEER>(grid, c->stream, s.cfg.precision, g, ld, Q, IE, JE); break;
+    case LIM_MLP3: launch_interp<LIM_MLP3>(grid, c->stream, s.cfg.precision, g, ld, Q, IE, JE); break;
+    default: launch_interp<LIM_MLP5>(grid, c->stream, s.cfg.precision, g, ld, Q, IE, JE); break;
+  }
+  HIPCK(hipGetLastError());
+  HIPCK(hipMemcpyAsync(i_edges, IE, ne * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipMemcpyAsync(j_edges, JE, ne * 8, hipMemcpyDeviceToHost, c->stream));
+  return hip_check(hipStreamSynchronize(c->stream), "interpolate_edges");
+}
+
+int cato_edge_fluxes(cato_ctx *c, const double *edges, const double *p, int ld, double *iflux, double *jflux) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  Solve &s = c->s;
+  if (!edges || !p || !iflux || !jflux) return fail(CATO_ERR_INVALID, "null pointer");
+  int nx = s.ni + 2 * s.nh, ny = s.nj + 2 * s.nh;
+  if (ld < nx) return fail(CATO_ERR_INVALID, "ld < ni + 2*nh");
+  size_t n = (size_t)ld * ny, ne = (size_t)2 * (s.ni + 1) * (s.nj + 1);
+  size_t nfi = (size_t)4 * (s.ni + 1) * s.nj, nfj = (size_t)4 * s.ni * (s.nj + 1);
+  DevBuf d;
+  HIPCK(hipMalloc(&d.p, (8 * ne + n + nfi + nfj) * 8));
+  double *E = (double *)d.p, *P = E + 8 * ne, *FI = P + n, *FJ = FI + nfi;
+  HIPCK(hipMemcpyAsync(E, edges, 8 * ne * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCK(hipMemcpyAsync(P, p, n * 8, hipMemcpyHostToDevice, c->stream));
+  FieldGeom g = s.fgeom();
+  int nt = (s.ni + 1) * s.nj + s.ni * (s.nj + 1);
+  dim3 grid((unsigned)((nt + 255) / 256));
+  int lm = s.cfg.apply_low_mach_fix, pr = s.cfg.precision;
+  switch (s.cfg.flux) {
+    case FLUX_AUSMPW_PLUS: launch_flux<FLUX_AUSMPW_PLUS>(grid, c->stream, pr, g, ld, E, P, lm, FI, FJ); break;
+    case FLUX_SLAU: launch_flux<FLUX_SLAU>(grid, c->stream, pr, g, ld, E, P, lm, FI, FJ); break;
+    case FLUX_SLAU2: launch_flux<FLUX_SLAU2>(grid, c->stream, pr, g, ld, E, P, lm, FI, FJ); break;
+    case FLUX_SD_SLAU: launch_flux<FLUX_SD_SLAU>(grid, c->stream, pr, g, ld, E, P, lm, FI, FJ); break;
+    case FLUX_SD_SLAU2: launch_flux<FLUX_SD_SLAU2>(grid, c->stream, pr, g, ld, E, P, lm, FI, FJ); break;
+    case FLUX_ROE: launch_flux<FLUX_ROE>(grid, c->stream, pr, g, ld, E, P, lm, FI, FJ); break;
+    case FLUX_AUSM_PLUS: launch_flux<FLUX_AUSM_PLUS>(grid, c->stream, pr, g, ld, E, P, lm, FI, FJ); break;
+    default: launch_flux<FLUX_HLLC>(grid, c->stream, pr, g, ld, E, P, lm, FI, FJ); break;
+  }
+  HIPCK(hipGetLastError());
+  HIPCK(hipMemcpyAsync(iflux, FI, nfi * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipMemcpyAsync(jflux, FJ, nfj * 8, hipMemcpyDeviceToHost, c->stream));
+  return hip_check(hipStreamSynchronize(c->stream), "edge_fluxes");
+}
+
+int cato_residual(cato_ctx *c, double *rho, double *u, double *v, double *p, double *d_rho, double *d_rho_u,
+                  double *d_rho_v, double *d_rho_E, int ld) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  Solve &s = c->s;
+  double *D[4] = {d_rho, d_rho_u, d_rho_v, d_rho_E};
+  for (int k = 0; k < 4; ++k)
+    if (!D[k]) return fail(CATO_ERR_INVALID, "null pointer");
+  // 1. boundary fill of the caller's halo'd primitives (intent(inout) in the reference)
+  if ((rc = cato_apply_bc(c, rho, u, v, p, ld))) return rc;
+  // 2. residual from the interior primitives (ghosts resolved in-kernel by the same BC map);
+  //    scratch: the two state buffers that do not hold U^n, so a pending initial state survives
+  double **Pbuf = s.U[(s.cur + 1) % 3];
+  double **Obuf = s.U[(s.cur + 2) % 3];
+  const double *P[4] = {rho, u, v, p};
+  for (int k = 0; k < 4; ++k)
+    if ((rc = upload_field(c, Pbuf[k], P[k], ld, s.nh))) return rc;
+  StageArgs a = s.stage_args(MODE_RESIDUAL, 0, 0, 0, -1, -1, (s.cur + 2) % 3, false, nullptr, 1.0);
+  for (int k = 0; k < 4; ++k) a.prim[k] = Pbuf[k];
+  a.block_min = nullptr;
+  a.block_err = nullptr;
+  dim3 grid((unsigned)(s.tiles_x * s.tiles_y));
+  s.launcher(grid, c->stream, a);
+  HIPCK(hipGetLastError());
+  int ny = s.nj + 2 * s.nh;
+  for (int k = 0; k < 4; ++k) {
+    std::memset(D[k], 0, (size_t)ld * ny * sizeof(double));
+    if ((rc = download_field(c, D[k], Obuf[k], ld, s.nh))) return rc;
+  }
+  return hip_check(hipStreamSynchronize(c->stream), "residual");
+}
+
+}  // extern "C"

@@ -1,0 +1,54 @@
+// Internal host-side types of the CATO HIP runtime.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "aux_kernels.hpp"
+#include "stage_dispatch.hpp"
+
+namespace cato {
+
+int fail(int code, const std::string &msg);
+int hip_check(hipError_t e, const char *what);
+
+// Separable geometry of a rectilinear block, indexed by halo'd cell (0-based):
+// dxi = x(i+1)-x(i), lxi = sqrt(dxi**2 + 0**2) (bottom/top edge length), same for j.
+struct Geometry1D {
+  std::vector<double> dxi, lxi, dyj, lyj;
+};
+int build_geometry(int ni, int nj, int nh, const double *nx, const double *ny, Geometry1D &g);
+int default_tile(const cato_config &c, int ni, int nj);
+
+// One solve: configuration, geometry and the device arrays it owns.
+struct Solve {
+  cato_config cfg{};
+  int ni = 0, nj = 0, nh = 0, pitch = 0;
+  int tile = 0, tiles_x = 0, tiles_y = 0;
+  Geometry1D geo;
+  StageLauncher launcher = nullptr;
+  // device arrays (carved from one allocation by bind())
+  double *U[3][4] = {};
+  double *prim[4] = {};
+  double *dxi = nullptr, *lxi = nullptr, *dyj = nullptr, *lyj = nullptr;
+  double *block_min = nullptr;
+  double *result = nullptr;
+  int *block_err = nullptr;
+  LoopState *loop = nullptr;
+  int cur = 0;              // which U[] holds U^n
+  bool prims_valid = false; // prim[] holds the primitives of U^n as given by the caller
+
+  int init(const cato_config &c, const cato_geom &g);
+  size_t field_elems() const;
+  size_t device_bytes() const;
+  void bind(char *base);
+  int upload_geometry(hipStream_t s);
+  FieldGeom fgeom() const;
+  StageArgs stage_args(int mode, double a, double b, double c, int in_buf, int u0_buf, int out_buf, bool last,
+                       const LoopState *lp, double dt_value) const;
+  std::vector<StageArgs> step_stages(const LoopState *lp, double dt_value) const;
+};
+
+}  // namespace cato

@@ -1,0 +1,43 @@
+// Launch table of the fused stage kernel: one entry per (flux, limiter,
+// precision, tile shape).  Each flux family is instantiated in its own
+// translation unit (stage_f<id>.hip) so the library builds in parallel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "cato_types.hpp"
+
+namespace cato {
+
+struct TileShape {
+  int tx, ty;
+};
+constexpr int kNumTiles = 4;
+constexpr TileShape kTiles[kNumTiles] = {{32, 8}, {64, 4}, {16, 16}, {64, 8}};
+
+using StageLauncher = void (*)(dim3 grid, hipStream_t stream, const StageArgs &args);
+
+StageLauncher stage_launcher_f0(int lim, int prec, int tile);
+StageLauncher stage_launcher_f1(int lim, int prec, int tile);
+StageLauncher stage_launcher_f2(int lim, int prec, int tile);
+StageLauncher stage_launcher_f3(int lim, int prec, int tile);
+StageLauncher stage_launcher_f4(int lim, int prec, int tile);
+StageLauncher stage_launcher_f5(int lim, int prec, int tile);
+StageLauncher stage_launcher_f6(int lim, int prec, int tile);
+StageLauncher stage_launcher_f7(int lim, int prec, int tile);
+
+inline StageLauncher stage_launcher(int flux, int lim, int prec, int tile) {
+  switch (flux) {
+    case 0: return stage_launcher_f0(lim, prec, tile);
+    case 1: return stage_launcher_f1(lim, prec, tile);
+    case 2: return stage_launcher_f2(lim, prec, tile);
+    case 3: return stage_launcher_f3(lim, prec, tile);
+    case 4: return stage_launcher_f4(lim, prec, tile);
+    case 5: return stage_launcher_f5(lim, prec, tile);
+    case 6: return stage_launcher_f6(lim, prec, tile);
+    case 7: return stage_launcher_f7(lim, prec, tile);
+    default: return nullptr;
+  }
+}
+
+}  // namespace cato

@@ -1,0 +1,266 @@
+// Fused SSP-RK stage kernel for gfx950.
+//
+// One workgroup owns a TX x TY tile of interior cells.  In one pass it:
+//   1. builds the primitive variables (rho, u, v, p) of the tile plus an
+//      H-cell halo in LDS, recomputing them from the conserved state U_k
+//      (fluid_t%calculate_derived_quantities) and resolving ghost cells through
+//      the boundary conditions' per-axis index map (apply_primitive_bc);
+//   2. reconstructs the L/R states and evaluates every edge flux of the tile
+//      exactly once (muscl_*%interpolate_edge_values + get_<dir>flux);
+//   3. forms dU/dt per cell (flux_split_edges) and the SSP stage combination
+//      (fluid_t operators, mod_fluid.f90:593-669) and writes U_{k+1};
+//   4. on the last stage, recovers the new primitives and reduces the CFL
+//      time step and the sanity flags per workgroup (get_timestep, sanity_check).
+// HBM traffic per cell is the compulsory minimum: U_k (+U_0) read once,
+// U_{k+1} written once; geometry of rectilinear grids is two 1D arrays.
+#pragma once
+
+#include "physics.hpp"
+
+namespace cato {
+
+constexpr int kThreads = 256;
+constexpr int kNumXcd = 8;  // MI355X: 8 XCDs with private L2, workgroups dealt round-robin
+
+// XCD-aware tile order: workgroup b runs on XCD b % 8; give each XCD a
+// contiguous run of tiles so neighbouring tiles (which share halo rows) hit
+// the same L2.
+__device__ inline int xcd_tile(int b, int nb) {
+  int x = b % kNumXcd, local = b / kNumXcd;
+  int q = nb / kNumXcd, r = nb % kNumXcd;
+  int start = x * q + (x < r ? x : r);
+  return start + local;
+}
+
+// Ghost-cell resolution: the per-axis composition of the BC fills
+// (flux_solver.f90:138-185 with periodic/zero-gradient/symmetry on one image).
+__device__ inline int map_axis(int g, int n, int bc_hi, int bc_lo, bool &flip) {
+  flip = false;
+  if (g < 1) {
+    if (bc_lo == BC_PERIODIC) return g + n;
+    if (bc_lo == BC_ZERO_GRADIENT) return 1;
+    flip = true;
+    return 1 - g;
+  }
+  if (g > n) {
+    if (bc_hi == BC_PERIODIC) return g - n;
+    if (bc_hi == BC_ZERO_GRADIENT) return n;
+    flip = true;
+    return 2 * n + 1 - g;
+  }
+  return g;
+}
+
+template <typename T> __device__ inline T wave_min(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    T w = __shfl_xor(v, o, 64);
+    v = (w < v) ? w : v;
+  }
+  return v;
+}
+__device__ inline int wave_or(int v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Primitive of halo'd cell (gi, gj) from the stored state, with BC mapping.
+__device__ inline Prim<double> load_prim(const StageArgs &a, int gi, int gj, int &err) {
+  bool fu, fv;
+  int si = map_axis(gi, a.ni, a.bc[0], a.bc[1], fu);
+  int sj = map_axis(gj, a.nj, a.bc[2], a.bc[3], fv);
+  size_t off = (size_t)(sj - 1) * a.pitch + (size_t)(si - 1);
+  Prim<double> pr;
+  if (a.prim_from_array) {
+    pr.rho = a.prim[0][off];
+    pr.u = a.prim[1][off];
+    pr.v = a.prim[2][off];
+    pr.p = a.prim[3][off];
+  } else {
+    pr = cons_to_prim(a.uk[0][off], a.uk[1][off], a.uk[2][off], a.uk[3][off], a.gamma, a.gamma - 1.0, nullptr);
+  }
+  if (pr.rho != pr.rho || pr.u != pr.u || pr.v != pr.v || pr.p != pr.p) err |= 1;
+  if (fu) pr.u = -pr.u;
+  if (fv) pr.v = -pr.v;
+  return pr;
+}
+
+template <int LIMC> constexpr int halo_of() { return LIMC == LIM_MLP5 ? 3 : 2; }
+
+// Reconstruct one variable at an edge from LDS row/column access.
+// s(k) returns q at normal offset k (k=-2..3) on the edge's row; t(k, dt) the
+// transverse neighbours.
+template <typename R, int LIMC>
+__device__ inline void recon_var(const R *base, int sn, int st, R &L, R &Rv) {
+  // base points at cell 0 (left of the edge); sn = stride along the normal, st = transverse stride
+  if constexpr (LIMC == LIM_MLP3 || LIMC == LIM_MLP5) {
+    R qm3 = (LIMC == LIM_MLP5) ? base[-2 * sn] : R(0);
+    R q3 = (LIMC == LIM_MLP5) ? base[3 * sn] : R(0);
+    recon_mlp<R, (LIMC == LIM_MLP5 ? 5 : 3)>(qm3, base[-sn], base[0], base[sn], base[2 * sn], q3, base[st], base[-st],
+                                               base[sn + st], base[sn - st], L, Rv);
+  } else {
+    recon_tvd2<R, LIMC>(base[-sn], base[0], base[sn], base[2 * sn], L, Rv);
+  }
+}
+
+template <int FLUX, int LIMC, typename R, int TX, int TY>
+__global__ __launch_bounds__(kThreads) void stage_kernel(StageArgs a) {
+  constexpr int H = halo_of<LIMC>();
+  constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
+  constexpr int NIE = (TX + 1) * TY, NJE = TX * (TY + 1);
+  __shared__ R sp[4][RY][RX];
+  __shared__ R sfi[4][TY][TX + 1];
+  __shared__ R sfj[4][TY + 1][TX];
+  __shared__ double red_min[kThreads / 64];
+  __shared__ int red_err[kThreads / 64];
+
+  if (a.loop && a.loop->done) return;
+  const double dt = a.loop ? a.loop->dt : a.dt_value;
+  const int tid = threadIdx.x;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+  const int i0 = tx * TX + 1, j0 = ty * TY + 1;
+  const int ni = a.ni, nj = a.nj, nh = a.nh;
+  int err = 0;
+
+  // ---- 1. primitives of the tile + halo ---------------------------------
+  for (int idx = tid; idx < RX * RY; idx += kThreads) {
+    int ly = idx / RX, lx = idx - ly * RX;
+    int gi = i0 - H + lx, gj = j0 - H + ly;
+    Prim<double> pr{0.0, 0.0, 0.0, 0.0};
+    if (gi >= 1 - nh && gi <= ni + nh && gj >= 1 - nh && gj <= nj + nh) {
+      int e2 = 0;
+      pr = load_prim(a, gi, gj, e2);
+      if (gi >= 1 && gi <= ni && gj >= 1 && gj <= nj) err |= e2;
+    }
+    sp[0][ly][lx] = (R)pr.rho;
+    sp[1][ly][lx] = (R)pr.u;
+    sp[2][ly][lx] = (R)pr.v;
+    sp[3][ly][lx] = (R)pr.p;
+  }
+  __syncthreads();
+
+  // ---- 2. edge states and fluxes, each edge once --------------------------
+  const R gamma = (R)a.gamma;
+  for (int it = tid; it < NIE + NJE; it += kThreads) {
+    const bool idir = it < NIE;
+    int ex, ey;
+    if (idir) {
+      ey = it / (TX + 1);
+      ex = it - ey * (TX + 1);
+    } else {
+      int k = it - NIE;
+      ey = k / TX;
+      ex = k - ey * TX;
+    }
+    // edge between cell (ci, cj) and its +normal neighbour
+    int ci = idir ? i0 - 1 + ex : i0 + ex;
+    int cj = idir ? j0 + ey : j0 - 1 + ey;
+    if (idir ? (ci > ni || cj > nj) : (ci > ni || cj > nj)) continue;
+    int lx = ci - (i0 - H), ly = cj - (j0 - H);
+    const int sn = idir ? 1 : RX, st = idir ? RX : 1;
+    EdgeState<R> s;
+    recon_var<R, LIMC>(&sp[0][ly][lx], sn, st, s.rL, s.rR);
+    recon_var<R, LIMC>(&sp[1][ly][lx], sn, st, s.uL, s.uR);
+    recon_var<R, LIMC>(&sp[2][ly][lx], sn, st, s.vL, s.vR);
+    recon_var<R, LIMC>(&sp[3][ly][lx], sn, st, s.pL, s.pR);
+    PressNbr<R> pn;
+    if constexpr (needs_pressure_nbrs<FLUX>()) {
+      const R *pc = &sp[3][ly][lx];
+      pn.p00 = pc[0];
+      pn.p10 = pc[sn];
+      pn.p20 = pc[2 * sn];
+      pn.pm10 = pc[-sn];
+      pn.p0p = pc[st];
+      pn.p0m = pc[-st];
+      pn.p1p = pc[sn + st];
+      pn.p1m = pc[sn - st];
+    }
+    R f[4];
+    if (idir) {
+      edge_flux<R, FLUX, 0>(gamma, a.low_mach_fix, s, pn, f);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) sfi[v][ey][ex] = f[v];
+    } else {
+      edge_flux<R, FLUX, 1>(gamma, a.low_mach_fix, s, pn, f);
+#pragma unroll
+      for (int v = 0; v < 4; ++v) sfj[v][ey][ex] = f[v];
+    }
+  }
+  __syncthreads();
+
+  // ---- 3. divergence + stage combination ---------------------------------
+  double lmin = __builtin_inf();
+  const double EPSD = Lim<double>::eps;
+  for (int c = tid; c < TX * TY; c += kThreads) {
+    int cy = c / TX, cx = c - cy * TX;
+    int gi = i0 + cx, gj = j0 + cy;
+    if (gi > ni || gj > nj) continue;
+    double dxv = a.dxi[gi + nh - 1], dyv = a.dyj[gj + nh - 1];
+    double l1 = a.lxi[gi + nh - 1], l2 = a.lyj[gj + nh - 1];
+    // edge-length equalisation (quad_cell.f90:110-112) over (l1, l2, l1, l2)
+    double mx = fmax2(l1, l2), mn = fmin2(l1, l2);
+    double lb = l1, lr = l2, lt = l1, ll = l2;
+    if (mx - mn < 2.0 * EPSD) lb = lr = lt = ll = mx;
+    double vol = dxv * dyv;
+    uint64_t vb = (uint64_t)__double_as_longlong(vol);
+    bool pow2 = (vb & 0x000FFFFFFFFFFFFFull) == 0 && ((vb >> 52) & 0x7FF) > 1 && ((vb >> 52) & 0x7FF) < 2046;
+    double inv_vol = pow2 ? 1.0 / vol : 0.0;
+    size_t off = (size_t)(gj - 1) * a.pitch + (size_t)(gi - 1);
+    double nv[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      double Fi = (double)sfi[v][cy][cx + 1], Fim = (double)sfi[v][cy][cx];
+      double Gj = (double)sfj[v][cy + 1][cx], Gjm = (double)sfj[v][cy][cx];
+      double L = divergence(Fi, Fim, Gj, Gjm, lb, lr, lt, ll, vol, pow2, inv_vol);
+      double r;
+      if (a.mode == MODE_RESIDUAL) {
+        r = L;
+      } else if (a.mode == MODE_FIRST) {
+        r = op_add(a.uk[v][off], L * (a.c * dt));
+      } else if (a.mode == MODE_BASE) {
+        r = op_add(a.u0[v][off], L * (dt * a.c));
+      } else {
+        double t = op_add(a.u0[v][off] * a.a, a.uk[v][off] * a.b);
+        r = op_add(t, L * (a.c * dt));
+      }
+      nv[v] = r;
+      a.out[v][off] = r;
+    }
+    if (a.last_stage) {
+      double cs;
+      Prim<double> q = cons_to_prim(nv[0], nv[1], nv[2], nv[3], a.gamma, a.gamma - 1.0, &cs);
+      if (q.rho != q.rho || q.u != q.u || q.v != q.v || q.p != q.p) err |= 1;
+      if (q.rho < 0.0) err |= 2;
+      if (q.p < 0.0) err |= 4;
+      // get_timestep (mod_fluid.f90:554-558); min_dx/min_dy = |dx|, |dy| on rectilinear cells
+      double val = a.cfl / (((absd(q.u) + cs) / absd(dxv)) + ((absd(q.v) + cs) / absd(dyv)));
+      lmin = (val < lmin) ? val : lmin;
+    }
+  }
+
+  if (a.last_stage || a.block_err) {
+    double wm = wave_min(lmin);
+    int we = wave_or(err);
+    int w = tid >> 6;
+    if ((tid & 63) == 0) {
+      red_min[w] = wm;
+      red_err[w] = we;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      double m = red_min[0];
+      int e = red_err[0];
+#pragma unroll
+      for (int k = 1; k < kThreads / 64; ++k) {
+        m = (red_min[k] < m) ? red_min[k] : m;
+        e |= red_err[k];
+      }
+      if (a.block_min) a.block_min[blockIdx.x] = m;
+      if (a.block_err) a.block_err[blockIdx.x] = e;
+    }
+  }
+}
+
+}  // namespace cato

@@ -1,0 +1,166 @@
+/*
+ * cato_hip.h -- C ABI of the MI355X-native CATO hot path (libcato_hip.so).
+ *
+ * The reference (smillerc/fvleg_2d, "CATO") advances the 2D Euler equations
+ * through four Fortran plugin interfaces; every entry point below replaces one
+ * of them and keeps its argument meaning:
+ *
+ *   flux_solver_t%solve            src/lib/flux_solvers/flux_solver.f90:66-87   -> cato_residual
+ *   boundary_condition_t%apply     src/lib/boundary_conditions/boundary_conditions.f90:60-70 and
+ *                                  flux_solver_t%apply_primitive_bc flux_solver.f90:138-185 -> cato_apply_bc
+ *   muscl_interpolation_t%interpolate_edge_values
+ *                                  src/lib/spatial_reconstruction/muscl_interpolation.f90:37-43 -> cato_interpolate_edges
+ *   get_<dir>flux of each solver   e.g. slau_solver.fypp:253-494               -> cato_edge_fluxes
+ *   fluid_t%get_timestep           src/lib/fluid/mod_fluid.f90:518-562          -> cato_compute_dt
+ *   fluid_t%integrate              src/lib/fluid/mod_fluid.f90:411-450          -> cato_step
+ *   program cato time loop         src/main.f90:138-198                         -> cato_run
+ *   fluid_t%initialize_from_hdf5   src/lib/fluid/mod_fluid.f90:256-381          -> cato_set_primitive
+ *
+ * Conventions
+ *   - Host arrays are Fortran-ordered (i fastest).  A field argument is a
+ *     pointer plus `ld` (elements between consecutive j) and `halo` (number of
+ *     ghost layers stored around the interior in that host array), so the
+ *     reference's data(ilo-nh:ihi+nh, jlo-nh:jhi+nh) is passed with
+ *     halo = nh, ld = ni + 2*nh.
+ *   - Every call returns CATO_OK (0) or a negative status; cato_last_error()
+ *     describes the last failure of the calling thread.  Nothing aborts.
+ *   - The solve's error code of the reference (error.f90:6-9) is returned
+ *     through `error_code`: 0 ALL_OK, 1 NEG_DENSITY, 2 NEG_PRESSURE, 3 NANS_FOUND.
+ *   - A context owns its device state, one HIP stream and no globals; distinct
+ *     contexts may be used from distinct threads.
+ */
+#ifndef CATO_HIP_H
+#define CATO_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CATO_HIP_ABI_VERSION 1
+
+/* flux_solver names of the input deck (mod_fluid.f90:223-238) */
+enum {
+  CATO_FLUX_AUSMPW_PLUS = 0, /* 'AUSMPW+'                                   */
+  CATO_FLUX_SLAU = 1,        /* 'SLAU'                                      */
+  CATO_FLUX_SLAU2 = 2,       /* 'SLAU2'                                     */
+  CATO_FLUX_SD_SLAU = 3,     /* 'SD-SLAU'                                   */
+  CATO_FLUX_SD_SLAU2 = 4,    /* 'SD-SLAU2'                                  */
+  CATO_FLUX_ROE = 5,         /* 'Roe', 'S-Roe', 'SP-Roe' (reference variant) */
+  CATO_FLUX_AUSM_PLUS = 6,   /* 'AUSM+' / 'AUSM+-u'                         */
+  CATO_FLUX_HLLC = 7         /* 'HLLC' (Toro, PVRS wave speeds)             */
+};
+/* limiter names (muscl_factory.f90:69-90) */
+enum {
+  CATO_LIMITER_MINMOD = 0,
+  CATO_LIMITER_SUPERBEE = 1,
+  CATO_LIMITER_VAN_LEER = 2,
+  CATO_LIMITER_MLP3 = 3,
+  CATO_LIMITER_MLP5 = 4
+};
+/* boundary condition names (bc_factory.f90:48-64) */
+enum { CATO_BC_PERIODIC = 0, CATO_BC_ZERO_GRADIENT = 1, CATO_BC_SYMMETRY = 2 };
+/* integration_strategy names (mod_fluid.f90:424-446) */
+enum {
+  CATO_RK_SSP2 = 0, /* 'ssp_rk2'             */
+  CATO_RK_SSP3 = 1, /* 'ssp_rk3', 'ssp_rk33' */
+  CATO_RK_1_3 = 2,  /* 'rk(1,3)'             */
+  CATO_RK_1_4 = 3,
+  CATO_RK_1_5 = 4,
+  CATO_RK_2_3 = 5,
+  CATO_RK_2_4 = 6,
+  CATO_RK_2_5 = 7
+};
+/* arithmetic of reconstruction + edge flux; state, divergence, RK and CFL stay FP64 */
+enum { CATO_PRECISION_FP64 = 0, CATO_PRECISION_FP32_FLUX = 1 };
+
+enum {
+  CATO_OK = 0,
+  CATO_ERR_INVALID = -1,     /* bad argument or configuration */
+  CATO_ERR_UNSUPPORTED = -2, /* valid for the reference but not for this path (e.g. curvilinear grid) */
+  CATO_ERR_HIP = -3,         /* HIP runtime failure */
+  CATO_ERR_NO_DEVICE = -4
+};
+
+typedef struct cato_config {
+  int flux;
+  int limiter;
+  int integrator;
+  int precision;
+  int bc_plus_x, bc_minus_x, bc_plus_y, bc_minus_y;
+  int n_ghost_layers;     /* 2 (TVD2, MLP3) or 3 (MLP5): mod_input.f90:320-338 */
+  int apply_low_mach_fix; /* AUSMPW+ option, deck default true (mod_input.f90:278) */
+  double gamma;           /* [physics] polytropic_index */
+  double cfl;             /* [time] cfl */
+  int tile;               /* -1: library choice; otherwise a tile id from cato_tile_shape */
+} cato_config;
+
+typedef struct cato_geom {
+  int ni, nj;             /* interior cells */
+  const double *node_x;   /* (ni+2nh+1) x (nj+2nh+1) node coordinates incl. ghost nodes,   */
+  const double *node_y;   /* column-major, as the pycato deck stores /x and /y            */
+} cato_geom;
+
+typedef struct cato_ctx cato_ctx;
+
+const char *cato_last_error(void);
+int cato_abi_version(void);
+
+/* Tile shapes of the fused stage kernel (used by the autotuner). */
+int cato_num_tiles(void);
+int cato_tile_shape(int tile, int *tile_x, int *tile_y);
+
+/* fluid_t%initialize + grid_block_2d_t geometry: validates the configuration,
+ * builds the per-cell geometry from the nodes (quad_cell.f90) and allocates the
+ * device-resident state.  Curvilinear grids return CATO_ERR_UNSUPPORTED. */
+int cato_ctx_create(const cato_config *cfg, const cato_geom *geom, cato_ctx **out);
+int cato_ctx_destroy(cato_ctx *ctx);
+
+/* Initial / restart state from primitives (mod_fluid.f90:339-363): conserved
+ * variables are formed on the device; the first stage uses these primitives. */
+int cato_set_primitive(cato_ctx *ctx, const double *rho, const double *u, const double *v, const double *p, int ld,
+                       int halo);
+/* State from conserved variables (primitives are recovered with the Mach filter). */
+int cato_set_conserved(cato_ctx *ctx, const double *rho, const double *rho_u, const double *rho_v,
+                       const double *rho_E, int ld, int halo);
+int cato_get_conserved(cato_ctx *ctx, double *rho, double *rho_u, double *rho_v, double *rho_E, int ld, int halo);
+/* u, v, p, cs of fluid_t after calculate_derived_quantities (any pointer may be NULL). */
+int cato_get_primitive(cato_ctx *ctx, double *rho, double *u, double *v, double *p, double *cs, int ld, int halo);
+
+/* fluid_t%get_timestep */
+int cato_compute_dt(cato_ctx *ctx, double *dt);
+/* fluid_t%integrate with a host-given dt (one full SSP/RK step). */
+int cato_step(cato_ctx *ctx, double dt, int *error_code);
+/* main.f90 time loop on the device: while time < max_time and iteration <
+ * max_iterations, dt = CFL (or constant_dt > 0), integrate, advance the driver
+ * time with the contour-interval clip.  Returns iterations, driver time and the
+ * first error code.  The host only polls every few steps. */
+int cato_run(cato_ctx *ctx, double max_time, long long max_iterations, double contour_interval_dt, double constant_dt,
+             long long *iterations, double *time, int *error_code);
+/* Enqueue exactly n steps with CFL dt and no host synchronisation (benchmarks). */
+int cato_advance(cato_ctx *ctx, long long n);
+int cato_synchronize(cato_ctx *ctx);
+
+/* Per-launch timing of the fused stage kernel, measured with HIP events on the
+ * context's stream while enabled. */
+int cato_set_timing(cato_ctx *ctx, int enable);
+int cato_get_timing(cato_ctx *ctx, double *stage_ms_total, long long *stage_launches, double *other_ms_total,
+                    long long *other_launches);
+
+/* ---- operator-level entry points (arrays carry n_ghost_layers halos, ld >= ni+2nh) ---- */
+/* apply_primitive_bc: fills the ghost cells of rho, u, v, p in place. */
+int cato_apply_bc(cato_ctx *ctx, double *rho, double *u, double *v, double *p, int ld);
+/* interpolate_edge_values for one halo'd field q: i_edges and j_edges are
+ * (2, 0:ni, 0:nj) arrays of (L, R) states. */
+int cato_interpolate_edges(cato_ctx *ctx, const double *q, int ld, double *i_edges, double *j_edges);
+/* get_<dir>flux: edges = 8 consecutive (2, 0:ni, 0:nj) arrays (rho_i, rho_j, u_i, u_j,
+ * v_i, v_j, p_i, p_j); p = halo'd cell pressure; iflux (4, 0:ni, 1:nj), jflux (4, 1:ni, 0:nj). */
+int cato_edge_fluxes(cato_ctx *ctx, const double *edges, const double *p, int ld, double *iflux, double *jflux);
+/* flux_solver_t%solve: BCs are applied to rho, u, v, p in place (intent(inout)),
+ * d_* receive d(rho, rho u, rho v, rho E)/dt with zeroed halos. */
+int cato_residual(cato_ctx *ctx, double *rho, double *u, double *v, double *p, double *d_rho, double *d_rho_u,
+                  double *d_rho_v, double *d_rho_E, int ld);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,268 @@
+"""Parity of the HIP path against the oracle (run on an MI355X).
+
+Every comparison goes through the C ABI of libcato_hip.so.  FP64 results are
+expected to be bit-identical to the oracle; the asserted bar is the
+north-star tolerance, relative L-infinity <= 1e-12 per conserved variable.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from fvleg_2d_amd import SchemeConfig, Solver
+from fvleg_2d_amd.problems import kelvin_helmholtz, riemann2d, sod, uniform_nodes
+
+pytestmark = pytest.mark.gpu
+
+FLUXES = ["AUSMPW+", "SLAU", "SLAU2", "SD-SLAU", "SD-SLAU2", "Roe", "AUSM+", "HLLC"]
+LIMITERS = ["minmod", "superbee", "van_leer", "MLP3", "MLP5"]
+TOL = 1e-12
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_linf(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def oracle_cfg(O, cfg: SchemeConfig, ni, nj):
+    return O.Config(ni, nj, flux=cfg.flux, limiter=cfg.limiter, integrator=cfg.integrator, bcs=cfg.bcs,
+                    gamma=cfg.gamma, cfl=cfg.cfl, apply_low_mach_fix=cfg.apply_low_mach_fix)
+
+
+def pad(a, nh):
+    return np.asfortranarray(np.pad(np.asarray(a, dtype=np.float64), nh))
+
+
+def shock_state(ni, nj, seed):
+    """Smooth random field with an embedded oblique jump (exercises limiters and sensors)."""
+    rng = np.random.default_rng(seed)
+    x = (np.arange(ni) + 0.5) / ni
+    y = (np.arange(nj) + 0.5) / nj
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    jump = (X + 0.3 * Y) > 0.55
+    rho = np.where(jump, 0.4, 1.2) + 0.05 * np.sin(2 * np.pi * (X + rng.uniform()))
+    u = 0.3 * np.cos(2 * np.pi * (Y + rng.uniform())) + np.where(jump, 0.2, -0.1)
+    v = 0.25 * np.sin(2 * np.pi * (X + 2 * Y + rng.uniform()))
+    p = np.where(jump, 0.3, 1.0) + 0.02 * np.cos(2 * np.pi * (X - Y + rng.uniform()))
+    return [np.asfortranarray(a) for a in (rho, u, v, p)]
+
+
+BC_SETS = [
+    ("zero_gradient",) * 4,
+    ("periodic",) * 4,
+    ("zero_gradient", "zero_gradient", "periodic", "periodic"),
+    ("periodic", "periodic", "zero_gradient", "zero_gradient"),
+    ("symmetry",) * 4,
+    ("symmetry", "zero_gradient", "zero_gradient", "symmetry"),
+    ("zero_gradient", "symmetry", "periodic", "periodic"),
+]
+
+
+@pytest.mark.parametrize("bcs", BC_SETS, ids=lambda b: "-".join(x[:3] for x in b))
+@pytest.mark.parametrize("lim", ["minmod", "MLP5"])
+def test_apply_bc_parity(oracle, bcs, lim):
+    say("apply_bc", bcs, lim)
+    ni, nj = 13, 9
+    cfg = SchemeConfig(limiter=lim, bcs=bcs)
+    nh = cfg.n_ghost
+    nx, ny = uniform_nodes(ni, nj, nh)
+    rng = np.random.default_rng(1)
+    fields = [np.asfortranarray(rng.uniform(-1, 1, (ni + 2 * nh, nj + 2 * nh))) for _ in range(4)]
+    with Solver(cfg, nx, ny) as s:
+        g = [f.copy(order="F") for f in fields]
+        s.apply_bc(*g)
+    oc = oracle_cfg(oracle, cfg, ni, nj)
+    o = [f.copy(order="F") for f in fields]
+    oracle.apply_bcs(oc, *o)
+    for a, b in zip(g, o):
+        assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("lim", LIMITERS)
+def test_interpolate_edges_parity(oracle, lim):
+    say("interpolate", lim)
+    ni, nj = 21, 17
+    cfg = SchemeConfig(limiter=lim)
+    nh = cfg.n_ghost
+    nx, ny = uniform_nodes(ni, nj, nh)
+    rng = np.random.default_rng(2)
+    q = np.asfortranarray(np.cumsum(rng.uniform(-1, 1, (ni + 2 * nh, nj + 2 * nh)), axis=0))
+    q[5:9, :] += 3.0
+    with Solver(cfg, nx, ny) as s:
+        gi, gj = s.interpolate_edges(q)
+    oi, oj = oracle.interpolate_edges(oracle_cfg(oracle, cfg, ni, nj), q)
+    assert np.array_equal(gi, oi)
+    assert np.array_equal(gj, oj)
+
+
+@pytest.mark.parametrize("flux", FLUXES)
+def test_edge_flux_parity(oracle, flux):
+    say("edge_flux", flux)
+    ni, nj = 19, 15
+    cfg = SchemeConfig(flux=flux, limiter="minmod")
+    nh = cfg.n_ghost
+    nx, ny = uniform_nodes(ni, nj, nh)
+    rho, u, v, p = (pad(a, nh) for a in shock_state(ni, nj, 3))
+    oc = oracle_cfg(oracle, cfg, ni, nj)
+    oracle.apply_bcs(oc, rho, u, v, p)
+    edges = []
+    for q in (rho, u, v, p):
+        edges += list(oracle.interpolate_edges(oc, q))
+    og = oracle.Geometry(oc, nx, ny)
+    oi, oj = oracle.edge_fluxes(oc, og, rho, u, v, p, edges)
+    with Solver(cfg, nx, ny) as s:
+        gi, gj = s.edge_fluxes(edges, p)
+    assert np.array_equal(gi, oi), np.abs(gi - oi).max()
+    assert np.array_equal(gj, oj), np.abs(gj - oj).max()
+
+
+@pytest.mark.parametrize("flux", FLUXES)
+@pytest.mark.parametrize("lim", LIMITERS)
+def test_residual_parity(oracle, flux, lim):
+    say("residual", flux, lim)
+    ni, nj = 37, 29  # not multiples of any tile: partial tiles on both edges
+    cfg = SchemeConfig(flux=flux, limiter=lim, bcs=("zero_gradient", "symmetry", "periodic", "periodic"))
+    nh = cfg.n_ghost
+    nx, ny = uniform_nodes(ni, nj, nh, (0.0, 1.0), (0.0, 0.8))
+    prims = [pad(a, nh) for a in shock_state(ni, nj, 4)]
+    oc = oracle_cfg(oracle, cfg, ni, nj)
+    og = oracle.Geometry(oc, nx, ny)
+    o_in = [a.copy(order="F") for a in prims]
+    od = oracle.solve(oc, og, *o_in)
+    with Solver(cfg, nx, ny) as s:
+        g_in = [a.copy(order="F") for a in prims]
+        gd = s.residual(*g_in)
+    for a, b in zip(g_in, o_in):
+        assert np.array_equal(a, b)
+    for k in range(4):
+        assert np.array_equal(gd[k], od[k]), (k, rel_linf(gd[k], od[k]))
+
+
+def run_steps(oracle, cfg, nx, ny, prims, nsteps):
+    ni, nj = prims[0].shape
+    nh = cfg.n_ghost
+    oc = oracle_cfg(oracle, cfg, ni, nj)
+    og = oracle.Geometry(oc, nx, ny)
+    f = oracle.Fluid(oc, og, *[pad(a, nh) for a in prims])
+    with Solver(cfg, nx, ny) as s:
+        s.set_primitive(*prims)
+        for k in range(nsteps):
+            dt_g = s.compute_dt()
+            dt_o = f.get_timestep()
+            assert dt_g == dt_o, (k, dt_g, dt_o)
+            eg = s.step(dt_g)
+            eo = f.integrate(dt_o)
+            assert eg == eo == 0
+        g = s.conserved()
+        gp = s.primitive()
+    o = [f.interior(n) for n in ("rho", "rho_u", "rho_v", "rho_E")]
+    op = [f.interior(n) for n in ("rho", "u", "v", "p", "cs")]
+    return g, o, gp, op
+
+
+@pytest.mark.parametrize("flux", FLUXES)
+@pytest.mark.parametrize("lim", LIMITERS)
+def test_step_parity_ssp_rk3(oracle, flux, lim):
+    say("step rk3", flux, lim)
+    ni, nj = 45, 38
+    cfg = SchemeConfig(flux=flux, limiter=lim, integrator="ssp_rk3", bcs=("zero_gradient",) * 4, cfl=0.4)
+    nx, ny = uniform_nodes(ni, nj, cfg.n_ghost)
+    g, o, gp, op = run_steps(oracle, cfg, nx, ny, shock_state(ni, nj, 5), 6)
+    for k in range(4):
+        assert rel_linf(g[k], o[k]) <= TOL
+        assert np.array_equal(g[k], o[k]), (k, rel_linf(g[k], o[k]))
+    for k in range(5):
+        assert np.array_equal(gp[k], op[k]), ("prim", k)
+
+
+@pytest.mark.parametrize("integrator", ["ssp_rk2", "rk(1,3)", "rk(2,4)", "rk(2,5)"])
+@pytest.mark.parametrize("flux", ["SLAU", "AUSMPW+"])
+def test_step_parity_integrators(oracle, integrator, flux):
+    say("step", integrator, flux)
+    ni, nj = 33, 20
+    cfg = SchemeConfig(flux=flux, limiter="MLP3", integrator=integrator, bcs=("periodic",) * 4, cfl=0.3)
+    nx, ny = uniform_nodes(ni, nj, cfg.n_ghost, (0.0, 1.0), (0.0, 0.6))
+    g, o, _, _ = run_steps(oracle, cfg, nx, ny, shock_state(ni, nj, 6), 5)
+    for k in range(4):
+        assert np.array_equal(g[k], o[k]), (k, rel_linf(g[k], o[k]))
+
+
+def run_both(oracle, pb, max_iterations=0):
+    nh = pb.cfg.n_ghost
+    oc = oracle_cfg(oracle, pb.cfg, pb.ni, pb.nj)
+    og = oracle.Geometry(oc, pb.node_x, pb.node_y)
+    f = oracle.Fluid(oc, og, *[pad(a, nh) for a in (pb.rho, pb.u, pb.v, pb.p)])
+    mi = max_iterations if max_iterations else 2 ** 31 - 1
+    n_o, t_o, e_o = f.run(pb.max_time, mi, pb.contour_interval_dt)
+    with Solver(pb.cfg, pb.node_x, pb.node_y) as s:
+        s.set_primitive(pb.rho, pb.u, pb.v, pb.p)
+        n_g, t_g, e_g = s.run(pb.max_time, max_iterations, pb.contour_interval_dt)
+        g = s.conserved()
+    o = [f.interior(n) for n in ("rho", "rho_u", "rho_v", "rho_E")]
+    return (n_g, t_g, e_g, g), (n_o, t_o, e_o, o)
+
+
+def test_sod_config1_full_run(oracle):
+    """Sod 400x4, AUSM+, MUSCL-minmod, SSP-RK2, CFL 0.4, to t = 0.2 (first benchmark configuration)."""
+    pb = sod()
+    say("sod run", pb.ni, pb.nj)
+    (n_g, t_g, e_g, g), (n_o, t_o, e_o, o) = run_both(oracle, pb)
+    say("iterations", n_g, n_o, "time", t_g, t_o)
+    assert e_g == e_o == 0
+    assert n_g == n_o and t_g == t_o
+    for k in range(4):
+        assert rel_linf(g[k], o[k]) <= TOL
+        assert np.array_equal(g[k], o[k])
+
+
+def test_riemann2d_reduced_full_run(oracle):
+    """Four-quadrant Riemann problem (HLLC, MLP5, SSP-RK3, CFL 0.4) to t = 0.25 at 96^2."""
+    pb = riemann2d(n=96, seed=1)
+    say("riemann2d run", pb.ni)
+    (n_g, t_g, e_g, g), (n_o, t_o, e_o, o) = run_both(oracle, pb)
+    say("iterations", n_g, n_o)
+    assert e_g == e_o == 0 and n_g == n_o and t_g == t_o
+    for k in range(4):
+        assert rel_linf(g[k], o[k]) <= TOL
+
+
+def test_kelvin_helmholtz_reduced(oracle):
+    """KH shear layer (SLAU, van Leer, SSP-RK3, CFL 0.5, periodic) at 128^2 for 60 steps."""
+    pb = kelvin_helmholtz(n=128, steps=60)
+    say("kh run", pb.ni)
+    (n_g, t_g, e_g, g), (n_o, t_o, e_o, o) = run_both(oracle, pb, max_iterations=60)
+    assert n_g == n_o == 60 and t_g == t_o
+    for k in range(4):
+        assert rel_linf(g[k], o[k]) <= TOL
+
+
+def test_error_codes_nan_and_negative_density():
+    say("error codes")
+    ni, nj = 20, 10
+    cfg = SchemeConfig(flux="SLAU", limiter="minmod", integrator="ssp_rk2", bcs=("zero_gradient",) * 4)
+    nx, ny = uniform_nodes(ni, nj, 2)
+    rho = np.ones((ni, nj), order="F")
+    u = np.zeros((ni, nj), order="F")
+    p = np.ones((ni, nj), order="F")
+    with Solver(cfg, nx, ny) as s:
+        r = rho.copy(order="F")
+        r[4, 4] = np.nan
+        s.set_primitive(r, u, u.copy(order="F"), p)
+        assert s.step(1e-3) == 3
+        r = rho.copy(order="F")
+        p2 = p.copy(order="F")
+        p2[7, 3] = 1e6  # violent blast with a huge dt drives density negative
+        s.set_primitive(r, u, u.copy(order="F"), p2)
+        assert s.step(0.5) in (1, 2, 3)
+
+
+def test_invalid_configuration_is_rejected():
+    cfg = SchemeConfig(flux="SLAU", limiter="MLP5")
+    cfg.tile = 99
+    nx, ny = uniform_nodes(8, 8, 3)
+    with pytest.raises(Exception):
+        Solver(cfg, nx, ny)

@@ -1,0 +1,289 @@
+"""Pins of the C oracle (CPU only).
+
+The oracle is checked against (1) the reference's own golden vectors
+(tests/golden/, from tests/unit/simple.h5 + mod_test_bc.f90 + mod_test_grid.f90),
+(2) textbook values of the limiter functions, (3) invariants any correct
+implementation satisfies exactly or to round-off (free stream, consistency
+F(U,U) = F(U), mirror and transpose symmetry, conservation on a periodic
+grid, linear exactness / no new extrema of the reconstruction), and (4) the
+exact Riemann solution of the Sod problem (accuracy, not parity).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from exact_riemann import sample
+
+GOLD = os.path.join(ROOT, "tests", "golden")
+FLUXES = ["AUSMPW+", "SLAU", "SLAU2", "SD-SLAU", "SD-SLAU2", "Roe", "AUSM+", "HLLC"]
+LIMITERS = ["minmod", "superbee", "van_leer", "MLP3", "MLP5"]
+
+
+def load_simple():
+    d = json.load(open(os.path.join(GOLD, "simple_h5.json")))
+    return {k: np.asfortranarray(np.array(d[k], dtype=np.float64)) for k in
+            ("x", "y", "density", "x_velocity", "y_velocity", "pressure")}, d["n_ghost_layers"]
+
+
+def fidx(i, nh=2):
+    return i + nh - 1
+
+
+@pytest.mark.parametrize("case", json.load(open(os.path.join(GOLD, "bc_expected.json")))["bc_tests"],
+                         ids=lambda c: c["name"])
+def test_bc_golden(oracle, case):
+    d, nh = load_simple()
+    b = case["deck_bcs"]
+    cfg = oracle.Config(4, 4, bcs=(b["+x"], b["-x"], b["+y"], b["-y"]))
+    f = {n: np.asfortranarray(d[k].copy()) for n, k in
+         (("rho", "density"), ("u", "x_velocity"), ("v", "y_velocity"), ("p", "pressure"))}
+    for a in f.values():  # zero_out_halo
+        a[:nh, :] = 0.0
+        a[-nh:, :] = 0.0
+        a[:, :nh] = 0.0
+        a[:, -nh:] = 0.0
+    for bc_type, loc in case["apply"]:
+        oracle.apply_bc_single(cfg, bc_type, loc, f["rho"], f["u"], f["v"], f["p"])
+    for fld, kind, idx, (lo, hi), vals in case["checks"]:
+        if kind == "col":
+            got = f[fld][fidx(idx), fidx(lo):fidx(hi) + 1]
+        else:
+            got = f[fld][fidx(lo):fidx(hi) + 1, fidx(idx)]
+        assert np.array_equal(got, np.array(vals, dtype=np.float64)), (fld, kind, idx, got, vals)
+
+
+def test_geometry_golden(oracle):
+    d, nh = load_simple()
+    exp = json.load(open(os.path.join(GOLD, "bc_expected.json")))["grid"]
+    for j in range(d["x"].shape[1]):
+        assert np.array_equal(d["x"][:, j], exp["node_x_row"])
+    for i in range(d["y"].shape[0]):
+        assert np.array_equal(d["y"][i, :], exp["node_y_col"])
+    cfg = oracle.Config(4, 4)
+    g = oracle.Geometry(cfg, d["x"], d["y"])
+    for e in range(4):
+        assert np.all(g.edge_norm[0, e] == exp["edge_norm"][e][0])
+        assert np.all(g.edge_norm[1, e] == exp["edge_norm"][e][1])
+        assert np.all(g.edge_lengths[e] == exp["edge_lengths"][e])
+    assert np.all(g.volume == exp["volume"])
+    assert np.all(g.dx == 2.0) and np.all(g.dy == 1.0)
+
+
+@pytest.mark.parametrize("lim,r,phi", [
+    ("minmod", -1.0, 0.0), ("minmod", 0.5, 0.5), ("minmod", 2.0, 1.0),
+    ("superbee", -1.0, 0.0), ("superbee", 0.25, 0.5), ("superbee", 0.75, 1.0), ("superbee", 1.5, 1.5),
+    ("superbee", 3.0, 2.0),
+    ("van_leer", -1.0, 0.0), ("van_leer", 0.5, 2.0 / 3.0), ("van_leer", 2.0, 4.0 / 3.0), ("van_leer", 1.0, 1.0),
+])
+def test_tvd_limiter_values(oracle, lim, r, phi):
+    """Textbook limiter values (Sweby diagram): q = [.., 0, 1, 1+r, ..] gives the left face
+    value q_i + phi(r)/2 with r = Delta_{i+1/2}/Delta_{i-1/2}."""
+    cfg = oracle.Config(6, 2, limiter=lim)
+    q = cfg.field()
+    line = np.array([-2.0, -1.0, 0.0, 1.0, 1.0 + r, 1.0 + 2 * r, 1.0 + 3 * r, 1.0 + 4 * r, 1.0 + 5 * r, 1.0 + 6 * r])
+    q[:, :] = line[:, None]
+    ie, _ = oracle.interpolate_edges(cfg, q)
+    # edge between halo'd index 3 (value 1) and 4 (value 1+r): cell i = 3 - nh + 1 = 2
+    L = ie[0, 2, 1]
+    assert abs((L - 1.0) - 0.5 * phi) < 1e-12
+
+
+@pytest.mark.parametrize("lim", LIMITERS)
+def test_reconstruction_constant_and_linear(oracle, lim):
+    cfg = oracle.Config(12, 10, limiter=lim)
+    q = cfg.field()
+    q[:] = 3.25
+    ie, je = oracle.interpolate_edges(cfg, q)
+    assert np.all(ie == 3.25) and np.all(je == 3.25)
+    # linear in i: face values are exact midpoints
+    nx, ny = cfg.shape
+    q[:] = (0.1 * np.arange(nx))[:, None] + 0.0 * np.arange(ny)[None, :]
+    ie, _ = oracle.interpolate_edges(cfg, q)
+    nh = cfg.nh
+    for i in range(0, cfg.ni + 1):
+        face = 0.1 * (i + nh - 1) + 0.05
+        assert abs(ie[0, i, 3] - face) < 1e-12 and abs(ie[1, i, 3] - face) < 1e-12
+
+
+@pytest.mark.parametrize("lim", LIMITERS)
+def test_reconstruction_no_new_extrema(oracle, lim):
+    rng = np.random.default_rng(3)
+    cfg = oracle.Config(40, 6, limiter=lim)
+    q = cfg.field()
+    steps = rng.uniform(0.0, 1.0, cfg.shape[0])
+    steps[rng.uniform(size=steps.size) < 0.3] *= 20.0
+    q[:] = np.cumsum(steps)[:, None]
+    ie, _ = oracle.interpolate_edges(cfg, q)
+    nh = cfg.nh
+    for i in range(0, cfg.ni + 1):
+        a, b = q[i + nh - 1, 3], q[i + nh, 3]
+        assert a - 1e-12 <= ie[0, i, 3] <= b + 1e-12
+        assert a - 1e-12 <= ie[1, i, 3] <= b + 1e-12
+
+
+def physical_flux(rho, u, v, p, g, nx, ny):
+    E = p / (g - 1.0) + 0.5 * rho * (u * u + v * v)
+    qn = u * nx + v * ny
+    return np.array([rho * qn, rho * u * qn + p * nx, rho * v * qn + p * ny, qn * (E + p)])
+
+
+@pytest.mark.parametrize("flux", FLUXES)
+def test_flux_consistency(oracle, flux):
+    """F(U, U) equals the physical Euler flux (uniform field, both directions)."""
+    rng = np.random.default_rng(11)
+    cfg = oracle.Config(4, 4, flux=flux)
+    nodes = oracle.uniform_nodes(4, 4, 2)
+    g = oracle.Geometry(cfg, *nodes)
+    for _ in range(20):
+        st = (rng.uniform(0.1, 2.0), rng.uniform(-1.5, 1.5), rng.uniform(-1.5, 1.5), rng.uniform(0.1, 1.5))
+        edges = []
+        for val in st:
+            e = np.full((2, 5, 5), val, order="F")
+            edges += [e, e]
+        p = cfg.field()
+        p[:] = st[3]
+        fi, fj = oracle.edge_fluxes(cfg, g, p, p, p, p, edges)
+        for F, n in ((fi[:, 2, 1], (1.0, 0.0)), (fj[:, 1, 2], (0.0, 1.0))):
+            ref = physical_flux(*st, 1.4, *n)
+            assert np.allclose(F, ref, rtol=1e-13, atol=1e-13), (flux, st, F, ref)
+
+
+def random_periodic(cfg, seed, amp=0.2):
+    rng = np.random.default_rng(seed)
+    nx, ny = cfg.shape
+    x = np.linspace(0, 2 * np.pi, cfg.ni, endpoint=False)
+    y = np.linspace(0, 2 * np.pi, cfg.nj, endpoint=False)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    fields = []
+    for base, scale in ((1.0, 0.3), (0.0, 0.6), (0.0, 0.6), (1.0, 0.3)):
+        a, b, c, d = rng.uniform(0.5, 1.5, 4)
+        f = base + scale * (np.sin(a * X + b) * np.cos(c * Y + d)) + amp * rng.uniform(-0.05, 0.05, X.shape)
+        full = cfg.field()
+        full[cfg.nh:-cfg.nh, cfg.nh:-cfg.nh] = f
+        fields.append(full)
+    fields[0] = np.abs(fields[0]) + 0.2
+    fields[3] = np.abs(fields[3]) + 0.2
+    return fields
+
+
+@pytest.mark.parametrize("flux", FLUXES)
+@pytest.mark.parametrize("lim", ["minmod", "van_leer", "MLP5"])
+def test_free_stream_preserved_exactly(oracle, flux, lim):
+    cfg = oracle.Config(9, 7, flux=flux, limiter=lim)
+    nodes = oracle.uniform_nodes(9, 7, cfg.nh, (0, 1.3), (0, 0.7))
+    g = oracle.Geometry(cfg, *nodes)
+    rho, u, v, p = (cfg.field() for _ in range(4))
+    rho[:] = 1.7
+    u[:] = 0.31
+    v[:] = -0.42
+    p[:] = 0.9
+    d = oracle.solve(cfg, g, rho, u, v, p)
+    for x in d:
+        assert np.all(x == 0.0)
+
+
+@pytest.mark.parametrize("flux", FLUXES)
+@pytest.mark.parametrize("lim", LIMITERS)
+def test_mirror_and_transpose_symmetry(oracle, flux, lim):
+    """residual(mirror_x(U)) = mirror_x(residual(U)) with rho*u negated, and
+    residual(transpose(U)) = transpose(residual(U)) with u <-> v: pins stencil
+    orientation and the i/j specialisations against each other."""
+    n = 12
+    cfg = oracle.Config(n, n, flux=flux, limiter=lim, bcs=("periodic",) * 4)
+    nodes = oracle.uniform_nodes(n, n, cfg.nh)
+    g = oracle.Geometry(cfg, *nodes)
+    rho, u, v, p = random_periodic(cfg, 5)
+    d0 = oracle.solve(cfg, g, rho.copy(order="F"), u.copy(order="F"), v.copy(order="F"), p.copy(order="F"))
+    nh = cfg.nh
+    inner = (slice(nh, nh + n), slice(nh, nh + n))
+
+    def mir(a):
+        b = a.copy(order="F")
+        b[inner] = a[inner][::-1, :]
+        return b
+
+    dm = oracle.solve(cfg, g, mir(rho), -mir(u), mir(v), mir(p))
+    scale = max(np.abs(x[inner]).max() for x in d0)
+    sign = [1, -1, 1, 1]
+    for k in range(4):
+        diff = np.abs(dm[k][inner] - sign[k] * d0[k][inner][::-1, :]).max()
+        assert diff <= 1e-11 * scale, (k, diff, scale)
+
+    def tr(a):
+        b = a.copy(order="F")
+        b[inner] = a[inner].T
+        return b
+
+    dt_ = oracle.solve(cfg, g, tr(rho), tr(v), tr(u), tr(p))
+    perm = [0, 2, 1, 3]
+    for k in range(4):
+        diff = np.abs(dt_[k][inner] - d0[perm[k]][inner].T).max()
+        assert diff <= 1e-11 * scale, (k, diff, scale)
+
+
+@pytest.mark.parametrize("flux", FLUXES)
+def test_conservation_on_periodic_grid(oracle, flux):
+    cfg = oracle.Config(16, 12, flux=flux, limiter="minmod", bcs=("periodic",) * 4)
+    nodes = oracle.uniform_nodes(16, 12, cfg.nh, (0, 1), (0, 0.75))
+    g = oracle.Geometry(cfg, *nodes)
+    rho, u, v, p = random_periodic(cfg, 9)
+    d = oracle.solve(cfg, g, rho, u, v, p)
+    nh = cfg.nh
+    inner = (slice(nh, -nh), slice(nh, -nh))
+    for x in d:
+        tot = np.sum(x[inner] * g.volume[inner])
+        mag = np.sum(np.abs(x[inner] * g.volume[inner]))
+        assert abs(tot) <= 1e-13 * mag + 1e-300
+
+
+@pytest.mark.parametrize("flux", ["AUSM+", "SLAU", "AUSMPW+", "Roe", "HLLC"])
+def test_sod_against_exact_riemann(oracle, flux):
+    """Accuracy (not parity): density L1 error of Sod at t = 0.2 on 400 cells."""
+    ni, nj = 400, 4
+    cfg = oracle.Config(ni, nj, flux=flux, limiter="minmod", integrator="ssp_rk2",
+                        bcs=("zero_gradient", "zero_gradient", "periodic", "periodic"), cfl=0.4)
+    nx, ny = oracle.uniform_nodes(ni, nj, cfg.nh, (0, 1), (0, 0.01))
+    g = oracle.Geometry(cfg, nx, ny)
+    xc = 0.5 * (nx[:-1, :-1] + nx[1:, :-1])
+    rho = np.asfortranarray(np.where(xc < 0.5, 1.0, 0.125))
+    p = np.asfortranarray(np.where(xc < 0.5, 1.0, 0.1))
+    z = cfg.field()
+    f = oracle.Fluid(cfg, g, rho, z, z.copy(order="F"), p)
+    n, t, err = f.run(0.2, contour_interval_dt=0.2)
+    assert err == 0 and t == 0.2 and n > 100
+    xi = xc[cfg.nh:-cfg.nh, cfg.nh]
+    ex = sample(xi, 0.2, 0.5, 1.0, 0.0, 1.0, 0.125, 0.0, 0.1)
+    r = f.interior("rho")
+    l1 = np.mean(np.abs(r[:, 0] - ex[0]))
+    assert l1 < 0.012, l1
+    assert np.abs(r - r[:, :1]).max() < 1e-8  # 1-D problem stays 1-D
+
+
+def test_run_loop_terminates_on_clipped_time(oracle):
+    """main.f90:178-194: the driver time lands on the contour cadence exactly and the loop
+    stops there; the state advances by whole CFL steps."""
+    ni, nj = 50, 4
+    cfg = oracle.Config(ni, nj, flux="SLAU", limiter="minmod", integrator="ssp_rk2",
+                        bcs=("zero_gradient", "zero_gradient", "periodic", "periodic"), cfl=0.4)
+    nx, ny = oracle.uniform_nodes(ni, nj, cfg.nh, (0, 1), (0, 0.08))
+    g = oracle.Geometry(cfg, nx, ny)
+    xc = 0.5 * (nx[:-1, :-1] + nx[1:, :-1])
+    rho = np.asfortranarray(np.where(xc < 0.5, 1.0, 0.125))
+    p = np.asfortranarray(np.where(xc < 0.5, 1.0, 0.1))
+    z = cfg.field()
+    f = oracle.Fluid(cfg, g, rho, z, z.copy(order="F"), p)
+    n, t, err = f.run(0.05, contour_interval_dt=0.01)
+    assert err == 0
+    assert t == 0.01 + 0.01 + 0.01 + 0.01 + 0.01 or abs(t - 0.05) < 1e-15
+    f2 = oracle.Fluid(cfg, g, rho, z, z.copy(order="F"), p)
+    n2 = 0
+    tt = 0.0
+    while tt < 0.05:  # without the clip: plain accumulation of CFL steps
+        dt = f2.get_timestep()
+        f2.integrate(dt)
+        tt += dt
+        n2 += 1
+    assert n <= n2 + 5
