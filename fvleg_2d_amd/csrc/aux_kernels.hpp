@@ -18,7 +18,7 @@ struct FieldGeom {
 };
 
 // equation_of_state.f90:297-324 on the interior.
-__global__ void k_prim_to_cons(FieldGeom g, const double *__restrict__ rho, const double *__restrict__ u,
+static __global__ void k_prim_to_cons(FieldGeom g, const double *__restrict__ rho, const double *__restrict__ u,
                                const double *__restrict__ v, const double *__restrict__ p, double *__restrict__ U0,
                                double *__restrict__ U1, double *__restrict__ U2, double *__restrict__ U3) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -38,7 +38,7 @@ __global__ void k_prim_to_cons(FieldGeom g, const double *__restrict__ rho, cons
 }
 
 // fluid_t%calculate_derived_quantities for download (u, v, p, cs).
-__global__ void k_derived(FieldGeom g, const double *__restrict__ U0, const double *__restrict__ U1,
+static __global__ void k_derived(FieldGeom g, const double *__restrict__ U0, const double *__restrict__ U1,
                           const double *__restrict__ U2, const double *__restrict__ U3, double *__restrict__ u,
                           double *__restrict__ v, double *__restrict__ p, double *__restrict__ cs) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -54,7 +54,7 @@ __global__ void k_derived(FieldGeom g, const double *__restrict__ U0, const doub
 }
 
 // Sound speed of stored primitives (equation_of_state.f90:250-251).
-__global__ void k_sound_speed(FieldGeom g, const double *__restrict__ rho, const double *__restrict__ p,
+static __global__ void k_sound_speed(FieldGeom g, const double *__restrict__ rho, const double *__restrict__ p,
                               double *__restrict__ cs) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   int j = blockIdx.y;
@@ -66,7 +66,7 @@ __global__ void k_sound_speed(FieldGeom g, const double *__restrict__ rho, const
 // CFL partial minima over the interior (get_timestep, mod_fluid.f90:554-558).
 // from_prims: u, v unfiltered and cs from (rho, p) as after fluid_t%initialize;
 // otherwise the primitives are recovered from the conserved state.
-__global__ __launch_bounds__(kThreads) void k_cfl_partial(FieldGeom g, int from_prims, const double *__restrict__ a0,
+static __global__ __launch_bounds__(kThreads) void k_cfl_partial(FieldGeom g, int from_prims, const double *__restrict__ a0,
                                                           const double *__restrict__ a1, const double *__restrict__ a2,
                                                           const double *__restrict__ a3, double *__restrict__ block_min,
                                                           int *__restrict__ block_err) {
@@ -126,10 +126,38 @@ __device__ inline int err_code(int flags) {
   return 0;
 }
 
+// Time-loop bookkeeping after a reduction of CFL minimum m and sanity flags e.
+// mode 1: initialise (first dt); mode 2: end of a step of main.f90:138-198
+// (master%time += dt; contour cadence; dt clip of the driver time).
+__device__ inline void loop_update(LoopState *L, double m, int e, int mode) {
+  if (mode == 1) {
+    L->dt = (L->constant_dt > 0.0) ? L->constant_dt : m;
+    if (!(L->time < L->max_time && L->iteration < L->max_iterations)) L->done = 1;
+  } else if (mode == 2) {
+    const double EPS = Lim<double>::eps;
+    L->master_time = L->master_time + L->dt;
+    double new_time = L->master_time;
+    if (absd(L->time - L->next_output_time) < EPS) L->next_output_time = L->next_output_time + L->contour_dt;
+    if (new_time > L->next_output_time) {
+      double delta_t = absd(L->next_output_time - L->time);
+      new_time = L->time + delta_t;
+    }
+    L->time = new_time;
+    L->iteration = L->iteration + 1;
+    int code = err_code(e);
+    if (code) {
+      L->error = code;
+      L->done = 1;
+    }
+    L->dt = (L->constant_dt > 0.0) ? L->constant_dt : m;
+    if (!(L->time < L->max_time && L->iteration < L->max_iterations)) L->done = 1;
+  }
+}
+
 // Reduce per-workgroup partials into out[0] = min dt, out[1] = flags; clears flags.
 // mode 0: plain reduction into out; mode 1: initialise the loop (first dt);
 // mode 2: end of a step of the main.f90 time loop.
-__global__ __launch_bounds__(1024) void k_reduce_finalize(int nb, double *__restrict__ block_min,
+static __global__ __launch_bounds__(1024) void k_reduce_finalize(int nb, double *__restrict__ block_min,
                                                           int *__restrict__ block_err, double *__restrict__ out,
                                                           LoopState *__restrict__ L, int mode) {
   __shared__ double sm[16];
@@ -162,39 +190,17 @@ __global__ __launch_bounds__(1024) void k_reduce_finalize(int nb, double *__rest
     out[0] = m;
     out[1] = (double)e;
   }
-  if (mode == 1) {
-    L->dt = (L->constant_dt > 0.0) ? L->constant_dt : m;
-    if (!(L->time < L->max_time && L->iteration < L->max_iterations)) L->done = 1;
-  } else if (mode == 2) {
-    // main.f90:167-196 (master%time += dt; contour cadence; dt clip on the driver time)
-    const double EPS = Lim<double>::eps;
-    L->master_time = L->master_time + L->dt;
-    double new_time = L->master_time;
-    if (absd(L->time - L->next_output_time) < EPS) L->next_output_time = L->next_output_time + L->contour_dt;
-    if (new_time > L->next_output_time) {
-      double delta_t = absd(L->next_output_time - L->time);
-      new_time = L->time + delta_t;
-    }
-    L->time = new_time;
-    L->iteration = L->iteration + 1;
-    int code = err_code(e);
-    if (code) {
-      L->error = code;
-      L->done = 1;
-    }
-    L->dt = (L->constant_dt > 0.0) ? L->constant_dt : m;
-    if (!(L->time < L->max_time && L->iteration < L->max_iterations)) L->done = 1;
-  }
+  loop_update(L, m, e, mode);
 }
 
-__global__ void k_set_scalar(double *dst, double value) { *dst = value; }
+static __global__ void k_set_scalar(double *dst, double value) { *dst = value; }
 
 // --------------------------- per-operator kernels --------------------------
 // Halo'd column-major arrays: element (i, j), i,j in [1-nh, n+nh], at
 // (i+nh-1) + (j+nh-1)*ld.
 
 // apply_primitive_bc: every ghost cell takes the value of its mapped interior cell.
-__global__ void k_apply_bc(FieldGeom g, int ld, double *rho, double *u, double *v, double *p) {
+static __global__ void k_apply_bc(FieldGeom g, int ld, double *rho, double *u, double *v, double *p) {
   int nx = g.ni + 2 * g.nh, ny = g.nj + 2 * g.nh;
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nx * ny) return;
@@ -215,7 +221,7 @@ __global__ void k_apply_bc(FieldGeom g, int ld, double *rho, double *u, double *
 // muscl_interpolation_t%interpolate_edge_values on a halo'd field q:
 // edges (2, 0:ni, 0:nj) for both directions.
 template <typename R, int LIMC>
-__global__ void k_interpolate(FieldGeom g, int ld, const double *__restrict__ q, double *__restrict__ ie,
+static __global__ void k_interpolate(FieldGeom g, int ld, const double *__restrict__ q, double *__restrict__ ie,
                               double *__restrict__ je) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   int n = (g.ni + 1) * (g.nj + 1);
@@ -244,7 +250,7 @@ __global__ void k_interpolate(FieldGeom g, int ld, const double *__restrict__ q,
 
 // get_<dir>flux on given edge states; p is the halo'd cell pressure.
 template <typename R, int FLUX>
-__global__ void k_edge_flux(FieldGeom g, int ld, const double *__restrict__ edges, const double *__restrict__ p,
+static __global__ void k_edge_flux(FieldGeom g, int ld, const double *__restrict__ edges, const double *__restrict__ p,
                             int low_mach_fix, double *__restrict__ iflux, double *__restrict__ jflux) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   int nI = (g.ni + 1) * g.nj, nJ = g.ni * (g.nj + 1);

@@ -1,0 +1,111 @@
+// Batched form: many independent solves (any grid size, flux, limiter) advanced
+// by ONE persistent launch per RK stage.  Each workgroup walks a contiguous
+// slice of the (solve, tile) work list of its XCD and dispatches the solve's
+// (flux, limiter) variant of the fused stage tile; all variants share one LDS
+// buffer sized for the largest.  Solves whose time loop is done are skipped.
+#pragma once
+
+#include "aux_kernels.hpp"
+
+namespace cato {
+
+struct BatchItem {
+  int solve;
+  int tile;
+};
+
+struct BatchSolveRef {
+  double *block_min;
+  int *block_err;
+  LoopState *loop;
+  int ntiles;
+  int pad;
+};
+
+struct BatchStageArgs {
+  const StageArgs *args;   // per solve, this stage
+  const int *variant;      // per solve: flux * LIM_COUNT + limiter
+  const BatchItem *items;  // work list, grouped by solve
+  int nitems;
+};
+
+template <typename R, int TX, int TY> constexpr size_t batch_smem_bytes() {
+  size_t a = sizeof(TileSmem<LIM_MINMOD, R, TX, TY>);
+  size_t b = sizeof(TileSmem<LIM_MLP5, R, TX, TY>);
+  return a > b ? a : b;
+}
+
+#define CATO_BATCH_CASE(F, L)                                                                                 \
+  case (F) * LIM_COUNT + (L):                                                                                 \
+    stage_tile<F, L, R, TX, TY>(a, it.tile, dt, *reinterpret_cast<TileSmem<L, R, TX, TY> *>(smem), red);    \
+    break;
+#define CATO_BATCH_FLUX(F)   \
+  CATO_BATCH_CASE(F, 0)      \
+  CATO_BATCH_CASE(F, 1)      \
+  CATO_BATCH_CASE(F, 2)      \
+  CATO_BATCH_CASE(F, 3)      \
+  CATO_BATCH_CASE(F, 4)
+
+template <typename R, int TX, int TY>
+__global__ __launch_bounds__(kThreads) void batch_stage_kernel(BatchStageArgs b) {
+  __shared__ __attribute__((aligned(16))) char smem[batch_smem_bytes<R, TX, TY>()];
+  __shared__ RedSmem red;
+  // XCD-aware split: blocks of XCD x take the x-th eighth of the work list.
+  const int G = gridDim.x;
+  const int x = blockIdx.x % kNumXcd, lb = blockIdx.x / kNumXcd;
+  const int nbx = (G - x + kNumXcd - 1) / kNumXcd;  // blocks on this XCD
+  const int lo = (int)(((long long)b.nitems * x) / kNumXcd), hi = (int)(((long long)b.nitems * (x + 1)) / kNumXcd);
+  for (int w = lo + lb; w < hi; w += nbx) {
+    const BatchItem it = b.items[w];
+    const StageArgs &a = b.args[it.solve];
+    if (a.loop && a.loop->done) continue;
+    const double dt = a.loop ? a.loop->dt : a.dt_value;
+    __syncthreads();  // the previous item's LDS is free
+    switch (b.variant[it.solve]) {
+      CATO_BATCH_FLUX(0)
+      CATO_BATCH_FLUX(1)
+      CATO_BATCH_FLUX(2)
+      CATO_BATCH_FLUX(3)
+      CATO_BATCH_FLUX(4)
+      CATO_BATCH_FLUX(5)
+      CATO_BATCH_FLUX(6)
+      CATO_BATCH_FLUX(7)
+      default: break;
+    }
+  }
+}
+
+#undef CATO_BATCH_CASE
+#undef CATO_BATCH_FLUX
+
+// Per-solve end-of-step reduction: one workgroup per solve.
+static __global__ __launch_bounds__(kThreads) void batch_finalize_kernel(const BatchSolveRef *refs, int mode) {
+  __shared__ double sm[kThreads / 64];
+  __shared__ int se[kThreads / 64];
+  const BatchSolveRef r = refs[blockIdx.x];
+  LoopState *L = r.loop;
+  if (mode == 2 && L->done) return;
+  double m = __builtin_inf();
+  int e = 0;
+  for (int k = threadIdx.x; k < r.ntiles; k += kThreads) {
+    double v = r.block_min[k];
+    m = (v < m) ? v : m;
+    e |= r.block_err[k];
+    r.block_err[k] = 0;
+  }
+  m = wave_min(m);
+  e = wave_or(e);
+  if ((threadIdx.x & 63) == 0) {
+    sm[threadIdx.x >> 6] = m;
+    se[threadIdx.x >> 6] = e;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int k = 0; k < kThreads / 64; ++k) {
+    m = (sm[k] < m) ? sm[k] : m;
+    e |= se[k];
+  }
+  loop_update(L, m, e, mode);
+}
+
+}  // namespace cato

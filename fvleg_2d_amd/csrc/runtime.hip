@@ -210,14 +210,14 @@ FieldGeom Solve::fgeom() const {
 }
 
 StageArgs Solve::stage_args(int mode, double a, double b, double c, int in_buf, int u0_buf, int out_buf, bool last,
-                            const LoopState *lp, double dt_value) const {
+                            const LoopState *lp, double dt_value, bool prims_from_array) const {
   StageArgs s{};
   s.ni = ni;
   s.nj = nj;
   s.nh = nh;
   s.pitch = pitch;
   s.mode = mode;
-  s.prim_from_array = (in_buf < 0) ? 1 : 0;
+  s.prim_from_array = prims_from_array ? 1 : 0;
   s.last_stage = last ? 1 : 0;
   s.low_mach_fix = cfg.apply_low_mach_fix;
   s.bc[0] = cfg.bc_plus_x;
@@ -255,16 +255,18 @@ StageArgs Solve::stage_args(int mode, double a, double b, double c, int in_buf, 
 std::vector<StageArgs> Solve::step_stages(const LoopState *lp, double dt_value) const {
   std::vector<StageArgs> st;
   const int n = cur, A = (cur + 1) % 3, B = (cur + 2) % 3;
-  const int in0 = prims_valid ? -1 : n;
+  // The first stage after cato_set_primitive takes its primitives from prim[]
+  // (fluid_t%initialize keeps the deck's u, v, p); U^n always comes from U[n].
+  const bool pfa = prims_valid;
   switch (cfg.integrator) {
     case CATO_RK_SSP2:
-      st.push_back(stage_args(MODE_FIRST, 0, 0, 1.0, in0, -1, A, false, lp, dt_value));
-      st.push_back(stage_args(MODE_COMBINE, 0.5, 0.5, 0.5, A, n, n, true, lp, dt_value));
+      st.push_back(stage_args(MODE_FIRST, 0, 0, 1.0, n, -1, A, false, lp, dt_value, pfa));
+      st.push_back(stage_args(MODE_COMBINE, 0.5, 0.5, 0.5, A, n, n, true, lp, dt_value, false));
       break;
     case CATO_RK_SSP3:
-      st.push_back(stage_args(MODE_FIRST, 0, 0, 1.0, in0, -1, A, false, lp, dt_value));
-      st.push_back(stage_args(MODE_COMBINE, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0, A, n, B, false, lp, dt_value));
-      st.push_back(stage_args(MODE_COMBINE, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, B, n, n, true, lp, dt_value));
+      st.push_back(stage_args(MODE_FIRST, 0, 0, 1.0, n, -1, A, false, lp, dt_value, pfa));
+      st.push_back(stage_args(MODE_COMBINE, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0, A, n, B, false, lp, dt_value, false));
+      st.push_back(stage_args(MODE_COMBINE, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, B, n, n, true, lp, dt_value, false));
       break;
     default: {
       static const double a13[3] = {0.1481, 0.4, 1.0}, a14[4] = {0.0833, 0.2069, 0.4265, 1.0},
@@ -280,17 +282,41 @@ std::vector<StageArgs> Solve::step_stages(const LoopState *lp, double dt_value) 
         case CATO_RK_2_4: al = a24; ns = 4; break;
         default: al = a25; ns = 5; break;
       }
-      int prev = in0;
+      int prev = n;
       for (int m = 0; m < ns; ++m) {
         bool last = (m == ns - 1);
         int out = last ? n : ((m % 2 == 0) ? A : B);
-        // first stage of the first step reads the stored primitives; U_0 is U^n either way
-        st.push_back(stage_args(MODE_BASE, 0, 0, al[m], prev, n, out, last, lp, dt_value));
+        st.push_back(stage_args(MODE_BASE, 0, 0, al[m], prev, n, out, last, lp, dt_value, pfa && m == 0));
         prev = out;
       }
     }
   }
   return st;
+}
+
+// Every pointer a stage launch will dereference must be set (checked before launch).
+int validate_stage_args(const StageArgs &s) {
+  auto need = [&](const double *const *p, const char *what) -> int {
+    for (int v = 0; v < 4; ++v)
+      if (!p[v]) return fail(CATO_ERR_INVALID, std::string("internal: stage launch without ") + what);
+    return CATO_OK;
+  };
+  int rc;
+  if ((rc = need(s.out, "output state"))) return rc;
+  if (s.prim_from_array) {
+    if ((rc = need(s.prim, "primitive arrays"))) return rc;
+  } else if ((rc = need(s.uk, "U_k state"))) {
+    return rc;
+  }
+  if (s.mode == MODE_FIRST && (rc = need(s.uk, "U_k state (first stage combine)"))) return rc;
+  if ((s.mode == MODE_COMBINE || s.mode == MODE_BASE) && (rc = need(s.u0, "U_0 state"))) return rc;
+  if (s.mode == MODE_COMBINE && (rc = need(s.uk, "U_k state"))) return rc;
+  if (!s.dxi || !s.lxi || !s.dyj || !s.lyj) return fail(CATO_ERR_INVALID, "internal: stage launch without geometry");
+  if (s.last_stage && (!s.block_min || !s.block_err))
+    return fail(CATO_ERR_INVALID, "internal: last stage without reduction buffers");
+  if (s.tiles_x <= 0 || s.tiles_y <= 0 || s.ni < s.nh || s.nj < s.nh || s.pitch < s.ni)
+    return fail(CATO_ERR_INVALID, "internal: inconsistent stage geometry");
+  return CATO_OK;
 }
 
 }  // namespace cato
@@ -364,7 +390,9 @@ int launch_stages(cato_ctx *c, const std::vector<StageArgs> &st) {
   Solve &s = c->s;
   dim3 grid((unsigned)(s.tiles_x * s.tiles_y));
   for (const StageArgs &a : st) {
-    int rc = timed(c, true, [&] { s.launcher(grid, c->stream, a); });
+    int rc = validate_stage_args(a);
+    if (rc) return rc;
+    rc = timed(c, true, [&] { s.launcher(grid, c->stream, a); });
     if (rc) return rc;
   }
   return CATO_OK;
@@ -794,10 +822,11 @@ int cato_residual(cato_ctx *c, double *rho, double *u, double *v, double *p, dou
   const double *P[4] = {rho, u, v, p};
   for (int k = 0; k < 4; ++k)
     if ((rc = upload_field(c, Pbuf[k], P[k], ld, s.nh))) return rc;
-  StageArgs a = s.stage_args(MODE_RESIDUAL, 0, 0, 0, -1, -1, (s.cur + 2) % 3, false, nullptr, 1.0);
+  StageArgs a = s.stage_args(MODE_RESIDUAL, 0, 0, 0, -1, -1, (s.cur + 2) % 3, false, nullptr, 1.0, true);
   for (int k = 0; k < 4; ++k) a.prim[k] = Pbuf[k];
   a.block_min = nullptr;
   a.block_err = nullptr;
+  if ((rc = validate_stage_args(a))) return rc;
   dim3 grid((unsigned)(s.tiles_x * s.tiles_y));
   s.launcher(grid, c->stream, a);
   HIPCK(hipGetLastError());

@@ -21,6 +21,7 @@ struct Geometry1D {
 };
 int build_geometry(int ni, int nj, int nh, const double *nx, const double *ny, Geometry1D &g);
 int default_tile(const cato_config &c, int ni, int nj);
+int validate_stage_args(const StageArgs &s);
 
 // One solve: configuration, geometry and the device arrays it owns.
 struct Solve {
@@ -47,7 +48,7 @@ struct Solve {
   int upload_geometry(hipStream_t s);
   FieldGeom fgeom() const;
   StageArgs stage_args(int mode, double a, double b, double c, int in_buf, int u0_buf, int out_buf, bool last,
-                       const LoopState *lp, double dt_value) const;
+                       const LoopState *lp, double dt_value, bool prims_from_array) const;
   std::vector<StageArgs> step_stages(const LoopState *lp, double dt_value) const;
 };
 

@@ -104,21 +104,35 @@ __device__ inline void recon_var(const R *base, int sn, int st, R &L, R &Rv) {
   }
 }
 
+// LDS image of one tile: primitives with halo, i-edge and j-edge fluxes.
+template <int LIMC, typename R, int TX, int TY> struct TileSmem {
+  static constexpr int H = halo_of<LIMC>();
+  static constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
+  R sp[4][RY][RX];
+  R sfi[4][TY][TX + 1];
+  R sfj[4][TY + 1][TX];
+};
+
+// Block-wide scratch for the CFL / sanity reduction.
+struct RedSmem {
+  double m[kThreads / 64];
+  int e[kThreads / 64];
+};
+
+// One tile of one stage.  Writes the block's CFL partial and sanity flags to
+// a.block_min[tile] / a.block_err[tile] when requested.
 template <int FLUX, int LIMC, typename R, int TX, int TY>
-__global__ __launch_bounds__(kThreads) void stage_kernel(StageArgs a) {
+__device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileSmem<LIMC, R, TX, TY> &sm,
+                                  RedSmem &red) {
   constexpr int H = halo_of<LIMC>();
   constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
   constexpr int NIE = (TX + 1) * TY, NJE = TX * (TY + 1);
-  __shared__ R sp[4][RY][RX];
-  __shared__ R sfi[4][TY][TX + 1];
-  __shared__ R sfj[4][TY + 1][TX];
-  __shared__ double red_min[kThreads / 64];
-  __shared__ int red_err[kThreads / 64];
-
-  if (a.loop && a.loop->done) return;
-  const double dt = a.loop ? a.loop->dt : a.dt_value;
+  auto &sp = sm.sp;
+  auto &sfi = sm.sfi;
+  auto &sfj = sm.sfj;
+  auto &red_min = red.m;
+  auto &red_err = red.e;
   const int tid = threadIdx.x;
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
   const int i0 = tx * TX + 1, j0 = ty * TY + 1;
   const int ni = a.ni, nj = a.nj, nh = a.nh;
@@ -257,10 +271,20 @@ __global__ __launch_bounds__(kThreads) void stage_kernel(StageArgs a) {
         m = (red_min[k] < m) ? red_min[k] : m;
         e |= red_err[k];
       }
-      if (a.block_min) a.block_min[blockIdx.x] = m;
-      if (a.block_err) a.block_err[blockIdx.x] = e;
+      if (a.block_min) a.block_min[tile] = m;
+      if (a.block_err && e) a.block_err[tile] |= e;
     }
   }
+}
+
+// Single-solve stage launch: one workgroup per tile, XCD-aware tile order.
+template <int FLUX, int LIMC, typename R, int TX, int TY>
+__global__ __launch_bounds__(kThreads) void stage_kernel(StageArgs a) {
+  __shared__ TileSmem<LIMC, R, TX, TY> sm;
+  __shared__ RedSmem red;
+  if (a.loop && a.loop->done) return;
+  const double dt = a.loop ? a.loop->dt : a.dt_value;
+  stage_tile<FLUX, LIMC, R, TX, TY>(a, xcd_tile(blockIdx.x, gridDim.x), dt, sm, red);
 }
 
 }  // namespace cato

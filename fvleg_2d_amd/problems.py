@@ -120,3 +120,26 @@ def kelvin_helmholtz(n=2048, seed=7, flux="SLAU", limiter="van_leer", integrator
     p = np.full_like(xc, 2.5)
     return Problem(f"kelvin_helmholtz_{n}x{n}", cfg, nx, ny, _f(rho), _f(u), _f(v), _f(p), max_time=1e300,
                    max_iterations=steps, contour_interval_dt=1e300, meta={"seed": seed})
+
+
+BATCH_FLUXES = ("AUSM+", "AUSMPW+", "HLLC", "Roe")
+BATCH_LIMITERS = ("minmod", "MLP5")
+
+
+def batch_riemann(n_solves=256, seed=2024, side_range=(64, 512), max_time=0.2, precision="fp64",
+                  integrator="ssp_rk3"):
+    """Independent four-quadrant Riemann solves: square side log-uniform in side_range, quadrant
+    states as in riemann2d, flux from {AUSM+, AUSMPW+, HLLC, Roe}, reconstruction from
+    {MUSCL-minmod, MLP5}."""
+    rng = np.random.default_rng(seed)
+    lo, hi = side_range
+    out = []
+    for k in range(n_solves):
+        side = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        flux = BATCH_FLUXES[int(rng.integers(len(BATCH_FLUXES)))]
+        lim = BATCH_LIMITERS[int(rng.integers(len(BATCH_LIMITERS)))]
+        pb = riemann2d(n=side, seed=int(rng.integers(1 << 30)), flux=flux, limiter=lim, integrator=integrator,
+                       max_time=max_time, precision=precision)
+        pb.name = f"batch{k}_{side}_{flux}_{lim}"
+        out.append(pb)
+    return out

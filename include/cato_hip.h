@@ -160,6 +160,21 @@ int cato_edge_fluxes(cato_ctx *ctx, const double *edges, const double *p, int ld
 int cato_residual(cato_ctx *ctx, double *rho, double *u, double *v, double *p, double *d_rho, double *d_rho_u,
                   double *d_rho_v, double *d_rho_E, int ld);
 
+/* ---- batched form: many independent solves in one persistent launch per stage ---- */
+typedef struct cato_batch cato_batch;
+/* Each solve k has its own config and geometry; all solves share the precision and tile. */
+int cato_batch_create(int n, const cato_config *cfgs, const cato_geom *geoms, int precision, int tile,
+                      cato_batch **out);
+int cato_batch_destroy(cato_batch *b);
+int cato_batch_set_primitive(cato_batch *b, int k, const double *rho, const double *u, const double *v,
+                             const double *p, int ld, int halo);
+int cato_batch_get_conserved(cato_batch *b, int k, double *rho, double *rho_u, double *rho_v, double *rho_E, int ld,
+                             int halo);
+/* Run every solve to its own max_time (main.f90 semantics per solve); steps are
+ * replayed from a captured hipGraph.  iterations/times/errors may be NULL. */
+int cato_batch_run(cato_batch *b, const double *max_time, const double *contour_interval_dt, long long max_iterations,
+                   int use_graph, long long *iterations, double *times, int *errors);
+
 #ifdef __cplusplus
 }
 #endif
