@@ -71,7 +71,22 @@ SIGNATURES = {
     "cato_batch_get_conserved": (C.c_int, [_CTX, C.c_int, _DP, _DP, _DP, _DP, C.c_int, C.c_int]),
     "cato_batch_run": (C.c_int, [_CTX, _DP, _DP, C.c_longlong, C.c_int, C.POINTER(C.c_longlong), _DP,
                                  C.POINTER(C.c_int)]),
+    "cato_server_create": (C.c_int, [C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(_CTX)]),
+    "cato_server_destroy": (C.c_int, [_CTX]),
+    "cato_server_submit": (C.c_int, [_CTX, C.c_void_p, C.c_double]),
+    "cato_server_tick": (C.c_int, [_CTX, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "cato_server_poll": (C.c_int, [_CTX, C.c_int, C.POINTER(C.c_int), _DP, C.POINTER(C.c_longlong),
+                                   C.POINTER(C.c_int)]),
+    "cato_server_clock": (C.c_double, [_CTX]),
 }
+
+
+class CatoRequest(C.Structure):
+    _fields_ = [("cfg", CatoConfig), ("geom", CatoGeom),
+                ("rho", _DP), ("u", _DP), ("v", _DP), ("p", _DP),
+                ("max_time", C.c_double), ("contour_interval_dt", C.c_double),
+                ("out_rho", _DP), ("out_rho_u", _DP), ("out_rho_v", _DP), ("out_rho_E", _DP)]
+
 
 _lib = None
 

@@ -17,6 +17,15 @@ struct FieldGeom {
   const double *dxi, *dyj;
 };
 
+// Strided 2D copy on the device: dst(i, j) = src(i, j) for i < ni, j < nj with
+// leading dimensions lds / ldd (host transfers stay contiguous 1D copies).
+static __global__ void k_repack(const double *__restrict__ src, int lds, double *__restrict__ dst, int ldd, int ni) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y;
+  if (i >= ni) return;
+  dst[(size_t)j * ldd + i] = src[(size_t)j * lds + i];
+}
+
 // equation_of_state.f90:297-324 on the interior.
 static __global__ void k_prim_to_cons(FieldGeom g, const double *__restrict__ rho, const double *__restrict__ u,
                                const double *__restrict__ v, const double *__restrict__ p, double *__restrict__ U0,

@@ -2,7 +2,8 @@
 // by ONE persistent launch per RK stage.  Each workgroup walks a contiguous
 // slice of the (solve, tile) work list of its XCD and dispatches the solve's
 // (flux, limiter) variant of the fused stage tile; all variants share one LDS
-// buffer sized for the largest.  Solves whose time loop is done are skipped.
+// buffer sized for the largest.  Solves whose time loop is done are skipped,
+// and the host drops them from the work list at its next poll.
 #pragma once
 
 #include "aux_kernels.hpp"
@@ -26,7 +27,7 @@ struct BatchStageArgs {
   const StageArgs *args;   // per solve, this stage
   const int *variant;      // per solve: flux * LIM_COUNT + limiter
   const BatchItem *items;  // work list, grouped by solve
-  int nitems;
+  const int *count;        // number of live entries of items (device-resident: graph replays see updates)
 };
 
 template <typename R, int TX, int TY> constexpr size_t batch_smem_bytes() {
@@ -52,9 +53,10 @@ __global__ __launch_bounds__(kThreads) void batch_stage_kernel(BatchStageArgs b)
   __shared__ RedSmem red;
   // XCD-aware split: blocks of XCD x take the x-th eighth of the work list.
   const int G = gridDim.x;
+  const int nitems = *b.count;
   const int x = blockIdx.x % kNumXcd, lb = blockIdx.x / kNumXcd;
   const int nbx = (G - x + kNumXcd - 1) / kNumXcd;  // blocks on this XCD
-  const int lo = (int)(((long long)b.nitems * x) / kNumXcd), hi = (int)(((long long)b.nitems * (x + 1)) / kNumXcd);
+  const int lo = (int)(((long long)nitems * x) / kNumXcd), hi = (int)(((long long)nitems * (x + 1)) / kNumXcd);
   for (int w = lo + lb; w < hi; w += nbx) {
     const BatchItem it = b.items[w];
     const StageArgs &a = b.args[it.solve];

@@ -62,6 +62,8 @@ int BatchEngine::init(int n, const cato_config *cfgs, const cato_geom *geoms, in
   off += align256((size_t)nitems * sizeof(BatchItem));
   refs_off = off;
   off += align256((size_t)n * sizeof(BatchSolveRef));
+  count_off = off;
+  off += 256;
   total_bytes = off;
   BCK(hipMalloc(&mem, total_bytes));
   for (int k = 0; k < n; ++k) {
@@ -71,8 +73,10 @@ int BatchEngine::init(int n, const cato_config *cfgs, const cato_geom *geoms, in
     if (rc) return rc;
   }
   // work list grouped by solve, variants, finalize references
-  std::vector<BatchItem> items;
+  std::vector<BatchItem> &items = live_items;
+  items.clear();
   items.reserve(nitems);
+  live_solves = n;
   std::vector<int> variant(n);
   std::vector<BatchSolveRef> refs(n);
   for (int k = 0; k < n; ++k) {
@@ -86,6 +90,7 @@ int BatchEngine::init(int n, const cato_config *cfgs, const cato_geom *geoms, in
   BCK(hipMemcpyAsync(mem + items_off, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice,
                      stream));
   BCK(hipMemcpyAsync(mem + refs_off, refs.data(), n * sizeof(BatchSolveRef), hipMemcpyHostToDevice, stream));
+  BCK(hipMemcpyAsync(mem + count_off, &nitems, sizeof(int), hipMemcpyHostToDevice, stream));
   hipDeviceProp_t prop;
   int dev = 0;
   BCK(hipGetDevice(&dev));
@@ -127,7 +132,7 @@ int BatchEngine::upload_args() {
     tab[(size_t)nstages * n + k] = first[0];
   }
   BCK(hipMemcpyAsync(mem + args_off, tab.data(), tab.size() * sizeof(StageArgs), hipMemcpyHostToDevice, stream));
-  return CATO_OK;
+  return hip_check(hipStreamSynchronize(stream), "batch args");  // tab is pageable and local
 }
 
 int BatchEngine::launch_step(bool first) {
@@ -138,7 +143,7 @@ int BatchEngine::launch_step(bool first) {
     b.args = (first && s == 0) ? args + (size_t)nstages * n : args + (size_t)s * n;
     b.variant = reinterpret_cast<const int *>(mem + variant_off);
     b.items = reinterpret_cast<const BatchItem *>(mem + items_off);
-    b.nitems = nitems;
+    b.count = reinterpret_cast<const int *>(mem + count_off);
     launcher(dim3((unsigned)grid), stream, b);
     BCK(hipGetLastError());
   }
@@ -146,6 +151,30 @@ int BatchEngine::launch_step(bool first) {
                      reinterpret_cast<const BatchSolveRef *>(mem + refs_off), 2);
   BCK(hipGetLastError());
   return CATO_OK;
+}
+
+// Drop finished solves from the work list so the persistent grid (split in
+// contiguous eighths over the XCDs) stays balanced as solves end at different
+// steps.  Called after a stream synchronisation; the copies are ordered before
+// the next replay on the same stream.
+int BatchEngine::compact(const std::vector<LoopState> &H) {
+  const int n = (int)solves.size();
+  int live = 0;
+  for (int k = 0; k < n; ++k) live += H[k].done ? 0 : 1;
+  if (live == live_solves) return CATO_OK;
+  live_solves = live;
+  live_items.clear();
+  for (int k = 0; k < n; ++k) {
+    if (H[k].done) continue;
+    int nt = solves[k].tiles_x * solves[k].tiles_y;
+    for (int t = 0; t < nt; ++t) live_items.push_back({k, t});
+  }
+  live_count = (int)live_items.size();
+  if (live_count)
+    BCK(hipMemcpyAsync(mem + items_off, live_items.data(), live_items.size() * sizeof(BatchItem),
+                       hipMemcpyHostToDevice, stream));
+  BCK(hipMemcpyAsync(mem + count_off, &live_count, sizeof(int), hipMemcpyHostToDevice, stream));
+  return hip_check(hipStreamSynchronize(stream), "batch compact");
 }
 
 int BatchEngine::run(const double *max_time, const double *contour_dt, long long max_iterations, int use_graph,
@@ -162,8 +191,9 @@ int BatchEngine::run(const double *max_time, const double *contour_dt, long long
     L[k].max_iterations = max_iterations;
   }
   BCK(hipMemcpyAsync(mem + loops_off, L.data(), n * sizeof(LoopState), hipMemcpyHostToDevice, stream));
-  int rc = upload_args();
+  int rc = upload_args();  // synchronises, so L may go
   if (rc) return rc;
+  if ((rc = compact(L))) return rc;  // every solve live again (a batch may be run more than once)
   // first dt of every solve (from the caller's primitives or the stored state)
   for (int k = 0; k < n; ++k) {
     Solve &sv = solves[k];
@@ -208,6 +238,7 @@ int BatchEngine::run(const double *max_time, const double *contour_dt, long long
     bool all = true;
     for (int k = 0; k < n; ++k) all = all && H[k].done;
     if (all) break;
+    if ((rc = compact(H))) return rc;
   }
   for (int k = 0; k < n; ++k) {
     if (iters) iters[k] = H[k].iteration;
@@ -263,27 +294,15 @@ static int batch_solve(cato_batch *b, int k, Solve **s) {
   return CATO_OK;
 }
 
-static int copy_field(hipStream_t st, const Solve &s, double *dev, const double *host, int ld, int halo, bool up) {
-  if (!host) return up ? fail(CATO_ERR_INVALID, "null field pointer") : CATO_OK;
-  if (halo < 0 || ld < s.ni + 2 * halo) return fail(CATO_ERR_INVALID, "bad leading dimension / halo");
-  size_t o = (size_t)halo + (size_t)halo * ld;
-  if (up)
-    return hip_check(hipMemcpy2DAsync(dev, (size_t)s.pitch * 8, host + o, (size_t)ld * 8, (size_t)s.ni * 8,
-                                      (size_t)s.nj, hipMemcpyHostToDevice, st),
-                     "upload");
-  return hip_check(hipMemcpy2DAsync(const_cast<double *>(host) + o, (size_t)ld * 8, dev, (size_t)s.pitch * 8,
-                                    (size_t)s.ni * 8, (size_t)s.nj, hipMemcpyDeviceToHost, st),
-                   "download");
-}
-
 int cato_batch_set_primitive(cato_batch *b, int k, const double *rho, const double *u, const double *v,
                              const double *p, int ld, int halo) {
   Solve *s;
   int rc = batch_solve(b, k, &s);
   if (rc) return rc;
   const double *P[4] = {rho, u, v, p};
+  double *stg = s->U[(s->cur + 1) % 3][0];
   for (int q = 0; q < 4; ++q)
-    if ((rc = copy_field(b->stream, *s, s->prim[q], P[q], ld, halo, true))) return rc;
+    if ((rc = field_up(b->stream, *s, s->prim[q], P[q], ld, halo, stg))) return rc;
   FieldGeom g = s->fgeom();
   hipLaunchKernelGGL(k_prim_to_cons, dim3((unsigned)((s->ni + 255) / 256), (unsigned)s->nj), dim3(256), 0, b->stream,
                      g, s->prim[0], s->prim[1], s->prim[2], s->prim[3], s->U[s->cur][0], s->U[s->cur][1],
@@ -299,8 +318,9 @@ int cato_batch_get_conserved(cato_batch *b, int k, double *rho, double *rho_u, d
   int rc = batch_solve(b, k, &s);
   if (rc) return rc;
   double *H[4] = {rho, rho_u, rho_v, rho_E};
+  double *stg = s->U[(s->cur + 1) % 3][0];
   for (int q = 0; q < 4; ++q)
-    if ((rc = copy_field(b->stream, *s, s->U[s->cur][q], H[q], ld, halo, false))) return rc;
+    if (H[q] && (rc = field_down(b->stream, *s, H[q], s->U[s->cur][q], ld, halo, stg))) return rc;
   return hip_check(hipStreamSynchronize(b->stream), "batch get_conserved");
 }
 

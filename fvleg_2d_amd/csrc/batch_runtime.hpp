@@ -16,7 +16,9 @@ struct BatchEngine {
   hipStream_t stream = nullptr;
   char *mem = nullptr;
   size_t total_bytes = 0;
-  size_t loops_off = 0, args_off = 0, variant_off = 0, items_off = 0, refs_off = 0;
+  size_t loops_off = 0, args_off = 0, variant_off = 0, items_off = 0, refs_off = 0, count_off = 0;
+  std::vector<BatchItem> live_items;  // host copy of the current work list
+  int live_solves = 0, live_count = 0;
   int precision = 0, tile = 0, nstages = 0, nitems = 0, grid = 0;
   BatchLauncher launcher = nullptr;
   hipGraph_t graph = nullptr;
@@ -26,6 +28,7 @@ struct BatchEngine {
   void release();
   int upload_args();
   int launch_step(bool first);
+  int compact(const std::vector<LoopState> &H);
   int run(const double *max_time, const double *contour_dt, long long max_iterations, int use_graph,
           long long *iters, double *times, int *errors);
 };

@@ -294,6 +294,46 @@ std::vector<StageArgs> Solve::step_stages(const LoopState *lp, double dt_value) 
   return st;
 }
 
+int field_up(hipStream_t st, const Solve &s, double *dev, const double *host, int ld, int halo, double *staging) {
+  if (!host) return fail(CATO_ERR_INVALID, "null field pointer");
+  if (halo < 0 || ld < s.ni + 2 * halo) return fail(CATO_ERR_INVALID, "bad leading dimension / halo");
+  const size_t n = (size_t)s.ni * s.nj;
+  if (ld == s.ni && halo == 0) {
+    if (s.pitch == s.ni) return hip_check(hipMemcpyAsync(dev, host, n * 8, hipMemcpyHostToDevice, st), "upload");
+    if (staging) {
+      int rc = hip_check(hipMemcpyAsync(staging, host, n * 8, hipMemcpyHostToDevice, st), "upload");
+      if (rc) return rc;
+      hipLaunchKernelGGL(k_repack, dim3((unsigned)((s.ni + 255) / 256), (unsigned)s.nj), dim3(256), 0, st,
+                         (const double *)staging, s.ni, dev, s.pitch, s.ni);
+      return hip_check(hipGetLastError(), "repack");
+    }
+  }
+  const double *p = host + (size_t)halo + (size_t)halo * ld;
+  return hip_check(hipMemcpy2DAsync(dev, (size_t)s.pitch * 8, p, (size_t)ld * 8, (size_t)s.ni * 8, (size_t)s.nj,
+                                    hipMemcpyHostToDevice, st),
+                   "upload");
+}
+
+int field_down(hipStream_t st, const Solve &s, double *host, const double *dev, int ld, int halo, double *staging) {
+  if (!host) return fail(CATO_ERR_INVALID, "null field pointer");
+  if (halo < 0 || ld < s.ni + 2 * halo) return fail(CATO_ERR_INVALID, "bad leading dimension / halo");
+  const size_t n = (size_t)s.ni * s.nj;
+  if (ld == s.ni && halo == 0) {
+    if (s.pitch == s.ni) return hip_check(hipMemcpyAsync(host, dev, n * 8, hipMemcpyDeviceToHost, st), "download");
+    if (staging) {
+      hipLaunchKernelGGL(k_repack, dim3((unsigned)((s.ni + 255) / 256), (unsigned)s.nj), dim3(256), 0, st, dev,
+                         s.pitch, staging, s.ni, s.ni);
+      int rc = hip_check(hipGetLastError(), "repack");
+      if (rc) return rc;
+      return hip_check(hipMemcpyAsync(host, staging, n * 8, hipMemcpyDeviceToHost, st), "download");
+    }
+  }
+  double *p = host + (size_t)halo + (size_t)halo * ld;
+  return hip_check(hipMemcpy2DAsync(p, (size_t)ld * 8, dev, (size_t)s.pitch * 8, (size_t)s.ni * 8, (size_t)s.nj,
+                                    hipMemcpyDeviceToHost, st),
+                   "download");
+}
+
 // Every pointer a stage launch will dereference must be set (checked before launch).
 int validate_stage_args(const StageArgs &s) {
   auto need = [&](const double *const *p, const char *what) -> int {
@@ -430,23 +470,12 @@ int launch_cfl(cato_ctx *c, int loop_mode) {
 int check_ctx(cato_ctx *c) { return c ? CATO_OK : fail(CATO_ERR_INVALID, "null context"); }
 
 // Host <-> device copies of interior fields with (ld, halo) host layout.
-int upload_field(cato_ctx *c, double *dst, const double *src, int ld, int halo) {
-  const Solve &s = c->s;
-  if (!src) return fail(CATO_ERR_INVALID, "null field pointer");
-  if (halo < 0 || ld < s.ni + 2 * halo) return fail(CATO_ERR_INVALID, "bad leading dimension / halo");
-  const double *p = src + (size_t)halo + (size_t)halo * ld;
-  return hip_check(hipMemcpy2DAsync(dst, (size_t)s.pitch * 8, p, (size_t)ld * 8, (size_t)s.ni * 8, (size_t)s.nj,
-                                    hipMemcpyHostToDevice, c->stream),
-                   "upload");
+int upload_field(cato_ctx *c, double *dst, const double *src, int ld, int halo, double *staging) {
+  return field_up(c->stream, c->s, dst, src, ld, halo, staging);
 }
-int download_field(cato_ctx *c, double *dst, const double *src, int ld, int halo) {
-  const Solve &s = c->s;
+int download_field(cato_ctx *c, double *dst, const double *src, int ld, int halo, double *staging) {
   if (!dst) return CATO_OK;
-  if (halo < 0 || ld < s.ni + 2 * halo) return fail(CATO_ERR_INVALID, "bad leading dimension / halo");
-  double *p = dst + (size_t)halo + (size_t)halo * ld;
-  return hip_check(hipMemcpy2DAsync(p, (size_t)ld * 8, src, (size_t)s.pitch * 8, (size_t)s.ni * 8, (size_t)s.nj,
-                                    hipMemcpyDeviceToHost, c->stream),
-                   "download");
+  return field_down(c->stream, c->s, dst, src, ld, halo, staging);
 }
 
 dim3 field_grid(const Solve &s) { return dim3((unsigned)((s.ni + 255) / 256), (unsigned)s.nj); }
@@ -532,8 +561,9 @@ int cato_set_primitive(cato_ctx *c, const double *rho, const double *u, const do
   int rc = check_ctx(c);
   if (rc) return rc;
   Solve &s = c->s;
-  if ((rc = upload_field(c, s.prim[0], rho, ld, halo)) || (rc = upload_field(c, s.prim[1], u, ld, halo)) ||
-      (rc = upload_field(c, s.prim[2], v, ld, halo)) || (rc = upload_field(c, s.prim[3], p, ld, halo)))
+  double *stg = s.U[(s.cur + 1) % 3][0];
+  if ((rc = upload_field(c, s.prim[0], rho, ld, halo, stg)) || (rc = upload_field(c, s.prim[1], u, ld, halo, stg)) ||
+      (rc = upload_field(c, s.prim[2], v, ld, halo, stg)) || (rc = upload_field(c, s.prim[3], p, ld, halo, stg)))
     return rc;
   FieldGeom g = s.fgeom();
   hipLaunchKernelGGL(k_prim_to_cons, field_grid(s), dim3(256), 0, c->stream, g, s.prim[0], s.prim[1], s.prim[2],
@@ -549,8 +579,9 @@ int cato_set_conserved(cato_ctx *c, const double *r, const double *ru, const dou
   if (rc) return rc;
   Solve &s = c->s;
   double **U = s.U[s.cur];
-  if ((rc = upload_field(c, U[0], r, ld, halo)) || (rc = upload_field(c, U[1], ru, ld, halo)) ||
-      (rc = upload_field(c, U[2], rv, ld, halo)) || (rc = upload_field(c, U[3], rE, ld, halo)))
+  double *stg = s.U[(s.cur + 1) % 3][0];
+  if ((rc = upload_field(c, U[0], r, ld, halo, stg)) || (rc = upload_field(c, U[1], ru, ld, halo, stg)) ||
+      (rc = upload_field(c, U[2], rv, ld, halo, stg)) || (rc = upload_field(c, U[3], rE, ld, halo, stg)))
     return rc;
   s.prims_valid = false;
   return hip_check(hipStreamSynchronize(c->stream), "set_conserved");
@@ -561,8 +592,9 @@ int cato_get_conserved(cato_ctx *c, double *r, double *ru, double *rv, double *r
   if (rc) return rc;
   Solve &s = c->s;
   double **U = s.U[s.cur];
-  if ((rc = download_field(c, r, U[0], ld, halo)) || (rc = download_field(c, ru, U[1], ld, halo)) ||
-      (rc = download_field(c, rv, U[2], ld, halo)) || (rc = download_field(c, rE, U[3], ld, halo)))
+  double *stg = s.U[(s.cur + 1) % 3][0];
+  if ((rc = download_field(c, r, U[0], ld, halo, stg)) || (rc = download_field(c, ru, U[1], ld, halo, stg)) ||
+      (rc = download_field(c, rv, U[2], ld, halo, stg)) || (rc = download_field(c, rE, U[3], ld, halo, stg)))
     return rc;
   return hip_check(hipStreamSynchronize(c->stream), "get_conserved");
 }
@@ -578,18 +610,21 @@ int cato_get_primitive(cato_ctx *c, double *rho, double *u, double *v, double *p
   if (s.prims_valid) {
     hipLaunchKernelGGL(k_sound_speed, field_grid(s), dim3(256), 0, c->stream, g, s.prim[0], s.prim[3], T2[0]);
     HIPCK(hipGetLastError());
-    if ((rc = download_field(c, rho, s.prim[0], ld, halo)) || (rc = download_field(c, u, s.prim[1], ld, halo)) ||
-        (rc = download_field(c, v, s.prim[2], ld, halo)) || (rc = download_field(c, p, s.prim[3], ld, halo)) ||
-        (rc = download_field(c, cs, T2[0], ld, halo)))
+    double *stg = T[0];
+    if ((rc = download_field(c, rho, s.prim[0], ld, halo, stg)) ||
+        (rc = download_field(c, u, s.prim[1], ld, halo, stg)) ||
+        (rc = download_field(c, v, s.prim[2], ld, halo, stg)) ||
+        (rc = download_field(c, p, s.prim[3], ld, halo, stg)) || (rc = download_field(c, cs, T2[0], ld, halo, stg)))
       return rc;
   } else {
     double **U = s.U[s.cur];
     hipLaunchKernelGGL(k_derived, field_grid(s), dim3(256), 0, c->stream, g, U[0], U[1], U[2], U[3], T[0], T[1], T[2],
                        T[3]);
     HIPCK(hipGetLastError());
-    if ((rc = download_field(c, rho, U[0], ld, halo)) || (rc = download_field(c, u, T[0], ld, halo)) ||
-        (rc = download_field(c, v, T[1], ld, halo)) || (rc = download_field(c, p, T[2], ld, halo)) ||
-        (rc = download_field(c, cs, T[3], ld, halo)))
+    double *stg = T2[0];
+    if ((rc = download_field(c, rho, U[0], ld, halo, stg)) || (rc = download_field(c, u, T[0], ld, halo, stg)) ||
+        (rc = download_field(c, v, T[1], ld, halo, stg)) || (rc = download_field(c, p, T[2], ld, halo, stg)) ||
+        (rc = download_field(c, cs, T[3], ld, halo, stg)))
       return rc;
   }
   return hip_check(hipStreamSynchronize(c->stream), "get_primitive");
@@ -821,7 +856,7 @@ int cato_residual(cato_ctx *c, double *rho, double *u, double *v, double *p, dou
   double **Obuf = s.U[(s.cur + 2) % 3];
   const double *P[4] = {rho, u, v, p};
   for (int k = 0; k < 4; ++k)
-    if ((rc = upload_field(c, Pbuf[k], P[k], ld, s.nh))) return rc;
+    if ((rc = upload_field(c, Pbuf[k], P[k], ld, s.nh, nullptr))) return rc;
   StageArgs a = s.stage_args(MODE_RESIDUAL, 0, 0, 0, -1, -1, (s.cur + 2) % 3, false, nullptr, 1.0, true);
   for (int k = 0; k < 4; ++k) a.prim[k] = Pbuf[k];
   a.block_min = nullptr;
@@ -833,7 +868,7 @@ int cato_residual(cato_ctx *c, double *rho, double *u, double *v, double *p, dou
   int ny = s.nj + 2 * s.nh;
   for (int k = 0; k < 4; ++k) {
     std::memset(D[k], 0, (size_t)ld * ny * sizeof(double));
-    if ((rc = download_field(c, D[k], Obuf[k], ld, s.nh))) return rc;
+    if ((rc = download_field(c, D[k], Obuf[k], ld, s.nh, nullptr))) return rc;
   }
   return hip_check(hipStreamSynchronize(c->stream), "residual");
 }

@@ -52,4 +52,11 @@ struct Solve {
   std::vector<StageArgs> step_stages(const LoopState *lp, double dt_value) const;
 };
 
+// Host <-> device copy of one interior field with host layout (ld, halo).
+// A tightly packed host field (ld == ni, halo == 0) moves as ONE contiguous
+// copy, repacked on the device through `staging` (a free device field of the
+// same solve) when the device pitch differs; other layouts use a 2D copy.
+int field_up(hipStream_t st, const Solve &s, double *dev, const double *host, int ld, int halo, double *staging);
+int field_down(hipStream_t st, const Solve &s, double *host, const double *dev, int ld, int halo, double *staging);
+
 }  // namespace cato

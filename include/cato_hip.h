@@ -175,6 +175,38 @@ int cato_batch_get_conserved(cato_batch *b, int k, double *rho, double *rho_u, d
 int cato_batch_run(cato_batch *b, const double *max_time, const double *contour_interval_dt, long long max_iterations,
                    int use_graph, long long *iterations, double *times, int *errors);
 
+/* ---- serving form: continuous batching under an HBM budget ----
+ * No counterpart in the reference (one solve per program run there).  A
+ * server owns one device arena of hbm_budget_bytes, allocated at create; the
+ * request loop itself never allocates device memory.  All requests of a
+ * server share its precision and the integrator of the first request. */
+typedef struct cato_server cato_server;
+typedef struct cato_request {
+  cato_config cfg;
+  cato_geom geom;
+  const double *rho, *u, *v, *p; /* interior primitives, Fortran order, ld = ni */
+  double max_time;               /* main.f90 loop semantics per request */
+  double contour_interval_dt;    /* 0: max_time */
+  double *out_rho, *out_rho_u, *out_rho_v, *out_rho_E; /* interior conserved result (ld = ni); may be NULL */
+} cato_request;
+int cato_server_create(double hbm_budget_bytes, int max_slots, int precision, int steps_per_tick,
+                       cato_server **out);
+int cato_server_destroy(cato_server *s);
+/* Queue a request (inputs are copied; the output pointers must stay valid
+ * until the request completes).  Returns the request id (>= 0) or a status. */
+int cato_server_submit(cato_server *s, const cato_request *req, double arrival_time);
+/* One tick: retire finished solves (results copied out, memory returned),
+ * admit queued requests in arrival order while a slot and arena memory are
+ * free, then advance every resident solve by steps_per_tick steps with one
+ * persistent launch per RK stage.  Any output pointer may be NULL. */
+int cato_server_tick(cato_server *s, int *n_resident, int *n_queued, int *n_completed_total);
+/* Completion record of request `id`: done flag, completion time on the
+ * server clock, iterations and solve error code (-1: request rejected). */
+int cato_server_poll(cato_server *s, int id, int *done, double *completion_time, long long *iterations,
+                     int *error_code);
+/* Seconds since cato_server_create. */
+double cato_server_clock(cato_server *s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,3 +65,28 @@ def test_struct_layout_matches_header():
         ty, rest = decl.split(None, 1)
         names += [n.strip() for n in rest.split(",")]
     assert names == [f[0] for f in N.CatoConfig._fields_]
+
+
+def test_struct_offsets_match_c_compiler(tmp_path):
+    """sizeof/offsetof of the ABI structs as gcc lays them out == the ctypes mirrors."""
+    import subprocess
+
+    from fvleg_2d_amd import _native as N
+
+    structs = {"cato_config": N.CatoConfig, "cato_geom": N.CatoGeom, "cato_request": N.CatoRequest}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "cato_hip.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} %zu\\n", sizeof({cname}));')
+        for f in py._fields_:
+            lines.append(f'  printf("{cname}.{f[0]} %zu\\n", offsetof({cname}, {f[0]}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)], check=True)
+    out = dict(line.rsplit(" ", 1) for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                              check=True).stdout.split("\n") if line)
+    for cname, py in structs.items():
+        assert int(out[cname]) == ctypes.sizeof(py), cname
+        for f in py._fields_:
+            assert int(out[f"{cname}.{f[0]}"]) == getattr(py, f[0]).offset, (cname, f[0])
