@@ -1,15 +1,30 @@
 """Benchmark of the CATO hot path on one MI355X.
 
-Workload (N=1): the Kelvin-Helmholtz shear layer on a 2048 x 2048 periodic
-grid, SLAU flux, MUSCL/TVD2 van Leer, SSP-RK3, CFL 0.5, FP64 (the
-configuration the >= 70 % HBM-roofline target is quoted on).  A "step" is one
-SSP-RK3 step over the whole grid (three fused stage launches + the CFL
-reduce); dt is the device-computed CFL step, so no host synchronisation
-happens inside the timed region.  Inputs are resident in HBM before timing.
+Headline workload (N=1): the Kelvin-Helmholtz shear layer on a 2048 x 2048
+periodic grid, SLAU flux, MUSCL/TVD2 van Leer, SSP-RK3, CFL 0.5, FP64, 2000
+steps.  A "step" is one SSP-RK3 step over the whole grid (three fused stage
+launches + the CFL reduce); dt is the device-computed CFL step, so no host
+synchronisation happens inside the timed region.  Inputs are resident in HBM
+before timing.
 
-Prints one JSON line (rank 0) with the throughput, the roofline of the fused
-stage kernel (per-launch time from HIP events on the solve's stream) and a
-CPU baseline (the C oracle on this host's cores, a bounded sample).
+The JSON line carries, beside the headline throughput:
+  roofline        the fused stage kernel's algorithmic HBM bytes per launch
+                  over its mean launch time (HIP events on the solve's stream)
+  cpu_baseline    the C oracle on this host (one thread, a bounded sample)
+  precision_form  the same workload with FP32 reconstruction/flux and FP64
+                  state: throughput, roofline and its L1/Linf difference from
+                  the FP64 state after the same steps
+  riemann_1024    four-quadrant Riemann, 1024^2, HLLC + MLP5, SSP-RK3 to
+                  t = 0.25, FP64 and FP32-flux forms, errors of the latter
+  batched         256 independent Riemann solves (side log-uniform in
+                  [64, 512]) in one persistent launch per stage vs one at a
+                  time through the plain path
+  served          Poisson arrivals of Riemann requests (side log-uniform in
+                  [128, 2048], t_end in [0.1, 0.3]) through the serving loop
+                  (FP32-flux) vs a one-at-a-time plain FP64 queue
+`--forms none` keeps only the headline and its roofline/CPU legs.
+`--mode copy` times a plain device copy of known size (the calibration run
+for the rocprofv3 FETCH_SIZE / WRITE_SIZE counters).
 """
 import argparse
 import json
@@ -36,10 +51,16 @@ def parse():
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_flux"])
     ap.add_argument("--tile", type=int, default=-1)
+    ap.add_argument("--forms", default="all", choices=["all", "none"])
+    ap.add_argument("--mode", default="bench", choices=["bench", "copy"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--profile-only", action="store_true", help="run the timed loop only (for rocprofv3)")
     return ap.parse_args()
+
+
+def say(*a):
+    print(*a, file=sys.stderr, flush=True)
 
 
 def cpu_baseline(pb, steps):
@@ -63,43 +84,48 @@ def cpu_baseline(pb, steps):
     return pb.ni * pb.nj * steps / el, el
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the solve does not "
-              "shard across GPUs here.  Run with --gpus 1.", file=sys.stderr)
-        sys.exit(2)
+def rel_errors(ref, got):
+    """Per conserved variable: (mean |got - ref| / max |ref|, max |got - ref| / max |ref|)."""
+    import numpy as np
 
+    out = {}
+    for name, r, g in zip(("rho", "rho_u", "rho_v", "rho_E"), ref, got):
+        scale = max(float(np.max(np.abs(r))), 1e-300)
+        d = np.abs(g - r)
+        out[name] = {"l1": float(np.mean(d)) / scale, "linf": float(np.max(d)) / scale}
+    return out
+
+
+def timed_kh(n, precision, steps, warmup, tile, profile_only=False):
+    """Advance the KH workload; returns (wall seconds, stage-kernel mean ms, launches, final state)."""
     from fvleg_2d_amd import Solver
     from fvleg_2d_amd.problems import kelvin_helmholtz
 
-    pb = kelvin_helmholtz(n=args.n, precision=args.precision)
-    pb.cfg.tile = args.tile
-    cells = pb.ni * pb.nj
-    s = Solver(pb.cfg, pb.node_x, pb.node_y)
-    s.set_primitive(pb.rho, pb.u, pb.v, pb.p)
-    s.advance(args.warmup)
-    s.synchronize()
+    pb = kelvin_helmholtz(n=n, precision=precision)
+    pb.cfg.tile = tile
+    with Solver(pb.cfg, pb.node_x, pb.node_y) as s:
+        s.set_primitive(pb.rho, pb.u, pb.v, pb.p)
+        s.advance(warmup)
+        s.synchronize()
+        t0 = time.perf_counter()
+        s.advance(steps)
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        if profile_only:
+            return pb, wall, None, 0, None
+        state = s.conserved()
+        # per-launch kernel time of the fused stage (HIP events on the solve's stream)
+        s.set_timing(True)
+        s.advance(min(steps, 400))
+        s.synchronize()
+        tm = s.timing()
+        s.set_timing(False)
+    return pb, wall, tm["stage_ms"] / max(tm["stage_launches"], 1), tm["stage_launches"], state
 
-    t0 = time.perf_counter()
-    s.advance(args.steps)
-    s.synchronize()
-    wall = time.perf_counter() - t0
-    if args.profile_only:
-        print(json.dumps({"profile_only": True, "steps": args.steps, "wall_s": wall}))
-        return
 
-    # per-launch kernel time of the fused stage (HIP events on the solve's stream)
-    tsteps = min(args.steps, 400)
-    s.set_timing(True)
-    s.advance(tsteps)
-    s.synchronize()
-    tm = s.timing()
-    s.set_timing(False)
-    s.close()
-
+def roofline(pb, stage_avg_ms, launches, precision, n):
     integ = pb.cfg.integrator
-    stage_avg_ms = tm["stage_ms"] / max(tm["stage_launches"], 1)
+    cells = pb.ni * pb.nj
     bytes_per_launch = STEP_BYTES_PER_CELL[integ] / STAGES[integ] * cells
     achieved = bytes_per_launch / (stage_avg_ms * 1e-3) / 1e9
     traffic = None
@@ -107,11 +133,78 @@ def main():
     if os.path.exists(tpath):
         try:
             tj = json.load(open(tpath))
-            if tj.get("n") == args.n and tj.get("precision") == args.precision:
-                traffic = tj.get("bytes_per_launch")
+            key = f"{n}_{precision}"
+            if key in tj:
+                traffic = tj[key].get("bytes_per_launch")
         except Exception:
             traffic = None
+    return {
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": traffic,
+        "kernel": "cato::stage_kernel (fused BC + reconstruction + flux + divergence + SSP stage)",
+        "algorithmic_bytes_per_launch": bytes_per_launch,
+        "kernel_avg_ms": stage_avg_ms,
+        "launches_timed": launches,
+    }
 
+
+def riemann_forms():
+    """Config: 1024^2 four-quadrant Riemann, HLLC + MLP5, SSP-RK3, CFL 0.4, to t = 0.25."""
+    from fvleg_2d_amd import Solver
+    from fvleg_2d_amd.problems import riemann2d
+
+    res = {}
+    states = {}
+    for prec in ("fp64", "fp32_flux"):
+        pb = riemann2d(n=1024, seed=1, precision=prec)
+        with Solver(pb.cfg, pb.node_x, pb.node_y) as s:
+            s.set_primitive(pb.rho, pb.u, pb.v, pb.p)
+            s.synchronize()
+            t0 = time.perf_counter()
+            it, t, e = s.run(pb.max_time, 0, pb.max_time)
+            wall = time.perf_counter() - t0
+            states[prec] = s.conserved()
+        res[prec] = {"steps": int(it), "t_end": t, "error_code": int(e), "wall_s": wall,
+                     "cell_updates_per_s": pb.ni * pb.nj * it / wall}
+    res["fp32_flux"]["error_vs_fp64"] = rel_errors(states["fp64"], states["fp32_flux"])
+    res["note"] = "the FP64 path is bit-identical to the C oracle (GPU parity tests), so it is the FP64 reference here"
+    return res
+
+
+def copy_calibration():
+    """A plain device copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel calibrate the counters."""
+    import torch
+
+    nbytes = 1 << 30
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 5}))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); the solve does not "
+              "shard across GPUs here.  Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    if args.mode == "copy":
+        copy_calibration()
+        return
+
+    say(f"[bench] KH {args.n}^2 {args.precision}: {args.warmup} warmup + {args.steps} timed steps")
+    pb, wall, stage_ms, launches, state = timed_kh(args.n, args.precision, args.steps, args.warmup, args.tile,
+                                                   args.profile_only)
+    if args.profile_only:
+        print(json.dumps({"profile_only": True, "steps": args.steps, "wall_s": wall}))
+        return
+    cells = pb.ni * pb.nj
     value = cells * args.steps / wall
     out = {
         "metric": METRIC,
@@ -129,22 +222,31 @@ def main():
         "config": {
             "workload": f"kelvin_helmholtz_{pb.ni}x{pb.nj}_periodic_SLAU_vanleer_ssp_rk3_cfl0.5_{args.precision}",
             "cells": cells,
-            "bytes_per_cell_step": STEP_BYTES_PER_CELL[integ],
+            "bytes_per_cell_step": STEP_BYTES_PER_CELL[pb.cfg.integrator],
             "tile": args.tile,
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
-            "kernel": "cato::stage_kernel (fused BC+reconstruction+flux+divergence+SSP stage)",
-            "kernel_avg_ms": stage_avg_ms,
-            "launches_timed": tm["stage_launches"],
-        },
+        "roofline": roofline(pb, stage_ms, launches, args.precision, args.n),
     }
+    if args.forms == "all" and args.precision == "fp64":
+        say("[bench] precision form (FP32 flux, FP64 state) on the same workload")
+        pb32, wall32, ms32, l32, state32 = timed_kh(args.n, "fp32_flux", args.steps, args.warmup, args.tile)
+        out["precision_form"] = {
+            "value": cells * args.steps / wall32,
+            "unit": "cell-updates/s",
+            "ms_per_step": wall32 / args.steps * 1e3,
+            "roofline": roofline(pb32, ms32, l32, "fp32_flux", args.n),
+            "difference_from_fp64_state": rel_errors(state, state32),
+        }
+        say("[bench] riemann 1024^2 HLLC/MLP5, FP64 and FP32-flux")
+        out["riemann_1024"] = riemann_forms()
+        from fvleg_2d_amd.benchforms import measure_batched, measure_served
+
+        say("[bench] batched: 256 Riemann solves vs one at a time")
+        out["batched"] = measure_batched(n_solves=256)
+        say("[bench] served: Poisson arrivals vs a plain FP64 queue")
+        out["served"] = measure_served(n=16)
     if not args.no_cpu and os.environ.get("CATO_BENCH_NO_CPU") != "1":
+        say("[bench] CPU baseline (C oracle, one thread)")
         rate, el = cpu_baseline(pb, args.cpu_steps)
         out["cpu_baseline"] = {
             "value": rate,

@@ -71,6 +71,7 @@ SIGNATURES = {
     "cato_batch_get_conserved": (C.c_int, [_CTX, C.c_int, _DP, _DP, _DP, _DP, C.c_int, C.c_int]),
     "cato_batch_run": (C.c_int, [_CTX, _DP, _DP, C.c_longlong, C.c_int, C.POINTER(C.c_longlong), _DP,
                                  C.POINTER(C.c_int)]),
+    "cato_device_bytes": (C.c_int, [C.POINTER(CatoConfig), C.POINTER(CatoGeom), C.POINTER(C.c_ulonglong)]),
     "cato_server_create": (C.c_int, [C.c_double, C.c_int, C.c_int, C.c_int, C.POINTER(_CTX)]),
     "cato_server_destroy": (C.c_int, [_CTX]),
     "cato_server_submit": (C.c_int, [_CTX, C.c_void_p, C.c_double]),

@@ -47,8 +47,17 @@ template <typename R, int TX, int TY> constexpr size_t batch_smem_bytes() {
   CATO_BATCH_CASE(F, 3)      \
   CATO_BATCH_CASE(F, 4)
 
+// Waves per SIMD the LDS footprint allows (at most 4): the register budget is
+// capped to match, since the forty variants merged in one body would
+// otherwise claim far more registers than any single variant needs.
+template <typename R, int TX, int TY> constexpr int batch_waves_per_simd() {
+  constexpr size_t lds = batch_smem_bytes<R, TX, TY>() + sizeof(RedSmem);
+  constexpr int fit = (int)((160 * 1024) / lds);
+  return fit < 1 ? 1 : (fit > 4 ? 4 : fit);
+}
+
 template <typename R, int TX, int TY>
-__global__ __launch_bounds__(kThreads) void batch_stage_kernel(BatchStageArgs b) {
+__global__ __launch_bounds__(kThreads, (batch_waves_per_simd<R, TX, TY>())) void batch_stage_kernel(BatchStageArgs b) {
   __shared__ __attribute__((aligned(16))) char smem[batch_smem_bytes<R, TX, TY>()];
   __shared__ RedSmem red;
   // XCD-aware split: blocks of XCD x take the x-th eighth of the work list.

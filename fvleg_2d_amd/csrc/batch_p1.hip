@@ -3,7 +3,7 @@
 
 namespace cato {
 namespace {
-using R1 = float;
+using R1 = f32x;  // FP32-flux form arithmetic
 template <int TX, int TY> void launch(dim3 g, hipStream_t s, const BatchStageArgs &a) {
   hipLaunchKernelGGL((batch_stage_kernel<R1, TX, TY>), g, dim3(kThreads), 0, s, a);
 }

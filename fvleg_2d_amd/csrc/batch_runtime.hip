@@ -194,14 +194,14 @@ int BatchEngine::run(const double *max_time, const double *contour_dt, long long
   int rc = upload_args();  // synchronises, so L may go
   if (rc) return rc;
   if ((rc = compact(L))) return rc;  // every solve live again (a batch may be run more than once)
-  // first dt of every solve (from the caller's primitives or the stored state)
+  // first dt of every solve
   for (int k = 0; k < n; ++k) {
     Solve &sv = solves[k];
     FieldGeom g = sv.fgeom();
     int nb = sv.tiles_x * sv.tiles_y;
     const double *a[4];
-    for (int v = 0; v < 4; ++v) a[v] = sv.prims_valid ? sv.prim[v] : sv.U[sv.cur][v];
-    hipLaunchKernelGGL(k_cfl_partial, dim3((unsigned)nb), dim3(kThreads), 0, stream, g, sv.prims_valid ? 1 : 0, a[0],
+    for (int v = 0; v < 4; ++v) a[v] = sv.P[0][v];  // primitives of U^n
+    hipLaunchKernelGGL(k_cfl_partial, dim3((unsigned)nb), dim3(kThreads), 0, stream, g, 1, a[0],
                        a[1], a[2], a[3], sv.block_min, sv.block_err);
     hipLaunchKernelGGL(k_reduce_finalize, dim3(1), dim3(1024), 0, stream, nb, sv.block_min, sv.block_err, sv.result,
                        sv.loop, 1);
@@ -302,10 +302,10 @@ int cato_batch_set_primitive(cato_batch *b, int k, const double *rho, const doub
   const double *P[4] = {rho, u, v, p};
   double *stg = s->U[(s->cur + 1) % 3][0];
   for (int q = 0; q < 4; ++q)
-    if ((rc = field_up(b->stream, *s, s->prim[q], P[q], ld, halo, stg))) return rc;
+    if ((rc = field_up(b->stream, *s, s->P[0][q], P[q], ld, halo, stg))) return rc;
   FieldGeom g = s->fgeom();
   hipLaunchKernelGGL(k_prim_to_cons, dim3((unsigned)((s->ni + 255) / 256), (unsigned)s->nj), dim3(256), 0, b->stream,
-                     g, s->prim[0], s->prim[1], s->prim[2], s->prim[3], s->U[s->cur][0], s->U[s->cur][1],
+                     g, s->P[0][0], s->P[0][1], s->P[0][2], s->P[0][3], s->U[s->cur][0], s->U[s->cur][1],
                      s->U[s->cur][2], s->U[s->cur][3]);
   if ((rc = hip_check(hipGetLastError(), "prim_to_cons"))) return rc;
   s->prims_valid = true;

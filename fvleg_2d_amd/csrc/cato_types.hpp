@@ -60,17 +60,18 @@ struct State4 {
 struct StageArgs {
   int ni, nj, nh, pitch;     // interior cells, halo width, row pitch (elements)
   int mode;
-  int prim_from_array;       // 1: primitives come from prim[] instead of U_k
+  int check_input;           // 1: flag NaN in the input primitives (the caller's, first stage)
   int last_stage;            // 1: write CFL partials and sanity flags for the new state
   int low_mach_fix;
   int bc[4];                 // +x, -x, +y, -y
   int tiles_x, tiles_y;      // tile grid
   double gamma, cfl;
   double a, b, c;            // stage coefficients
-  const double *uk[4];       // U_k (primitives source)
+  const double *uk[4];       // U_k at the cell (MODE_FIRST / MODE_COMBINE)
   const double *u0[4];       // U_0 (MODE_COMBINE / MODE_BASE)
-  const double *prim[4];     // rho, u, v, p when prim_from_array
-  double *out[4];
+  const double *prim[4];     // rho, u, v, p of U_k (read with halo through the BC map)
+  double *out[4];            // U_{k+1}
+  double *prim_out[4];       // rho, u, v, p of U_{k+1} (null in MODE_RESIDUAL)
   const double *dxi, *lxi;   // per halo'd i: x spacing and x-edge length
   const double *dyj, *lyj;   // per halo'd j
   const LoopState *loop;     // dt source and done flag (may be null -> dt_value)

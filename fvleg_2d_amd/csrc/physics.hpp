@@ -34,6 +34,46 @@ template <> struct Lim<float> {
   static constexpr float tiny = 1.17549435082228750797e-38f;
 };
 
+// Arithmetic type of the FP32-flux precision form: IEEE binary32 except that
+// division and square root use the hardware reciprocal and square root
+// (v_rcp_f32 plus one Newton correction, v_sqrt_f32: about 1 ulp) instead of
+// the correctly rounded sequences, whose scale/fixup steps dominate the cost
+// of the reconstruction and flux code.  Only explicit conversions exist, so
+// an unintended double or int operand fails to compile.
+struct f32x {
+  float v;
+  f32x() = default;
+  __host__ __device__ constexpr explicit f32x(float x) : v(x) {}
+  __host__ __device__ constexpr explicit f32x(double x) : v((float)x) {}
+  __host__ __device__ constexpr explicit f32x(int x) : v((float)x) {}
+  __host__ __device__ constexpr explicit operator double() const { return (double)v; }
+  __host__ __device__ constexpr explicit operator float() const { return v; }
+};
+__device__ inline f32x operator+(f32x a, f32x b) { return f32x(a.v + b.v); }
+__device__ inline f32x operator-(f32x a, f32x b) { return f32x(a.v - b.v); }
+__device__ inline f32x operator*(f32x a, f32x b) { return f32x(a.v * b.v); }
+__device__ inline f32x operator-(f32x a) { return f32x(-a.v); }
+__device__ inline f32x operator/(f32x a, f32x b) {
+  const float r = __builtin_amdgcn_rcpf(b.v);
+  const float q = a.v * r;
+  const float e = __builtin_fmaf(-b.v, q, a.v);
+  // e is NaN only when b is 0 or inf (0 * inf): then q is already the IEEE quotient
+  return f32x((e == e) ? __builtin_fmaf(e, r, q) : q);
+}
+__device__ inline bool operator<(f32x a, f32x b) { return a.v < b.v; }
+__device__ inline bool operator>(f32x a, f32x b) { return a.v > b.v; }
+__device__ inline bool operator<=(f32x a, f32x b) { return a.v <= b.v; }
+__device__ inline bool operator>=(f32x a, f32x b) { return a.v >= b.v; }
+__device__ inline bool operator==(f32x a, f32x b) { return a.v == b.v; }
+__device__ inline bool operator!=(f32x a, f32x b) { return a.v != b.v; }
+template <> struct Lim<f32x> {
+  static constexpr f32x eps = f32x(Lim<float>::eps);
+  static constexpr f32x tiny = f32x(Lim<float>::tiny);
+};
+__device__ inline f32x absd(f32x x) { return f32x(__builtin_fabsf(x.v)); }
+__device__ inline f32x sqrtr(f32x x) { return f32x(__builtin_amdgcn_sqrtf(x.v)); }
+__device__ inline f32x copysignr(f32x a, f32x b) { return f32x(__builtin_copysignf(a.v, b.v)); }
+
 template <typename R> __host__ __device__ inline R fmax2(R a, R b) { return (b > a) ? b : a; }
 template <typename R> __host__ __device__ inline R fmin2(R a, R b) { return (b < a) ? b : a; }
 template <typename R> __host__ __device__ inline R sq(R x) { return x * x; }
@@ -519,6 +559,20 @@ template <int FLUX> constexpr bool needs_pressure_nbrs() {
   return FLUX == FLUX_AUSMPW_PLUS || FLUX == FLUX_SD_SLAU || FLUX == FLUX_SD_SLAU2 || FLUX == FLUX_ROE;
 }
 
+// qa = a / m and qb = b / m for m = max(|a|, |b|, 1) != 1.  A finite m is |a|
+// or |b| itself, and x / |x| is exactly copysign(1, x), so one IEEE division
+// gives both quotients bit for bit; inf / NaN operands keep both divisions.
+__device__ inline void scaled_pair(double a, double b, double m, double &qa, double &qb) {
+  const bool a_big = absd(a) == m;
+  const double q = (a_big ? b : a) / m;
+  qa = a_big ? copysignr(1.0, a) : q;
+  qb = a_big ? q : copysignr(1.0, b);
+  if (!(m <= 1.7976931348623157e308)) {
+    qa = a / m;
+    qb = b / m;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Divergence for one conserved variable of one cell: flux_split_edges
 // (flux_solver.f90:252-277 and 395-425).  l1..l4 = bottom/right/top/left lengths.
@@ -533,12 +587,16 @@ __device__ inline double divergence(double Fi, double Fim, double Gj, double Gjm
   if (mi == 1.0) {
     ri = (Fi * l2) + (-(Fim * l4));  // x/1 and x*1 are exact
   } else {
-    ri = ((Fi / mi * l2) + (-(Fim / mi * l4))) * mi;
+    double qa, qb;
+    scaled_pair(Fi, Fim, mi, qa, qb);
+    ri = ((qa * l2) + (-(qb * l4))) * mi;
   }
   if (mj == 1.0) {
     rj = (Gj * l3) + (-(Gjm * l1));
   } else {
-    rj = ((Gj / mj * l3) + (-(Gjm / mj * l1))) * mj;
+    double qa, qb;
+    scaled_pair(Gj, Gjm, mj, qa, qb);
+    rj = ((qa * l3) + (-(qb * l1))) * mj;
   }
   if (absd(ri) < mi * REL_TOL) ri = 0.0;
   if (absd(rj) < mj * REL_TOL) rj = 0.0;

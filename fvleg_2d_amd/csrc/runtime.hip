@@ -151,7 +151,7 @@ size_t Solve::device_bytes() const {
   size_t f = field_elems();
   size_t nb = (size_t)tiles_x * tiles_y;
   size_t geo_n = (size_t)2 * (ni + 2 * nh) + 2 * (nj + 2 * nh);
-  return (12 + 4) * f * sizeof(double) + geo_n * sizeof(double) + nb * (sizeof(double) + sizeof(int)) +
+  return (12 + 12) * f * sizeof(double) + geo_n * sizeof(double) + nb * (sizeof(double) + sizeof(int)) +
          sizeof(LoopState) + 4 * sizeof(double) + 1024;
 }
 
@@ -164,10 +164,11 @@ void Solve::bind(char *base) {
       U[s][v] = p;
       p += f;
     }
-  for (int v = 0; v < 4; ++v) {
-    prim[v] = p;
-    p += f;
-  }
+  for (int s = 0; s < 3; ++s)
+    for (int v = 0; v < 4; ++v) {
+      P[s][v] = p;
+      p += f;
+    }
   dxi = p; p += ni + 2 * nh;
   lxi = p; p += ni + 2 * nh;
   dyj = p; p += nj + 2 * nh;
@@ -209,15 +210,15 @@ FieldGeom Solve::fgeom() const {
   return g;
 }
 
-StageArgs Solve::stage_args(int mode, double a, double b, double c, int in_buf, int u0_buf, int out_buf, bool last,
-                            const LoopState *lp, double dt_value, bool prims_from_array) const {
+StageArgs Solve::stage_args(int mode, double a, double b, double c, int in_buf, int u0_buf, int out_buf, int p_in,
+                            int p_out, bool last, const LoopState *lp, double dt_value, bool check_input) const {
   StageArgs s{};
   s.ni = ni;
   s.nj = nj;
   s.nh = nh;
   s.pitch = pitch;
   s.mode = mode;
-  s.prim_from_array = prims_from_array ? 1 : 0;
+  s.check_input = check_input ? 1 : 0;
   s.last_stage = last ? 1 : 0;
   s.low_mach_fix = cfg.apply_low_mach_fix;
   s.bc[0] = cfg.bc_plus_x;
@@ -234,8 +235,9 @@ StageArgs Solve::stage_args(int mode, double a, double b, double c, int in_buf, 
   for (int v = 0; v < 4; ++v) {
     s.uk[v] = in_buf >= 0 ? U[in_buf][v] : nullptr;
     s.u0[v] = u0_buf >= 0 ? U[u0_buf][v] : nullptr;
-    s.prim[v] = prim[v];
-    s.out[v] = out_buf >= 0 ? U[out_buf][v] : prim[v];
+    s.prim[v] = P[p_in][v];
+    s.out[v] = U[out_buf][v];
+    s.prim_out[v] = p_out >= 0 ? P[p_out][v] : nullptr;
   }
   s.dxi = dxi;
   s.lxi = lxi;
@@ -248,25 +250,47 @@ StageArgs Solve::stage_args(int mode, double a, double b, double c, int in_buf, 
   return s;
 }
 
+// Primitive buffer sets read and written by stage m of an ns-stage step.  P[0]
+// holds the primitives of U^n at every step boundary, so the arguments of a
+// step never change (one captured graph serves every step): even ns cycles
+// 0->1->0->...->0, odd ns cycles 0->1->2->1->2->...->0.
+static void prim_cycle(int m, int ns, int &pin, int &pout) {
+  if (ns % 2 == 0) {
+    pin = m % 2;
+    pout = (m + 1) % 2;
+    return;
+  }
+  pin = (m == 0) ? 0 : 1 + (m - 1) % 2;
+  pout = (m == ns - 1) ? 0 : 1 + m % 2;
+}
+
 // Stage sequence of one step: ssp_rk_2_2 (mod_fluid.f90:1115-1129),
-// ssp_rk_3_3 (773-802) and rk_multistage (957-966).  Buffers: cur holds U^n,
-// the two others hold intermediate stages; the last stage overwrites U^n in
-// place (it reads U^n only at its own cell).
+// ssp_rk_3_3 (773-802) and rk_multistage (957-966).  State buffers: cur holds
+// U^n, the two others hold intermediate stages; the last stage overwrites U^n
+// in place (it reads U^n only at its own cell).
 std::vector<StageArgs> Solve::step_stages(const LoopState *lp, double dt_value) const {
   std::vector<StageArgs> st;
   const int n = cur, A = (cur + 1) % 3, B = (cur + 2) % 3;
-  // The first stage after cato_set_primitive takes its primitives from prim[]
-  // (fluid_t%initialize keeps the deck's u, v, p); U^n always comes from U[n].
-  const bool pfa = prims_valid;
+  // The first stage after cato_set_primitive reads the caller's primitives
+  // (fluid_t%initialize keeps the deck's u, v, p) and checks them for NaN.
+  const bool chk = prims_valid;
+  int pi, po;
   switch (cfg.integrator) {
     case CATO_RK_SSP2:
-      st.push_back(stage_args(MODE_FIRST, 0, 0, 1.0, n, -1, A, false, lp, dt_value, pfa));
-      st.push_back(stage_args(MODE_COMBINE, 0.5, 0.5, 0.5, A, n, n, true, lp, dt_value, false));
+      prim_cycle(0, 2, pi, po);
+      st.push_back(stage_args(MODE_FIRST, 0, 0, 1.0, n, -1, A, pi, po, false, lp, dt_value, chk));
+      prim_cycle(1, 2, pi, po);
+      st.push_back(stage_args(MODE_COMBINE, 0.5, 0.5, 0.5, A, n, n, pi, po, true, lp, dt_value, false));
       break;
     case CATO_RK_SSP3:
-      st.push_back(stage_args(MODE_FIRST, 0, 0, 1.0, n, -1, A, false, lp, dt_value, pfa));
-      st.push_back(stage_args(MODE_COMBINE, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0, A, n, B, false, lp, dt_value, false));
-      st.push_back(stage_args(MODE_COMBINE, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, B, n, n, true, lp, dt_value, false));
+      prim_cycle(0, 3, pi, po);
+      st.push_back(stage_args(MODE_FIRST, 0, 0, 1.0, n, -1, A, pi, po, false, lp, dt_value, chk));
+      prim_cycle(1, 3, pi, po);
+      st.push_back(
+          stage_args(MODE_COMBINE, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0, A, n, B, pi, po, false, lp, dt_value, false));
+      prim_cycle(2, 3, pi, po);
+      st.push_back(
+          stage_args(MODE_COMBINE, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, B, n, n, pi, po, true, lp, dt_value, false));
       break;
     default: {
       static const double a13[3] = {0.1481, 0.4, 1.0}, a14[4] = {0.0833, 0.2069, 0.4265, 1.0},
@@ -286,7 +310,8 @@ std::vector<StageArgs> Solve::step_stages(const LoopState *lp, double dt_value) 
       for (int m = 0; m < ns; ++m) {
         bool last = (m == ns - 1);
         int out = last ? n : ((m % 2 == 0) ? A : B);
-        st.push_back(stage_args(MODE_BASE, 0, 0, al[m], prev, n, out, last, lp, dt_value, pfa && m == 0));
+        prim_cycle(m, ns, pi, po);
+        st.push_back(stage_args(MODE_BASE, 0, 0, al[m], prev, n, out, pi, po, last, lp, dt_value, chk && m == 0));
         prev = out;
       }
     }
@@ -334,6 +359,21 @@ int field_down(hipStream_t st, const Solve &s, double *host, const double *dev, 
                    "download");
 }
 
+// P[0] = primitives of the stored state U^n (calculate_derived_quantities),
+// as every stage leaves them for the next; used after cato_set_conserved.
+int derive_primitives(hipStream_t st, const Solve &s) {
+  FieldGeom g = s.fgeom();
+  double *const *U = s.U[s.cur];
+  double *const *P = s.P[0];
+  double *cs = s.U[(s.cur + 1) % 3][0];
+  hipLaunchKernelGGL(k_derived, dim3((unsigned)((s.ni + 255) / 256), (unsigned)s.nj), dim3(256), 0, st, g, U[0], U[1],
+                     U[2], U[3], P[1], P[2], P[3], cs);
+  int rc = hip_check(hipGetLastError(), "derive primitives");
+  if (rc) return rc;
+  return hip_check(hipMemcpyAsync(P[0], U[0], s.field_elems() * sizeof(double), hipMemcpyDeviceToDevice, st),
+                   "derive primitives");
+}
+
 // Every pointer a stage launch will dereference must be set (checked before launch).
 int validate_stage_args(const StageArgs &s) {
   auto need = [&](const double *const *p, const char *what) -> int {
@@ -343,11 +383,8 @@ int validate_stage_args(const StageArgs &s) {
   };
   int rc;
   if ((rc = need(s.out, "output state"))) return rc;
-  if (s.prim_from_array) {
-    if ((rc = need(s.prim, "primitive arrays"))) return rc;
-  } else if ((rc = need(s.uk, "U_k state"))) {
-    return rc;
-  }
+  if ((rc = need(s.prim, "primitive arrays"))) return rc;
+  if (s.mode != MODE_RESIDUAL && (rc = need(s.prim_out, "output primitive arrays"))) return rc;
   if (s.mode == MODE_FIRST && (rc = need(s.uk, "U_k state (first stage combine)"))) return rc;
   if ((s.mode == MODE_COMBINE || s.mode == MODE_BASE) && (rc = need(s.u0, "U_0 state"))) return rc;
   if (s.mode == MODE_COMBINE && (rc = need(s.uk, "U_k state"))) return rc;
@@ -452,16 +489,10 @@ int launch_cfl(cato_ctx *c, int loop_mode) {
   Solve &s = c->s;
   FieldGeom g = s.fgeom();
   int nb = s.tiles_x * s.tiles_y;
-  const double *a0, *a1, *a2, *a3;
-  if (s.prims_valid) {
-    a0 = s.prim[0]; a1 = s.prim[1]; a2 = s.prim[2]; a3 = s.prim[3];
-  } else {
-    a0 = s.U[s.cur][0]; a1 = s.U[s.cur][1]; a2 = s.U[s.cur][2]; a3 = s.U[s.cur][3];
-  }
-  int from_prims = s.prims_valid ? 1 : 0;
+  // P[0] holds the primitives of U^n (the caller's after cato_set_primitive)
   int rc = timed(c, false, [&] {
-    hipLaunchKernelGGL(k_cfl_partial, dim3((unsigned)nb), dim3(kThreads), 0, c->stream, g, from_prims, a0, a1, a2, a3,
-                       s.block_min, s.block_err);
+    hipLaunchKernelGGL(k_cfl_partial, dim3((unsigned)nb), dim3(kThreads), 0, c->stream, g, 1, s.P[0][0], s.P[0][1],
+                       s.P[0][2], s.P[0][3], s.block_min, s.block_err);
   });
   if (rc) return rc;
   return launch_reduce(c, loop_mode);
@@ -483,7 +514,7 @@ dim3 field_grid(const Solve &s) { return dim3((unsigned)((s.ni + 255) / 256), (u
 template <int LIMC> void launch_interp(dim3 g, hipStream_t st, int prec, FieldGeom fg, int ld, const double *q,
                                               double *ie, double *je) {
   if (prec == CATO_PRECISION_FP32_FLUX)
-    hipLaunchKernelGGL((k_interpolate<float, LIMC>), g, dim3(256), 0, st, fg, ld, q, ie, je);
+    hipLaunchKernelGGL((k_interpolate<f32x, LIMC>), g, dim3(256), 0, st, fg, ld, q, ie, je);
   else
     hipLaunchKernelGGL((k_interpolate<double, LIMC>), g, dim3(256), 0, st, fg, ld, q, ie, je);
 }
@@ -491,7 +522,7 @@ template <int LIMC> void launch_interp(dim3 g, hipStream_t st, int prec, FieldGe
 template <int F> void launch_flux(dim3 g, hipStream_t st, int prec, FieldGeom fg, int ld, const double *e,
                                          const double *p, int lm, double *fi, double *fj) {
   if (prec == CATO_PRECISION_FP32_FLUX)
-    hipLaunchKernelGGL((k_edge_flux<float, F>), g, dim3(256), 0, st, fg, ld, e, p, lm, fi, fj);
+    hipLaunchKernelGGL((k_edge_flux<f32x, F>), g, dim3(256), 0, st, fg, ld, e, p, lm, fi, fj);
   else
     hipLaunchKernelGGL((k_edge_flux<double, F>), g, dim3(256), 0, st, fg, ld, e, p, lm, fi, fj);
 }
@@ -503,6 +534,15 @@ extern "C" {
 
 const char *cato_last_error(void) { return g_last_error.c_str(); }
 int cato_abi_version(void) { return CATO_HIP_ABI_VERSION; }
+
+int cato_device_bytes(const cato_config *cfg, const cato_geom *geom, unsigned long long *bytes) {
+  if (!cfg || !geom || !bytes) return fail(CATO_ERR_INVALID, "null argument");
+  Solve s;
+  int rc = s.init(*cfg, *geom);
+  if (rc) return rc;
+  *bytes = (unsigned long long)s.device_bytes();
+  return CATO_OK;
+}
 int cato_num_tiles(void) { return kNumTiles; }
 int cato_tile_shape(int tile, int *tx, int *ty) {
   if (tile < 0 || tile >= kNumTiles) return fail(CATO_ERR_INVALID, "unknown tile id");
@@ -562,12 +602,13 @@ int cato_set_primitive(cato_ctx *c, const double *rho, const double *u, const do
   if (rc) return rc;
   Solve &s = c->s;
   double *stg = s.U[(s.cur + 1) % 3][0];
-  if ((rc = upload_field(c, s.prim[0], rho, ld, halo, stg)) || (rc = upload_field(c, s.prim[1], u, ld, halo, stg)) ||
-      (rc = upload_field(c, s.prim[2], v, ld, halo, stg)) || (rc = upload_field(c, s.prim[3], p, ld, halo, stg)))
+  double **P = s.P[0];
+  if ((rc = upload_field(c, P[0], rho, ld, halo, stg)) || (rc = upload_field(c, P[1], u, ld, halo, stg)) ||
+      (rc = upload_field(c, P[2], v, ld, halo, stg)) || (rc = upload_field(c, P[3], p, ld, halo, stg)))
     return rc;
   FieldGeom g = s.fgeom();
-  hipLaunchKernelGGL(k_prim_to_cons, field_grid(s), dim3(256), 0, c->stream, g, s.prim[0], s.prim[1], s.prim[2],
-                     s.prim[3], s.U[s.cur][0], s.U[s.cur][1], s.U[s.cur][2], s.U[s.cur][3]);
+  hipLaunchKernelGGL(k_prim_to_cons, field_grid(s), dim3(256), 0, c->stream, g, P[0], P[1], P[2], P[3],
+                     s.U[s.cur][0], s.U[s.cur][1], s.U[s.cur][2], s.U[s.cur][3]);
   HIPCK(hipGetLastError());
   s.prims_valid = true;
   return hip_check(hipStreamSynchronize(c->stream), "set_primitive");
@@ -583,6 +624,7 @@ int cato_set_conserved(cato_ctx *c, const double *r, const double *ru, const dou
   if ((rc = upload_field(c, U[0], r, ld, halo, stg)) || (rc = upload_field(c, U[1], ru, ld, halo, stg)) ||
       (rc = upload_field(c, U[2], rv, ld, halo, stg)) || (rc = upload_field(c, U[3], rE, ld, halo, stg)))
     return rc;
+  if ((rc = derive_primitives(c->stream, s))) return rc;
   s.prims_valid = false;
   return hip_check(hipStreamSynchronize(c->stream), "set_conserved");
 }
@@ -604,29 +646,16 @@ int cato_get_primitive(cato_ctx *c, double *rho, double *u, double *v, double *p
   if (rc) return rc;
   Solve &s = c->s;
   FieldGeom g = s.fgeom();
-  // scratch: the two non-current state buffers
+  // P[0]: primitives of U^n; scratch: the two non-current state buffers
+  double **P = s.P[0];
   double **T = s.U[(s.cur + 1) % 3];
-  double **T2 = s.U[(s.cur + 2) % 3];
-  if (s.prims_valid) {
-    hipLaunchKernelGGL(k_sound_speed, field_grid(s), dim3(256), 0, c->stream, g, s.prim[0], s.prim[3], T2[0]);
-    HIPCK(hipGetLastError());
-    double *stg = T[0];
-    if ((rc = download_field(c, rho, s.prim[0], ld, halo, stg)) ||
-        (rc = download_field(c, u, s.prim[1], ld, halo, stg)) ||
-        (rc = download_field(c, v, s.prim[2], ld, halo, stg)) ||
-        (rc = download_field(c, p, s.prim[3], ld, halo, stg)) || (rc = download_field(c, cs, T2[0], ld, halo, stg)))
-      return rc;
-  } else {
-    double **U = s.U[s.cur];
-    hipLaunchKernelGGL(k_derived, field_grid(s), dim3(256), 0, c->stream, g, U[0], U[1], U[2], U[3], T[0], T[1], T[2],
-                       T[3]);
-    HIPCK(hipGetLastError());
-    double *stg = T2[0];
-    if ((rc = download_field(c, rho, U[0], ld, halo, stg)) || (rc = download_field(c, u, T[0], ld, halo, stg)) ||
-        (rc = download_field(c, v, T[1], ld, halo, stg)) || (rc = download_field(c, p, T[2], ld, halo, stg)) ||
-        (rc = download_field(c, cs, T[3], ld, halo, stg)))
-      return rc;
-  }
+  hipLaunchKernelGGL(k_sound_speed, field_grid(s), dim3(256), 0, c->stream, g, P[0], P[3], T[1]);
+  HIPCK(hipGetLastError());
+  double *stg = T[0];
+  if ((rc = download_field(c, rho, P[0], ld, halo, stg)) || (rc = download_field(c, u, P[1], ld, halo, stg)) ||
+      (rc = download_field(c, v, P[2], ld, halo, stg)) || (rc = download_field(c, p, P[3], ld, halo, stg)) ||
+      (rc = download_field(c, cs, T[1], ld, halo, stg)))
+    return rc;
   return hip_check(hipStreamSynchronize(c->stream), "get_primitive");
 }
 
@@ -857,7 +886,7 @@ int cato_residual(cato_ctx *c, double *rho, double *u, double *v, double *p, dou
   const double *P[4] = {rho, u, v, p};
   for (int k = 0; k < 4; ++k)
     if ((rc = upload_field(c, Pbuf[k], P[k], ld, s.nh, nullptr))) return rc;
-  StageArgs a = s.stage_args(MODE_RESIDUAL, 0, 0, 0, -1, -1, (s.cur + 2) % 3, false, nullptr, 1.0, true);
+  StageArgs a = s.stage_args(MODE_RESIDUAL, 0, 0, 0, -1, -1, (s.cur + 2) % 3, 0, -1, false, nullptr, 1.0, false);
   for (int k = 0; k < 4; ++k) a.prim[k] = Pbuf[k];
   a.block_min = nullptr;
   a.block_err = nullptr;

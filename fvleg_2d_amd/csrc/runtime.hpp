@@ -32,14 +32,14 @@ struct Solve {
   StageLauncher launcher = nullptr;
   // device arrays (carved from one allocation by bind())
   double *U[3][4] = {};
-  double *prim[4] = {};
+  double *P[3][4] = {};      // primitive buffer sets; P[0] = primitives of U^n between steps
   double *dxi = nullptr, *lxi = nullptr, *dyj = nullptr, *lyj = nullptr;
   double *block_min = nullptr;
   double *result = nullptr;
   int *block_err = nullptr;
   LoopState *loop = nullptr;
   int cur = 0;              // which U[] holds U^n
-  bool prims_valid = false; // prim[] holds the primitives of U^n as given by the caller
+  bool prims_valid = false; // P[0] holds the caller's primitives (not yet checked by a stage)
 
   int init(const cato_config &c, const cato_geom &g);
   size_t field_elems() const;
@@ -47,8 +47,8 @@ struct Solve {
   void bind(char *base);
   int upload_geometry(hipStream_t s);
   FieldGeom fgeom() const;
-  StageArgs stage_args(int mode, double a, double b, double c, int in_buf, int u0_buf, int out_buf, bool last,
-                       const LoopState *lp, double dt_value, bool prims_from_array) const;
+  StageArgs stage_args(int mode, double a, double b, double c, int in_buf, int u0_buf, int out_buf, int p_in,
+                       int p_out, bool last, const LoopState *lp, double dt_value, bool check_input) const;
   std::vector<StageArgs> step_stages(const LoopState *lp, double dt_value) const;
 };
 
@@ -56,6 +56,7 @@ struct Solve {
 // A tightly packed host field (ld == ni, halo == 0) moves as ONE contiguous
 // copy, repacked on the device through `staging` (a free device field of the
 // same solve) when the device pitch differs; other layouts use a 2D copy.
+int derive_primitives(hipStream_t st, const Solve &s);
 int field_up(hipStream_t st, const Solve &s, double *dev, const double *host, int ld, int halo, double *staging);
 int field_down(hipStream_t st, const Solve &s, double *host, const double *dev, int ld, int halo, double *staging);
 

@@ -241,10 +241,10 @@ int admit_into(cato_server *S, Request &R, Solve &solve, int k, size_t off) {
   hipStream_t st = S->stream;
   const std::vector<double> *P[4] = {&R.rho, &R.u, &R.v, &R.p};
   for (int q = 0; q < 4; ++q)
-    if ((rc = field_up(st, *sv, sv->prim[q], P[q]->data(), R.ni, 0, sv->U[1][0]))) return rc;
+    if ((rc = field_up(st, *sv, sv->P[0][q], P[q]->data(), R.ni, 0, sv->U[1][0]))) return rc;
   FieldGeom fg = sv->fgeom();
   hipLaunchKernelGGL(k_prim_to_cons, dim3((unsigned)((R.ni + 255) / 256), (unsigned)R.nj), dim3(256), 0, st, fg,
-                     sv->prim[0], sv->prim[1], sv->prim[2], sv->prim[3], sv->U[0][0], sv->U[0][1], sv->U[0][2],
+                     sv->P[0][0], sv->P[0][1], sv->P[0][2], sv->P[0][3], sv->U[0][0], sv->U[0][1], sv->U[0][2],
                      sv->U[0][3]);
   sv->prims_valid = true;
   LoopState L{};
@@ -255,8 +255,8 @@ int admit_into(cato_server *S, Request &R, Solve &solve, int k, size_t off) {
   L.max_iterations = 0x7fffffffLL;
   if ((rc = hip_check(hipMemcpyAsync(sv->loop, &L, sizeof(L), hipMemcpyHostToDevice, st), "loop"))) return rc;
   int nb = sv->tiles_x * sv->tiles_y;
-  hipLaunchKernelGGL(k_cfl_partial, dim3((unsigned)nb), dim3(kThreads), 0, st, fg, 1, sv->prim[0], sv->prim[1],
-                     sv->prim[2], sv->prim[3], sv->block_min, sv->block_err);
+  hipLaunchKernelGGL(k_cfl_partial, dim3((unsigned)nb), dim3(kThreads), 0, st, fg, 1, sv->P[0][0], sv->P[0][1],
+                     sv->P[0][2], sv->P[0][3], sv->block_min, sv->block_err);
   hipLaunchKernelGGL(k_reduce_finalize, dim3(1), dim3(1024), 0, st, nb, sv->block_min, sv->block_err, sv->result,
                      sv->loop, 1);
   if ((rc = hip_check(hipGetLastError(), "admit kernels"))) return rc;

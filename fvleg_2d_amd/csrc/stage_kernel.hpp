@@ -1,18 +1,21 @@
 // Fused SSP-RK stage kernel for gfx950.
 //
 // One workgroup owns a TX x TY tile of interior cells.  In one pass it:
-//   1. builds the primitive variables (rho, u, v, p) of the tile plus an
-//      H-cell halo in LDS, recomputing them from the conserved state U_k
-//      (fluid_t%calculate_derived_quantities) and resolving ghost cells through
-//      the boundary conditions' per-axis index map (apply_primitive_bc);
+//   1. loads the primitive variables (rho, u, v, p) of U_k for the tile plus
+//      an H-cell halo into LDS, resolving ghost cells through the boundary
+//      conditions' per-axis index map (apply_primitive_bc);
 //   2. reconstructs the L/R states and evaluates every edge flux of the tile
 //      exactly once (muscl_*%interpolate_edge_values + get_<dir>flux);
 //   3. forms dU/dt per cell (flux_split_edges) and the SSP stage combination
 //      (fluid_t operators, mod_fluid.f90:593-669) and writes U_{k+1};
-//   4. on the last stage, recovers the new primitives and reduces the CFL
-//      time step and the sanity flags per workgroup (get_timestep, sanity_check).
-// HBM traffic per cell is the compulsory minimum: U_k (+U_0) read once,
-// U_{k+1} written once; geometry of rectilinear grids is two 1D arrays.
+//   4. recovers the primitives of U_{k+1} once per cell and stores them for
+//      the next stage (fluid_t%calculate_derived_quantities), and on the last
+//      stage reduces the CFL time step and the sanity flags per workgroup
+//      (get_timestep, sanity_check).
+// Each cell's primitives are computed once per stage instead of once per
+// tile that needs them as halo; the price is 32 B/cell of extra writes on a
+// kernel whose time is spent in FP64 division, not in HBM traffic.  The
+// geometry of rectilinear grids is two 1D arrays.
 #pragma once
 
 #include "physics.hpp"
@@ -65,22 +68,20 @@ __device__ inline int wave_or(int v) {
   return v;
 }
 
-// Primitive of halo'd cell (gi, gj) from the stored state, with BC mapping.
-__device__ inline Prim<double> load_prim(const StageArgs &a, int gi, int gj, int &err) {
+// Primitives of halo'd cell (gi, gj), with the BC mapping.  The stored
+// primitives of a state are those fluid_t%calculate_derived_quantities gives
+// for it (written by the stage that produced the state), or the caller's on
+// the first stage after cato_set_primitive.
+__device__ inline Prim<double> load_prim(const StageArgs &a, int gi, int gj) {
   bool fu, fv;
   int si = map_axis(gi, a.ni, a.bc[0], a.bc[1], fu);
   int sj = map_axis(gj, a.nj, a.bc[2], a.bc[3], fv);
   size_t off = (size_t)(sj - 1) * a.pitch + (size_t)(si - 1);
   Prim<double> pr;
-  if (a.prim_from_array) {
-    pr.rho = a.prim[0][off];
-    pr.u = a.prim[1][off];
-    pr.v = a.prim[2][off];
-    pr.p = a.prim[3][off];
-  } else {
-    pr = cons_to_prim(a.uk[0][off], a.uk[1][off], a.uk[2][off], a.uk[3][off], a.gamma, a.gamma - 1.0, nullptr);
-  }
-  if (pr.rho != pr.rho || pr.u != pr.u || pr.v != pr.v || pr.p != pr.p) err |= 1;
+  pr.rho = a.prim[0][off];
+  pr.u = a.prim[1][off];
+  pr.v = a.prim[2][off];
+  pr.p = a.prim[3][off];
   if (fu) pr.u = -pr.u;
   if (fv) pr.v = -pr.v;
   return pr;
@@ -144,9 +145,10 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
     int gi = i0 - H + lx, gj = j0 - H + ly;
     Prim<double> pr{0.0, 0.0, 0.0, 0.0};
     if (gi >= 1 - nh && gi <= ni + nh && gj >= 1 - nh && gj <= nj + nh) {
-      int e2 = 0;
-      pr = load_prim(a, gi, gj, e2);
-      if (gi >= 1 && gi <= ni && gj >= 1 && gj <= nj) err |= e2;
+      pr = load_prim(a, gi, gj);
+      if (a.check_input && gi >= 1 && gi <= ni && gj >= 1 && gj <= nj &&
+          (pr.rho != pr.rho || pr.u != pr.u || pr.v != pr.v || pr.p != pr.p))
+        err |= 1;
     }
     sp[0][ly][lx] = (R)pr.rho;
     sp[1][ly][lx] = (R)pr.u;
@@ -242,10 +244,17 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
       nv[v] = r;
       a.out[v][off] = r;
     }
+    if (a.mode == MODE_RESIDUAL) continue;
+    // primitives of the new state for the next stage (assign_fluid ->
+    // calculate_derived_quantities, mod_fluid.f90:564-591) and its NaN check
+    double cs;
+    Prim<double> q = cons_to_prim(nv[0], nv[1], nv[2], nv[3], a.gamma, a.gamma - 1.0, &cs);
+    a.prim_out[0][off] = q.rho;
+    a.prim_out[1][off] = q.u;
+    a.prim_out[2][off] = q.v;
+    a.prim_out[3][off] = q.p;
+    if (q.rho != q.rho || q.u != q.u || q.v != q.v || q.p != q.p) err |= 1;
     if (a.last_stage) {
-      double cs;
-      Prim<double> q = cons_to_prim(nv[0], nv[1], nv[2], nv[3], a.gamma, a.gamma - 1.0, &cs);
-      if (q.rho != q.rho || q.u != q.u || q.v != q.v || q.p != q.p) err |= 1;
       if (q.rho < 0.0) err |= 2;
       if (q.p < 0.0) err |= 4;
       // get_timestep (mod_fluid.f90:554-558); min_dx/min_dy = |dx|, |dy| on rectilinear cells

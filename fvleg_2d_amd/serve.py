@@ -16,6 +16,19 @@ from . import _native as N
 from .solver import _f64f, _p
 
 
+def device_bytes(pb) -> int:
+    """HBM footprint of one solve of problem ``pb`` (what the server charges against its budget)."""
+    lib = N.load()
+    nh = pb.cfg.n_ghost
+    nx, ny = _f64f(pb.node_x), _f64f(pb.node_y)
+    ni, nj = nx.shape[0] - 1 - 2 * nh, nx.shape[1] - 1 - 2 * nh
+    cfg = pb.cfg.to_c()
+    geom = N.CatoGeom(ni, nj, _p(nx), _p(ny))
+    out = C.c_ulonglong()
+    N.check(lib.cato_device_bytes(C.byref(cfg), C.byref(geom), C.byref(out)), "cato_device_bytes")
+    return int(out.value)
+
+
 class Server:
     def __init__(self, hbm_budget_bytes: float, max_slots: int = 64, precision: str = "fp64",
                  steps_per_tick: int = 8):

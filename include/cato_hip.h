@@ -175,6 +175,11 @@ int cato_batch_get_conserved(cato_batch *b, int k, double *rho, double *rho_u, d
 int cato_batch_run(cato_batch *b, const double *max_time, const double *contour_interval_dt, long long max_iterations,
                    int use_graph, long long *iterations, double *times, int *errors);
 
+/* Device memory one solve of this configuration and grid occupies (its HBM
+ * footprint, as the serving form charges it against its budget).  Needs no
+ * device. */
+int cato_device_bytes(const cato_config *cfg, const cato_geom *geom, unsigned long long *bytes);
+
 /* ---- serving form: continuous batching under an HBM budget ----
  * No counterpart in the reference (one solve per program run there).  A
  * server owns one device arena of hbm_budget_bytes, allocated at create; the

@@ -90,3 +90,15 @@ def test_struct_offsets_match_c_compiler(tmp_path):
         assert int(out[cname]) == ctypes.sizeof(py), cname
         for f in py._fields_:
             assert int(out[f"{cname}.{f[0]}"]) == getattr(py, f[0]).offset, (cname, f[0])
+
+
+def test_device_bytes_needs_no_device():
+    """The HBM footprint query validates the configuration on the host only."""
+    from fvleg_2d_amd.problems import riemann2d
+    from fvleg_2d_amd.serve import device_bytes
+
+    small = riemann2d(n=32, seed=1, flux="HLLC", limiter="minmod")
+    big = riemann2d(n=64, seed=1, flux="HLLC", limiter="MLP5")
+    b_small, b_big = device_bytes(small), device_bytes(big)
+    assert b_small >= 24 * 8 * 32 * 32  # three conserved + three primitive buffer sets, FP64
+    assert b_big > 3 * b_small

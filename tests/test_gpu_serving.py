@@ -12,7 +12,7 @@ import pytest
 
 from fvleg_2d_amd import Solver
 from fvleg_2d_amd.problems import riemann2d, sod
-from fvleg_2d_amd.serve import Server
+from fvleg_2d_amd.serve import Server, device_bytes
 
 pytestmark = pytest.mark.gpu
 
@@ -39,8 +39,7 @@ def alone(pb):
 
 
 def field_bytes(pb):
-    pitch = (pb.ni + 15) // 16 * 16
-    return 16 * 8 * pitch * pb.nj
+    return device_bytes(pb)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32_flux"])
