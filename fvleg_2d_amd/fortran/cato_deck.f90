@@ -1,0 +1,328 @@
+!> The pycato input deck of the reference: the input.ini entries the hot path
+!> needs (src/lib/io/mod_input.f90:200-410) and the HDF5 initial-condition
+!> file (pycato/generate_initial_grids.py:603-672, read by
+!> src/lib/grid/grid_block_2d.f90:194-265 and src/lib/fluid/mod_fluid.f90:256-381).
+!>
+!> HDF5 is reached through its C library (ISO_C_BINDING), so no Fortran HDF5
+!> modules are needed.  Datasets are stored transposed (HDF5 dims (nj, ni)),
+!> which is exactly a Fortran (ni, nj) array in memory.
+module cato_deck
+  use, intrinsic :: iso_c_binding
+  implicit none
+  private
+
+  integer, parameter :: rk = c_double
+  public :: cato_input_t, read_input_ini, read_initial_conditions, write_state
+
+  !> Deck entries of the hot path (names as in input.ini)
+  type :: cato_input_t
+    character(len=256) :: title = ''
+    ! [time]
+    real(rk) :: max_time = 0.0_rk
+    real(rk) :: initial_delta_t = 0.0_rk
+    logical :: use_constant_delta_t = .false.
+    real(rk) :: cfl = 0.1_rk
+    character(len=32) :: integration_strategy = ''
+    integer :: max_iterations = huge(1)
+    ! [reference_state]
+    logical :: non_dimensionalize = .true.
+    real(rk) :: reference_length = 1.0_rk, reference_density = 1.0_rk, reference_velocity = 1.0_rk
+    ! [physics]
+    real(rk) :: polytropic_index = 5.0_rk / 3.0_rk
+    ! [scheme]
+    character(len=32) :: flux_solver = ''
+    character(len=32) :: spatial_reconstruction = 'MUSCL'
+    character(len=32) :: limiter = 'minmod'
+    logical :: apply_low_mach_fix = .true.
+    ! [initial_conditions]
+    logical :: read_from_file = .true.
+    character(len=256) :: initial_condition_file = ''
+    ! [boundary_conditions]
+    character(len=32) :: plus_x = 'periodic', minus_x = 'periodic', plus_y = 'periodic', minus_y = 'periodic'
+    ! [io]
+    real(rk) :: contour_interval_dt = huge(1.0_rk)
+    ! [grid]
+    integer :: n_ghost_layers = 0
+  end type
+
+  integer(c_int), parameter :: H5F_ACC_RDONLY = 0, H5F_ACC_TRUNC = 2
+  integer(c_int64_t), parameter :: H5P_DEFAULT = 0, H5S_ALL = 0
+  integer(c_int64_t), bind(C, name="H5T_NATIVE_DOUBLE_g") :: h5t_native_double
+  integer(c_int64_t), bind(C, name="H5T_NATIVE_INT_g") :: h5t_native_int
+
+  interface
+    function H5open() bind(C, name="H5open") result(r)
+      import :: c_int
+      integer(c_int) :: r
+    end function
+    function H5Fopen(name, flags, fapl) bind(C, name="H5Fopen") result(h)
+      import :: c_char, c_int, c_int64_t
+      character(kind=c_char), intent(in) :: name(*)
+      integer(c_int), value :: flags
+      integer(c_int64_t), value :: fapl
+      integer(c_int64_t) :: h
+    end function
+    function H5Fcreate(name, flags, fcpl, fapl) bind(C, name="H5Fcreate") result(h)
+      import :: c_char, c_int, c_int64_t
+      character(kind=c_char), intent(in) :: name(*)
+      integer(c_int), value :: flags
+      integer(c_int64_t), value :: fcpl, fapl
+      integer(c_int64_t) :: h
+    end function
+    function H5Fclose(h) bind(C, name="H5Fclose") result(r)
+      import :: c_int, c_int64_t
+      integer(c_int64_t), value :: h
+      integer(c_int) :: r
+    end function
+    function H5Dopen2(loc, name, dapl) bind(C, name="H5Dopen2") result(h)
+      import :: c_char, c_int64_t
+      integer(c_int64_t), value :: loc, dapl
+      character(kind=c_char), intent(in) :: name(*)
+      integer(c_int64_t) :: h
+    end function
+    function H5Dclose(h) bind(C, name="H5Dclose") result(r)
+      import :: c_int, c_int64_t
+      integer(c_int64_t), value :: h
+      integer(c_int) :: r
+    end function
+    function H5Dget_space(d) bind(C, name="H5Dget_space") result(h)
+      import :: c_int64_t
+      integer(c_int64_t), value :: d
+      integer(c_int64_t) :: h
+    end function
+    function H5Sget_simple_extent_ndims(s) bind(C, name="H5Sget_simple_extent_ndims") result(n)
+      import :: c_int, c_int64_t
+      integer(c_int64_t), value :: s
+      integer(c_int) :: n
+    end function
+    function H5Sget_simple_extent_dims(s, dims, maxdims) bind(C, name="H5Sget_simple_extent_dims") result(n)
+      import :: c_int, c_int64_t, c_ptr
+      integer(c_int64_t), value :: s
+      integer(c_int64_t), intent(out) :: dims(*)
+      type(c_ptr), value :: maxdims
+      integer(c_int) :: n
+    end function
+    function H5Sclose(s) bind(C, name="H5Sclose") result(r)
+      import :: c_int, c_int64_t
+      integer(c_int64_t), value :: s
+      integer(c_int) :: r
+    end function
+    function H5Screate_simple(rank, dims, maxdims) bind(C, name="H5Screate_simple") result(h)
+      import :: c_int, c_int64_t, c_ptr
+      integer(c_int), value :: rank
+      integer(c_int64_t), intent(in) :: dims(*)
+      type(c_ptr), value :: maxdims
+      integer(c_int64_t) :: h
+    end function
+    function H5Dread(d, mtype, mspace, fspace, plist, buf) bind(C, name="H5Dread") result(r)
+      import :: c_int, c_int64_t, c_ptr
+      integer(c_int64_t), value :: d, mtype, mspace, fspace, plist
+      type(c_ptr), value :: buf
+      integer(c_int) :: r
+    end function
+    function H5Dcreate2(loc, name, dtype, space, lcpl, dcpl, dapl) bind(C, name="H5Dcreate2") result(h)
+      import :: c_char, c_int64_t
+      integer(c_int64_t), value :: loc, dtype, space, lcpl, dcpl, dapl
+      character(kind=c_char), intent(in) :: name(*)
+      integer(c_int64_t) :: h
+    end function
+    function H5Dwrite(d, mtype, mspace, fspace, plist, buf) bind(C, name="H5Dwrite") result(r)
+      import :: c_int, c_int64_t, c_ptr
+      integer(c_int64_t), value :: d, mtype, mspace, fspace, plist
+      type(c_ptr), value :: buf
+      integer(c_int) :: r
+    end function
+  end interface
+
+contains
+
+  pure function lower(s) result(r)
+    character(len=*), intent(in) :: s
+    character(len=len(s)) :: r
+    integer :: k
+    r = s
+    do k = 1, len(s)
+      if (r(k:k) >= 'A' .and. r(k:k) <= 'Z') r(k:k) = achar(iachar(r(k:k)) + 32)
+    end do
+  end function
+
+  pure function unquote(s) result(r)
+    character(len=*), intent(in) :: s
+    character(len=len(s)) :: r
+    integer :: n
+    r = adjustl(s)
+    n = len_trim(r)
+    if (n >= 2) then
+      if ((r(1:1) == "'" .and. r(n:n) == "'") .or. (r(1:1) == '"' .and. r(n:n) == '"')) r = r(2:n - 1)
+    end if
+  end function
+
+  logical function to_logical(s)
+    character(len=*), intent(in) :: s
+    character(len=len(s)) :: t
+    t = lower(adjustl(s))
+    to_logical = (t(1:1) == 't' .or. t(1:2) == '.t' .or. t(1:1) == 'y' .or. t(1:1) == '1')
+  end function
+
+  real(rk) function to_real(s, key)
+    character(len=*), intent(in) :: s, key
+    integer :: ios
+    read(s, *, iostat=ios) to_real
+    if (ios /= 0) error stop 'input.ini: bad real value for '//trim(key)
+  end function
+
+  !> Read the [section] key = value pairs the hot path uses (mod_input.f90:200-410).
+  subroutine read_input_ini(filename, inp)
+    character(len=*), intent(in) :: filename
+    type(cato_input_t), intent(out) :: inp
+    character(len=1024) :: line
+    character(len=64) :: section, key
+    character(len=960) :: val
+    integer :: u, ios, eq, c
+    open(newunit=u, file=filename, status='old', action='read', iostat=ios)
+    if (ios /= 0) error stop 'input.ini: cannot open '//trim(filename)
+    section = ''
+    do
+      read(u, '(a)', iostat=ios) line
+      if (ios /= 0) exit
+      c = scan(line, ';#')
+      if (c > 0) line = line(1:c - 1)
+      line = adjustl(line)
+      if (len_trim(line) == 0) cycle
+      if (line(1:1) == '[') then
+        section = lower(line(2:index(line, ']') - 1))
+        cycle
+      end if
+      eq = index(line, '=')
+      if (eq == 0) cycle
+      key = lower(adjustl(line(1:eq - 1)))
+      val = unquote(trim(line(eq + 1:)))
+      select case (trim(section)//'.'//trim(key))
+      case ('general.title'); inp%title = val
+      case ('time.max_time'); inp%max_time = to_real(val, key)
+      case ('time.initial_delta_t'); inp%initial_delta_t = to_real(val, key)
+      case ('time.use_constant_delta_t'); inp%use_constant_delta_t = to_logical(val)
+      case ('time.cfl'); inp%cfl = to_real(val, key)
+      case ('time.integration_strategy'); inp%integration_strategy = val
+      case ('time.max_iterations'); inp%max_iterations = int(to_real(val, key))
+      case ('reference_state.non_dimensionalize'); inp%non_dimensionalize = to_logical(val)
+      case ('reference_state.reference_length'); inp%reference_length = to_real(val, key)
+      case ('reference_state.reference_density'); inp%reference_density = to_real(val, key)
+      case ('reference_state.reference_velocity'); inp%reference_velocity = to_real(val, key)
+      case ('physics.polytropic_index'); inp%polytropic_index = to_real(val, key)
+      case ('scheme.flux_solver'); inp%flux_solver = val
+      case ('scheme.spatial_reconstruction'); inp%spatial_reconstruction = val
+      case ('scheme.limiter'); inp%limiter = val
+      case ('scheme.apply_low_mach_fix'); inp%apply_low_mach_fix = to_logical(val)
+      case ('initial_conditions.read_from_file'); inp%read_from_file = to_logical(val)
+      case ('initial_conditions.initial_condition_file'); inp%initial_condition_file = val
+      case ('boundary_conditions.plus_x'); inp%plus_x = val
+      case ('boundary_conditions.minus_x'); inp%minus_x = val
+      case ('boundary_conditions.plus_y'); inp%plus_y = val
+      case ('boundary_conditions.minus_y'); inp%minus_y = val
+      case ('io.contour_interval_dt'); inp%contour_interval_dt = to_real(val, key)
+      case ('grid.n_ghost_layers'); inp%n_ghost_layers = int(to_real(val, key))
+      end select
+    end do
+    close(u)
+  end subroutine
+
+  function cstr(s) result(r)
+    character(len=*), intent(in) :: s
+    character(kind=c_char, len=len_trim(s) + 1) :: r
+    r = trim(s)//c_null_char
+  end function
+
+  !> 2D dataset of doubles, stored transposed: returns a Fortran (n1, n2) array.
+  subroutine read_2d(f, name, a)
+    integer(c_int64_t), intent(in) :: f
+    character(len=*), intent(in) :: name
+    real(rk), allocatable, target, intent(out) :: a(:, :)
+    integer(c_int64_t) :: d, s, dims(2)
+    integer(c_int) :: r
+    d = H5Dopen2(f, cstr(name), H5P_DEFAULT)
+    if (d < 0) error stop 'initial conditions: missing dataset '//trim(name)
+    s = H5Dget_space(d)
+    if (H5Sget_simple_extent_ndims(s) /= 2) error stop 'initial conditions: dataset is not 2D: '//trim(name)
+    r = H5Sget_simple_extent_dims(s, dims, c_null_ptr)
+    allocate(a(dims(2), dims(1)))
+    r = H5Dread(d, h5t_native_double, H5S_ALL, H5S_ALL, H5P_DEFAULT, c_loc(a))
+    if (r < 0) error stop 'initial conditions: cannot read '//trim(name)
+    r = H5Sclose(s)
+    r = H5Dclose(d)
+  end subroutine
+
+  !> The deck: node coordinates and cell primitives incl. ghost cells (grid_block_2d.f90:194-265).
+  subroutine read_initial_conditions(filename, n_ghost, x, y, rho, u, v, p)
+    character(len=*), intent(in) :: filename
+    integer, intent(out) :: n_ghost
+    real(rk), allocatable, intent(out) :: x(:, :), y(:, :), rho(:, :), u(:, :), v(:, :), p(:, :)
+    integer(c_int64_t) :: f, d
+    integer(c_int), target :: nh(1)
+    integer(c_int) :: r
+    r = H5open()
+    f = H5Fopen(cstr(filename), H5F_ACC_RDONLY, H5P_DEFAULT)
+    if (f < 0) error stop 'initial conditions: cannot open '//trim(filename)
+    d = H5Dopen2(f, cstr('/n_ghost_layers'), H5P_DEFAULT)
+    if (d < 0) error stop 'initial conditions: missing /n_ghost_layers'
+    r = H5Dread(d, h5t_native_int, H5S_ALL, H5S_ALL, H5P_DEFAULT, c_loc(nh))
+    r = H5Dclose(d)
+    n_ghost = nh(1)
+    call read_2d(f, '/x', x)
+    call read_2d(f, '/y', y)
+    call read_2d(f, '/density', rho)
+    call read_2d(f, '/x_velocity', u)
+    call read_2d(f, '/y_velocity', v)
+    call read_2d(f, '/pressure', p)
+    r = H5Fclose(f)
+  end subroutine
+
+  subroutine write_2d(f, name, a)
+    integer(c_int64_t), intent(in) :: f
+    character(len=*), intent(in) :: name
+    real(rk), target, intent(in) :: a(:, :)
+    integer(c_int64_t) :: s, d, dims(2)
+    integer(c_int) :: r
+    dims = [int(size(a, 2), c_int64_t), int(size(a, 1), c_int64_t)]
+    s = H5Screate_simple(2_c_int, dims, c_null_ptr)
+    d = H5Dcreate2(f, cstr(name), h5t_native_double, s, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT)
+    if (d < 0) error stop 'output: cannot create dataset '//trim(name)
+    r = H5Dwrite(d, h5t_native_double, H5S_ALL, H5S_ALL, H5P_DEFAULT, c_loc(a))
+    if (r < 0) error stop 'output: cannot write '//trim(name)
+    r = H5Dclose(d)
+    r = H5Sclose(s)
+  end subroutine
+
+  subroutine write_scalar(f, name, val)
+    integer(c_int64_t), intent(in) :: f
+    character(len=*), intent(in) :: name
+    real(rk), intent(in) :: val
+    real(rk), target :: a(1, 1)
+    a(1, 1) = val
+    call write_2d(f, name, a)
+  end subroutine
+
+  !> Interior state of a solve: conserved and primitive fields, time and iteration count.
+  subroutine write_state(filename, time, iteration, rho, rho_u, rho_v, rho_E, u, v, p)
+    character(len=*), intent(in) :: filename
+    real(rk), intent(in) :: time
+    integer, intent(in) :: iteration
+    real(rk), intent(in) :: rho(:, :), rho_u(:, :), rho_v(:, :), rho_E(:, :), u(:, :), v(:, :), p(:, :)
+    integer(c_int64_t) :: f
+    integer(c_int) :: r
+    r = H5open()
+    f = H5Fcreate(cstr(filename), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT)
+    if (f < 0) error stop 'output: cannot create '//trim(filename)
+    call write_scalar(f, '/time', time)
+    call write_scalar(f, '/iteration', real(iteration, rk))
+    call write_2d(f, '/density', rho)
+    call write_2d(f, '/rho_u', rho_u)
+    call write_2d(f, '/rho_v', rho_v)
+    call write_2d(f, '/rho_E', rho_E)
+    call write_2d(f, '/x_velocity', u)
+    call write_2d(f, '/y_velocity', v)
+    call write_2d(f, '/pressure', p)
+    r = H5Fclose(f)
+  end subroutine
+
+end module cato_deck

@@ -1,0 +1,153 @@
+!> Coarray-free CATO driver on the HIP hot path: reads the reference's
+!> input.ini and pycato HDF5 deck, runs the time loop of src/main.f90:117-198
+!> through the plugin objects, and writes the final state.
+!>
+!>   cato_hip_host input.ini [output.h5] [--device-loop] [--dry-run]
+!>
+!> --device-loop runs the same loop on the device (cato_run) instead of one
+!> host call per step; the results are identical.  --dry-run reads and checks
+!> the deck, prints the configuration and stops before touching a device.
+program cato_hip_host
+  use, intrinsic :: iso_c_binding
+  use cato_hip_api
+  use cato_deck
+  use cato_hip_plugins
+  implicit none
+
+  type(cato_input_t) :: input
+  type(hip_grid_t), target :: grid
+  type(hip_solve_t), target :: dev
+  type(hip_fluid_t) :: fluid
+  type(cato_config_t) :: cfg
+  real(rk), allocatable :: x(:, :), y(:, :), rho(:, :), u(:, :), v(:, :), p(:, :)
+  real(rk), allocatable :: r(:, :), ru(:, :), rv(:, :), rE(:, :), cs(:, :)
+  character(len=512) :: ini_file, out_file, arg
+  logical :: device_loop, dry_run
+  integer :: nh, nargs, k, error_code, iteration
+  real(rk) :: time, new_time, delta_t, next_output_time, contour_interval_dt, master_time
+  integer(c_long_long) :: iters
+  integer(c_int) :: e
+
+  nargs = command_argument_count()
+  if (nargs < 1) then
+    write(*, '(a)') 'usage: cato_hip_host input.ini [output.h5] [--device-loop]'
+    error stop 1
+  end if
+  call get_command_argument(1, ini_file)
+  out_file = 'final_state.h5'
+  device_loop = .false.
+  dry_run = .false.
+  do k = 2, nargs
+    call get_command_argument(k, arg)
+    if (trim(arg) == '--device-loop') then
+      device_loop = .true.
+    else if (trim(arg) == '--dry-run') then
+      dry_run = .true.
+    else
+      out_file = arg
+    end if
+  end do
+
+  call read_input_ini(trim(ini_file), input)
+  if (.not. input%read_from_file) then
+    write(*, '(a)') 'Error: only initial conditions read from file are supported'
+    error stop 1
+  end if
+  if (input%non_dimensionalize .and. (input%reference_length /= 1.0_rk .or. input%reference_density /= 1.0_rk &
+                                      .or. input%reference_velocity /= 1.0_rk)) then
+    write(*, '(a)') 'Error: reference_state scaling other than 1 is not supported by this host'
+    error stop 1
+  end if
+  cfg = config_from_input(input)
+
+  ! grid_block_2d_t%initialize from the deck (grid_block_2d.f90:194-265)
+  call read_initial_conditions(trim(input%initial_condition_file), nh, x, y, rho, u, v, p)
+  if (nh /= cfg%n_ghost_layers) then
+    write(*, '(a, i0, a, i0)') 'Error: deck has n_ghost_layers = ', nh, ', the scheme needs ', cfg%n_ghost_layers
+    error stop 1
+  end if
+  grid%n_halo = nh
+  grid%ni = size(rho, 1) - 2 * nh
+  grid%nj = size(rho, 2) - 2 * nh
+  grid%ihi = grid%ni
+  grid%jhi = grid%nj
+  allocate(grid%node_x(1 - nh:grid%ni + nh + 1, 1 - nh:grid%nj + nh + 1), source=x)
+  allocate(grid%node_y(1 - nh:grid%ni + nh + 1, 1 - nh:grid%nj + nh + 1), source=y)
+
+  if (dry_run) then
+    write(*, '(a, i0, a, i0, a, i0)') 'deck: ni = ', grid%ni, ', nj = ', grid%nj, ', n_ghost_layers = ', nh
+    write(*, '(a, 8(1x, i0))') 'config: flux limiter integrator bcs(+x -x +y -y) n_ghost =', cfg%flux, cfg%limiter, &
+      cfg%integrator, cfg%bc_plus_x, cfg%bc_minus_x, cfg%bc_plus_y, cfg%bc_minus_y, cfg%n_ghost_layers
+    write(*, '(a, 3(1x, es23.16))') 'gamma cfl max_time =', cfg%gamma, cfg%cfl, input%max_time
+    write(*, '(a, 2(1x, es23.16))') 'x(1-nh,1-nh) y(ni+nh+1,nj+nh+1) =', grid%node_x(1 - nh, 1 - nh), &
+      grid%node_y(grid%ni + nh + 1, grid%nj + nh + 1)
+    write(*, '(a, 2(1x, es23.16))') 'density(1,1) pressure(ni,nj) =', rho(1 + nh, 1 + nh), p(grid%ni + nh, grid%nj + nh)
+    stop
+  end if
+  call dev%create(cfg, grid)
+  call fluid%initialize(dev, rho, u, v, p)
+  write(*, '(a, i0, a, i0, a)') 'CATO (HIP hot path): ', grid%ni, ' x ', grid%nj, ' cells, '// &
+    trim(input%flux_solver)//' / '//trim(input%limiter)//' / '//trim(input%integration_strategy)
+
+  contour_interval_dt = input%contour_interval_dt
+  time = 0.0_rk
+  iteration = 0
+  if (device_loop .and. .not. input%use_constant_delta_t .and. input%initial_delta_t > 0.0_rk) then
+    write(*, '(a)') 'Error: --device-loop does not support a given first time step (initial_delta_t)'
+    error stop 1
+  end if
+  if (device_loop) then
+    call check_rc(cato_run(dev%ctx, input%max_time, int(input%max_iterations, c_long_long), contour_interval_dt, &
+                           merge(input%initial_delta_t, 0.0_rk, input%use_constant_delta_t), iters, time, e), &
+                  'cato_run')
+    iteration = int(iters)
+    error_code = e
+  else
+    ! main.f90:138-198 with the contour-interval clip of the driver time
+    next_output_time = 0.0_rk + contour_interval_dt
+    master_time = 0.0_rk
+    error_code = ALL_OK
+    do while (time < input%max_time .and. iteration < input%max_iterations)
+      if (input%use_constant_delta_t .or. (iteration == 0 .and. input%initial_delta_t > 0.0_rk)) then
+        delta_t = input%initial_delta_t
+      else
+        delta_t = fluid%get_timestep()
+      end if
+      call fluid%integrate(delta_t, error_code)
+      master_time = master_time + delta_t  ! master_puppeteer%time
+      new_time = master_time
+      if (error_code /= ALL_OK) exit
+      if (abs(time - next_output_time) < epsilon(1.0_rk)) next_output_time = next_output_time + contour_interval_dt
+      if (new_time > next_output_time) then
+        delta_t = abs(next_output_time - time)
+        new_time = time + delta_t
+      end if
+      time = new_time
+      iteration = iteration + 1
+    end do
+  end if
+  if (error_code /= ALL_OK) write(*, '(a, i0)') 'Something went wrong in the time integration, error code ', error_code
+
+  allocate(r(grid%ni, grid%nj), ru(grid%ni, grid%nj), rv(grid%ni, grid%nj), rE(grid%ni, grid%nj))
+  allocate(cs(grid%ni, grid%nj))
+  deallocate(rho, u, v, p)
+  allocate(rho(grid%ni, grid%nj), u(grid%ni, grid%nj), v(grid%ni, grid%nj), p(grid%ni, grid%nj))
+  call fluid%get_conserved(r, ru, rv, rE)
+  call fluid%get_primitives(rho, u, v, p, cs)
+  call write_state(trim(out_file), time, iteration, r, ru, rv, rE, u, v, p)
+  write(*, '(a, es22.15, a, i0, a)') 'Done: time = ', time, ', ', iteration, ' iterations -> '//trim(out_file)
+  call fluid%finalize()
+  if (error_code /= ALL_OK) error stop 1
+
+contains
+
+  subroutine check_rc(rc, what)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    if (rc /= CATO_OK) then
+      write(*, '(a)') 'Error: '//what//': '//cato_error_message()
+      error stop 1
+    end if
+  end subroutine
+
+end program cato_hip_host
