@@ -20,6 +20,15 @@
 #define EPS DBL_EPSILON
 #define TINY DBL_MIN
 
+/* ORACLE_OMP (liboracle_omp.so, CPU-baseline timing only): rows of the
+ * cell/edge loops in parallel; every cell is computed exactly as in the serial
+ * build, so the results are the same. */
+#ifdef ORACLE_OMP
+#define OR_PAR _Pragma("omp parallel for schedule(static)")
+#else
+#define OR_PAR
+#endif
+
 static inline double fmax2(double a, double b) { return (b > a) ? b : a; }
 static inline double fmin2(double a, double b) { return (b < a) ? b : a; }
 static inline double fmax3(double a, double b, double c) { return fmax2(fmax2(a, b), c); }
@@ -114,6 +123,7 @@ void or_prim_to_cons(const or_config *c, const double *rho, const double *u, con
   const int ni = c->ni, nj = c->nj, nh = c->nh;
   const double gm1 = c->gamma - 1.0;
   const double one_over = 1.0 / gm1; /* field_div_real_1d multiplies by 1/x (field_2d.f90:1169-1190) */
+  OR_PAR
   for (int j = 1; j <= nj; ++j)
     for (int i = 1; i <= ni; ++i) {
       size_t k = CI(i, j);
@@ -128,6 +138,7 @@ void or_prim_to_cons(const or_config *c, const double *rho, const double *u, con
 
 void or_sound_speed(const or_config *c, const double *rho, const double *p, double *cs) {
   const int ni = c->ni, nj = c->nj, nh = c->nh;
+  OR_PAR
   for (int j = 1; j <= nj; ++j)
     for (int i = 1; i <= ni; ++i) {
       size_t k = CI(i, j);
@@ -139,6 +150,7 @@ void or_derived_quantities(const or_config *c, const double *rho, const double *
                            const double *rho_E, double *u, double *v, double *p, double *cs) {
   const int ni = c->ni, nj = c->nj, nh = c->nh;
   const double gm1 = c->gamma - 1.0;
+  OR_PAR
   for (int j = 1; j <= nj; ++j)
     for (int i = 1; i <= ni; ++i) {
       size_t k = CI(i, j);
@@ -288,6 +300,7 @@ static inline double tvd_phi(int lim, double r) {
 
 static void recon_tvd2(const or_config *c, int dir, const double *q, double *ev) {
   const int ni = c->ni, nj = c->nj, nh = c->nh;
+  OR_PAR
   for (int j = 0; j <= nj; ++j)
     for (int i = 0; i <= ni; ++i) {
       double q0, qm, q1, q2;
@@ -310,6 +323,7 @@ static void recon_mlp(const or_config *c, int dir, int order, const double *q, d
   const int ni = c->ni, nj = c->nj, nh = c->nh;
   const double SMALL = 1e-30;
 #define Q(a, b) q[CI((a), (b))]
+  OR_PAR
   for (int j = 0; j <= nj; ++j)
     for (int i = 0; i <= ni; ++i) {
       double dm, dp, d3, dm3 = 0.0, d5 = 0.0;
@@ -719,6 +733,7 @@ void or_edge_fluxes(const or_config *c, const or_geometry *g, const double *rho,
     const double *er = dir == 0 ? rho_i : rho_j, *eu = dir == 0 ? u_i : u_j;
     const double *evv = dir == 0 ? v_i : v_j, *ep = dir == 0 ? p_i : p_j;
     int i0 = dir == 0 ? 0 : 1, j0 = dir == 0 ? 1 : 0;
+    OR_PAR
     for (int j = j0; j <= nj; ++j)
       for (int i = i0; i <= ni; ++i) {
         edge_state s = {er[EI(0, i, j)], eu[EI(0, i, j)], evv[EI(0, i, j)], ep[EI(0, i, j)],
@@ -799,6 +814,7 @@ void or_flux_split_edges(const or_config *c, const or_geometry *g, const double 
   const size_t ncell = (size_t)(ni + 2 * nh) * (nj + 2 * nh);
   for (int v = 0; v < 4; ++v) memset(dst[v], 0, ncell * sizeof(double));
   for (int v = 0; v < 4; ++v)
+    OR_PAR
     for (int j = 1; j <= nj; ++j)
       for (int i = 1; i <= ni; ++i) {
         size_t cc = CI(i, j);
@@ -816,6 +832,7 @@ void or_flux_split_edges(const or_config *c, const or_geometry *g, const double 
         dst[v][cc] = -fl;
       }
   for (int v = 0; v < 4; ++v)
+    OR_PAR
     for (int j = 1; j <= nj; ++j)
       for (int i = 1; i <= ni; ++i) {
         size_t cc = CI(i, j);
@@ -856,6 +873,12 @@ double or_get_timestep(const or_config *c, const or_geometry *g, const or_fluid 
   const int ni = c->ni, nj = c->nj, nh = c->nh;
   double m = 0.0;
   int first = 1;
+#ifdef ORACLE_OMP
+  /* timing build: the same minimum as the serial scan whenever no value is NaN */
+  m = INFINITY;
+  first = 0;
+#pragma omp parallel for schedule(static) reduction(min : m)
+#endif
   for (int j = 1; j <= nj; ++j)
     for (int i = 1; i <= ni; ++i) {
       size_t k = CI(i, j);
@@ -884,17 +907,25 @@ static void fluid_to_stage(const or_fluid *f, stage_buf *s, size_t n) {
 
 static int nan_check(const or_config *c, const double *x) {
   const int ni = c->ni, nj = c->nj, nh = c->nh;
+  int found = 0;
+#ifdef ORACLE_OMP
+#pragma omp parallel for schedule(static) reduction(| : found)
+#endif
   for (int j = 1; j <= nj; ++j)
     for (int i = 1; i <= ni; ++i)
-      if (isnan(x[CI(i, j)])) return 1;
-  return 0;
+      found |= isnan(x[CI(i, j)]) ? 1 : 0;
+  return found;
 }
 static int neg_check(const or_config *c, const double *x) {
   const int ni = c->ni, nj = c->nj, nh = c->nh;
+  int found = 0;
+#ifdef ORACLE_OMP
+#pragma omp parallel for schedule(static) reduction(| : found)
+#endif
   for (int j = 1; j <= nj; ++j)
     for (int i = 1; i <= ni; ++i)
-      if (x[CI(i, j)] < 0.0) return 1;
-  return 0;
+      found |= (x[CI(i, j)] < 0.0) ? 1 : 0;
+  return found;
 }
 
 /* L(U) into d[0..3] (time_derivative, mod_fluid.f90:460-516): prims of U get BC-filled in place */
@@ -906,6 +937,7 @@ static void time_derivative(const or_config *c, const or_geometry *g, stage_buf 
 static void combine1(const or_config *c, const stage_buf *base, double **L, double s, stage_buf *out) {
   const int ni = c->ni, nj = c->nj, nh = c->nh;
   for (int v = 0; v < 4; ++v)
+    OR_PAR
     for (int j = 1; j <= nj; ++j)
       for (int i = 1; i <= ni; ++i) {
         size_t k = CI(i, j);
@@ -917,6 +949,7 @@ static void combine2(const or_config *c, const stage_buf *U0, double a, const st
                      double cdt, stage_buf *out) {
   const int ni = c->ni, nj = c->nj, nh = c->nh;
   for (int v = 0; v < 4; ++v)
+    OR_PAR
     for (int j = 1; j <= nj; ++j)
       for (int i = 1; i <= ni; ++i) {
         size_t k = CI(i, j);

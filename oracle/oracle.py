@@ -17,7 +17,10 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+# CATO_ORACLE_OMP=1 selects the OpenMP build (rows in parallel, same results),
+# used only to time the CPU baseline on all host cores.
+_OMP = os.environ.get("CATO_ORACLE_OMP") == "1"
+LIB_PATH = os.path.join(HERE, "liboracle_omp.so" if _OMP else "liboracle.so")
 
 FLUX_IDS = {
     "AUSMPW+": 0, "SLAU": 1, "SLAU2": 2, "SD-SLAU": 3, "SD-SLAU2": 4,
@@ -57,7 +60,7 @@ def build(force: bool = False) -> str:
     """Compile the oracle with the reference-faithful flags (no FMA contraction)."""
     src = os.path.join(HERE, "cato_oracle.c")
     if force or not os.path.exists(LIB_PATH) or os.path.getmtime(LIB_PATH) < os.path.getmtime(src):
-        subprocess.check_call(["make", "-s", "-C", HERE, "liboracle.so"])
+        subprocess.check_call(["make", "-s", "-C", HERE, os.path.basename(LIB_PATH)])
     return LIB_PATH
 
 
