@@ -50,7 +50,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_flux"])
-    ap.add_argument("--tile", type=int, default=-1)
+    ap.add_argument("--tile", type=int, default=-1, help="-1: chosen by the autotuner")
     ap.add_argument("--forms", default="all", choices=["all", "none"])
     ap.add_argument("--mode", default="bench", choices=["bench", "copy"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -106,7 +106,14 @@ def timed_kh(n, precision, steps, warmup, tile, profile_only=False):
     from fvleg_2d_amd.problems import kelvin_helmholtz
 
     pb = kelvin_helmholtz(n=n, precision=precision)
+    tuned = None
+    if tile < 0 and not profile_only:
+        from fvleg_2d_amd.autotune import tune
+
+        tile, times = tune(pb)
+        tuned = {"tile": tile, "stage_ms_by_tile": times}
     pb.cfg.tile = tile
+    pb.meta["autotune"] = tuned
     with Solver(pb.cfg, pb.node_x, pb.node_y) as s:
         s.set_primitive(pb.rho, pb.u, pb.v, pb.p)
         s.advance(warmup)
@@ -226,7 +233,8 @@ def main():
             "workload": f"kelvin_helmholtz_{pb.ni}x{pb.nj}_periodic_SLAU_vanleer_ssp_rk3_cfl0.5_{args.precision}",
             "cells": cells,
             "bytes_per_cell_step": STEP_BYTES_PER_CELL[pb.cfg.integrator],
-            "tile": args.tile,
+            "tile": pb.cfg.tile,
+            "autotune": pb.meta.get("autotune"),
         },
         "roofline": roofline(pb, stage_ms, launches, args.precision, args.n),
     }
@@ -237,6 +245,8 @@ def main():
             "value": cells * args.steps / wall32,
             "unit": "cell-updates/s",
             "ms_per_step": wall32 / args.steps * 1e3,
+            "tile": pb32.cfg.tile,
+            "autotune": pb32.meta.get("autotune"),
             "roofline": roofline(pb32, ms32, l32, "fp32_flux", args.n),
             "difference_from_fp64_state": rel_errors(state, state32),
         }

@@ -47,6 +47,12 @@ def batched(pbs, precision="fp64", tile=-1, use_graph=True):
 def measure_batched(n_solves=256, seed=2024, side_range=(64, 512), max_time=0.2, tile=-1, check=True):
     pbs = batch_riemann(n_solves=n_solves, seed=seed, side_range=side_range, max_time=max_time)
     cells = int(sum(pb.ni * pb.nj for pb in pbs))
+    tuned = None
+    if tile < 0:
+        from .autotune import tune_batch
+
+        tile, times = tune_batch(pbs)
+        tuned = {"tile": tile, "s_per_step_by_tile": times}
     # warm the kernels of both paths on a tiny batch
     warm = batch_riemann(n_solves=4, seed=1, side_range=(32, 40), max_time=0.01)
     batched(warm, tile=tile)
@@ -69,6 +75,8 @@ def measure_batched(n_solves=256, seed=2024, side_range=(64, 512), max_time=0.2,
         "batched_cell_updates_per_s": cell_updates / b["run_s"],
         "bit_identical_to_plain": identical,
         "errors": int(np.count_nonzero(b["errors"])),
+        "tile": tile,
+        "autotune": tuned,
     }
 
 

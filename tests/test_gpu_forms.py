@@ -117,3 +117,46 @@ def test_fp32_flux_form_kh_conservation():
         mag = np.abs(u0[k]).sum()
         say("fp32-flux KH total", k, tot0, tot1)
         assert abs(tot1 - tot0) <= 1e-10 * mag
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_flux"])
+def test_results_independent_of_tile_and_autotune(precision):
+    """Every tile shape computes each cell with the same arithmetic: identical states after 20 steps;
+    the autotuner returns one of them with a positive time for each."""
+    import copy
+
+    from fvleg_2d_amd.autotune import num_tiles, tune
+
+    pb = kelvin_helmholtz(n=96, steps=20, precision=precision)
+    states = []
+    for t in range(num_tiles()):
+        cfg = copy.copy(pb.cfg)
+        cfg.tile = t
+        with Solver(cfg, pb.node_x, pb.node_y) as s:
+            s.set_primitive(pb.rho, pb.u, pb.v, pb.p)
+            s.advance(20)
+            states.append(s.conserved())
+    for st in states[1:]:
+        for v in range(4):
+            assert np.array_equal(st[v], states[0][v])
+    best, times = tune(pb, steps=4, warmup=1)
+    say("autotune", precision, best, times)
+    assert best in times and all(v > 0 for v in times.values())
+
+
+def test_batch_results_independent_of_tile():
+    from fvleg_2d_amd.autotune import num_tiles
+
+    pbs = mixed_batch()[:5]
+    got = []
+    for t in range(num_tiles()):
+        with Batch(pbs, tile=t) as b:
+            for k, pb in enumerate(pbs):
+                b.set_primitive(k, pb.rho, pb.u, pb.v, pb.p)
+            mt = [pb.max_time for pb in pbs]
+            b.run(mt, mt)
+            got.append([b.conserved(k) for k in range(len(pbs))])
+    for g in got[1:]:
+        for k in range(len(pbs)):
+            for v in range(4):
+                assert np.array_equal(g[k][v], got[0][k][v])

@@ -287,3 +287,63 @@ def test_run_loop_terminates_on_clipped_time(oracle):
         tt += dt
         n2 += 1
     assert n <= n2 + 5
+
+
+def test_eos_closed_forms_and_mach_filter(oracle):
+    """primitive -> conserved -> primitive (equation_of_state.f90:286-321, mod_fluid.f90:564-591) against the
+    ideal-gas closed forms; the Mach filter zeroes |u/cs| < 1e-13 only."""
+    import ctypes as C
+
+    rng = np.random.default_rng(21)
+    cfg = oracle.Config(6, 5, gamma=1.4)
+    nodes = oracle.uniform_nodes(6, 5, cfg.nh)
+    g = oracle.Geometry(cfg, *nodes)
+    rho, u, v, p = (cfg.field() for _ in range(4))
+    nh = cfg.nh
+    inner = (slice(nh, -nh), slice(nh, -nh))
+    rho[inner] = rng.uniform(0.1, 2.0, (6, 5))
+    u[inner] = rng.uniform(-1.0, 1.0, (6, 5))
+    v[inner] = rng.uniform(-1.0, 1.0, (6, 5))
+    p[inner] = rng.uniform(0.1, 1.5, (6, 5))
+    cs_true = np.sqrt(1.4 * p[inner] / rho[inner])
+    u[nh, nh] = 1e-15 * cs_true[0, 0]  # |M| < 1e-13: filtered
+    v[nh + 1, nh] = 5e-13 * cs_true[1, 0]  # |M| > 1e-13: kept
+    f = oracle.Fluid(cfg, g, rho, u, v, p)
+    E = p[inner] / 0.4 + 0.5 * rho[inner] * (u[inner] ** 2 + v[inner] ** 2)
+    assert np.allclose(f.rho_E[inner], E, rtol=2e-15, atol=0)
+    assert np.array_equal(f.rho_u[inner], rho[inner] * u[inner])
+    out = [cfg.field() for _ in range(4)]
+    c = cfg.c()
+    oracle.lib().or_derived_quantities(C.byref(c), *[oracle._p(a) for a in (f.rho, f.rho_u, f.rho_v, f.rho_E)],
+                                       *[oracle._p(a) for a in out])
+    du, dv, dp, dcs = out
+    assert du[nh, nh] == 0.0
+    assert dv[nh + 1, nh] != 0.0
+    mask = np.ones((6, 5), bool)
+    mask[0, 0] = False
+    assert np.allclose(du[inner][mask], u[inner][mask], rtol=1e-13, atol=1e-15)
+    assert np.allclose(dv[inner], v[inner], rtol=1e-13, atol=1e-15)
+    assert np.allclose(dp[inner], p[inner], rtol=1e-13)
+    assert np.allclose(dcs[inner], np.sqrt(1.4 * dp[inner] / rho[inner]), rtol=1e-15)
+
+
+def test_timestep_is_the_cfl_minimum(oracle):
+    """get_timestep (mod_fluid.f90:554-558): min over cells of cfl / ((|u|+cs)/dx + (|v|+cs)/dy),
+    brute force on a small non-uniform, non-square grid."""
+    rng = np.random.default_rng(5)
+    ni, nj = 7, 5
+    cfg = oracle.Config(ni, nj, cfl=0.37, gamma=1.4)
+    nodes = oracle.uniform_nodes(ni, nj, cfg.nh, (0.0, 1.3), (0.0, 0.4))
+    g = oracle.Geometry(cfg, *nodes)
+    rho, u, v, p = (cfg.field() for _ in range(4))
+    nh = cfg.nh
+    inner = (slice(nh, -nh), slice(nh, -nh))
+    rho[inner] = rng.uniform(0.2, 2.0, (ni, nj))
+    u[inner] = rng.uniform(-2.0, 2.0, (ni, nj))
+    v[inner] = rng.uniform(-2.0, 2.0, (ni, nj))
+    p[inner] = rng.uniform(0.1, 1.5, (ni, nj))
+    f = oracle.Fluid(cfg, g, rho, u, v, p)
+    dx, dy = 1.3 / ni, 0.4 / nj
+    cs = np.sqrt(1.4 * p[inner] / rho[inner])
+    per_cell = 0.37 / ((np.abs(u[inner]) + cs) / dx + (np.abs(v[inner]) + cs) / dy)
+    assert abs(f.get_timestep() - per_cell.min()) <= 1e-14 * per_cell.min()
