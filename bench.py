@@ -110,7 +110,7 @@ def timed_kh(n, precision, steps, warmup, tile, profile_only=False):
     if tile < 0 and not profile_only:
         from fvleg_2d_amd.autotune import tune
 
-        tile, times = tune(pb)
+        tile, times = tune(pb, steps=30, warmup=50)
         tuned = {"tile": tile, "stage_ms_by_tile": times}
     pb.cfg.tile = tile
     pb.meta["autotune"] = tuned

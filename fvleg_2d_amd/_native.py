@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libcato_hip.so")
 CATO_OK = 0
 
 FLUX = {"AUSMPW+": 0, "SLAU": 1, "SLAU2": 2, "SD-SLAU": 3, "SD-SLAU2": 4,
-        "Roe": 5, "S-Roe": 5, "SP-Roe": 5, "AUSM+": 6, "AUSM+-u": 6, "HLLC": 7}
+        "Roe": 5, "S-Roe": 5, "SP-Roe": 5, "AUSM+": 6, "AUSM+-u": 6, "HLLC": 7, "M-AUSMPW+": 8}
 LIMITER = {"minmod": 0, "superbee": 1, "van_leer": 2, "vanleer": 2, "MLP3": 3, "MLP5": 4}
 BC = {"periodic": 0, "zero_gradient": 1, "symmetry": 2, "mirror": 2, "reflection": 2}
 INTEGRATOR = {"ssp_rk2": 0, "ssp_rk3": 1, "ssp_rk33": 1, "rk(1,3)": 2, "rk(1,4)": 3, "rk(1,5)": 4,

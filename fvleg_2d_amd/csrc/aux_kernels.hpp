@@ -287,16 +287,24 @@ static __global__ void k_edge_flux(FieldGeom g, int ld, const double *__restrict
     int gj = dir == 0 ? j + c : j + a;
     return (R)p[(size_t)(gi + g.nh - 1) + (size_t)(gj + g.nh - 1) * ld];
   };
+  EdgeState<R> sb = s;
+  if constexpr (needs_superbee_states<FLUX>()) {  // superbee states follow the 8 arrays
+    const double *q = b + 8 * ne;
+    sb.rL = (R)q[0 * 2 * ne + e]; sb.rR = (R)q[0 * 2 * ne + e + 1];
+    sb.uL = (R)q[1 * 2 * ne + e]; sb.uR = (R)q[1 * 2 * ne + e + 1];
+    sb.vL = (R)q[2 * 2 * ne + e]; sb.vR = (R)q[2 * 2 * ne + e + 1];
+    sb.pL = (R)q[3 * 2 * ne + e]; sb.pR = (R)q[3 * 2 * ne + e + 1];
+  }
   PressNbr<R> pn;
   pn.p00 = P(0, 0); pn.p10 = P(1, 0); pn.p20 = P(2, 0); pn.pm10 = P(-1, 0);
   pn.p0p = P(0, 1); pn.p0m = P(0, -1); pn.p1p = P(1, 1); pn.p1m = P(1, -1);
   R f[4];
   if (!dir) {
-    edge_flux<R, FLUX, 0>((R)g.gamma, low_mach_fix, s, pn, f);
+    edge_flux<R, FLUX, 0>((R)g.gamma, low_mach_fix, s, pn, f, &sb);
     double *d = iflux + 4 * ((size_t)i + (size_t)(g.ni + 1) * (j - 1));
     for (int v = 0; v < 4; ++v) d[v] = (double)f[v];
   } else {
-    edge_flux<R, FLUX, 1>((R)g.gamma, low_mach_fix, s, pn, f);
+    edge_flux<R, FLUX, 1>((R)g.gamma, low_mach_fix, s, pn, f, &sb);
     double *d = jflux + 4 * ((size_t)(i - 1) + (size_t)g.ni * j);
     for (int v = 0; v < 4; ++v) d[v] = (double)f[v];
   }

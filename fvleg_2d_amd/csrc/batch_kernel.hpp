@@ -26,8 +26,9 @@ struct BatchSolveRef {
 struct BatchStageArgs {
   const StageArgs *args;   // per solve, this stage
   const int *variant;      // per solve: flux * LIM_COUNT + limiter
-  const BatchItem *items;  // work list, grouped by solve
+  const BatchItem *items;  // work list, grouped by variant and solve
   const int *count;        // number of live entries of items (device-resident: graph replays see updates)
+  int *cursor;             // kNumXcd work cursors of this stage: zero at launch, reset by the finalize kernel
 };
 
 template <typename R, int TX, int TY> constexpr size_t batch_smem_bytes() {
@@ -60,28 +61,41 @@ template <typename R, int TX, int TY>
 __global__ __launch_bounds__(kThreads, (batch_waves_per_simd<R, TX, TY>())) void batch_stage_kernel(BatchStageArgs b) {
   __shared__ __attribute__((aligned(16))) char smem[batch_smem_bytes<R, TX, TY>()];
   __shared__ RedSmem red;
-  // XCD-aware split: blocks of XCD x take the x-th eighth of the work list.
-  const int G = gridDim.x;
+  __shared__ int s_item;
+  // The work list is cut into kNumXcd contiguous slices.  Workgroups of XCD x
+  // take items of slice x through its cursor (neighbouring tiles stay in one
+  // L2), then help the other slices: items differ in cost by variant, so a
+  // static split would leave XCDs idle while others finish.
   const int nitems = *b.count;
-  const int x = blockIdx.x % kNumXcd, lb = blockIdx.x / kNumXcd;
-  const int nbx = (G - x + kNumXcd - 1) / kNumXcd;  // blocks on this XCD
-  const int lo = (int)(((long long)nitems * x) / kNumXcd), hi = (int)(((long long)nitems * (x + 1)) / kNumXcd);
-  for (int w = lo + lb; w < hi; w += nbx) {
-    const BatchItem it = b.items[w];
-    const StageArgs &a = b.args[it.solve];
-    if (a.loop && a.loop->done) continue;
-    const double dt = a.loop ? a.loop->dt : a.dt_value;
-    __syncthreads();  // the previous item's LDS is free
-    switch (b.variant[it.solve]) {
-      CATO_BATCH_FLUX(0)
-      CATO_BATCH_FLUX(1)
-      CATO_BATCH_FLUX(2)
-      CATO_BATCH_FLUX(3)
-      CATO_BATCH_FLUX(4)
-      CATO_BATCH_FLUX(5)
-      CATO_BATCH_FLUX(6)
-      CATO_BATCH_FLUX(7)
-      default: break;
+  const int x0 = blockIdx.x % kNumXcd;
+  for (int k = 0; k < kNumXcd; ++k) {
+    const int x = (x0 + k) % kNumXcd;
+    const int lo = (int)(((long long)nitems * x) / kNumXcd), hi = (int)(((long long)nitems * (x + 1)) / kNumXcd);
+    while (true) {
+      __syncthreads();  // the previous item (its LDS and s_item) is finished
+      if (threadIdx.x == 0) {
+        const int w = lo + atomicAdd(&b.cursor[x], 1);
+        s_item = (w < hi) ? w : -1;
+      }
+      __syncthreads();
+      const int w = s_item;
+      if (w < 0) break;
+      const BatchItem it = b.items[w];
+      const StageArgs &a = b.args[it.solve];
+      if (a.loop && a.loop->done) continue;
+      const double dt = a.loop ? a.loop->dt : a.dt_value;
+      switch (b.variant[it.solve]) {
+        CATO_BATCH_FLUX(0)
+        CATO_BATCH_FLUX(1)
+        CATO_BATCH_FLUX(2)
+        CATO_BATCH_FLUX(3)
+        CATO_BATCH_FLUX(4)
+        CATO_BATCH_FLUX(5)
+        CATO_BATCH_FLUX(6)
+        CATO_BATCH_FLUX(7)
+        CATO_BATCH_FLUX(8)
+        default: break;
+      }
     }
   }
 }
@@ -89,10 +103,14 @@ __global__ __launch_bounds__(kThreads, (batch_waves_per_simd<R, TX, TY>())) void
 #undef CATO_BATCH_CASE
 #undef CATO_BATCH_FLUX
 
-// Per-solve end-of-step reduction: one workgroup per solve.
-static __global__ __launch_bounds__(kThreads) void batch_finalize_kernel(const BatchSolveRef *refs, int mode) {
+// Per-solve end-of-step reduction: one workgroup per solve.  Workgroup 0 also
+// zeroes the stage work cursors for the next step (all stages have finished).
+static __global__ __launch_bounds__(kThreads) void batch_finalize_kernel(const BatchSolveRef *refs, int mode,
+                                                                       int *cursors, int ncursors) {
   __shared__ double sm[kThreads / 64];
   __shared__ int se[kThreads / 64];
+  if (blockIdx.x == 0)
+    for (int k = threadIdx.x; k < ncursors; k += kThreads) cursors[k] = 0;
   const BatchSolveRef r = refs[blockIdx.x];
   LoopState *L = r.loop;
   if (mode == 2 && L->done) return;

@@ -64,6 +64,8 @@ int BatchEngine::init(int n, const cato_config *cfgs, const cato_geom *geoms, in
   off += align256((size_t)n * sizeof(BatchSolveRef));
   count_off = off;
   off += 256;
+  cursor_off = off;
+  off += align256((size_t)nstages * kNumXcd * sizeof(int));
   total_bytes = off;
   BCK(hipMalloc(&mem, total_bytes));
   for (int k = 0; k < n; ++k) {
@@ -82,15 +84,24 @@ int BatchEngine::init(int n, const cato_config *cfgs, const cato_geom *geoms, in
   for (int k = 0; k < n; ++k) {
     const Solve &sv = solves[k];
     int nt = sv.tiles_x * sv.tiles_y;
-    for (int t = 0; t < nt; ++t) items.push_back({k, t});
     variant[k] = sv.cfg.flux * LIM_COUNT + sv.cfg.limiter;
     refs[k] = BatchSolveRef{sv.block_min, sv.block_err, sv.loop, nt, 0};
+  }
+  // work list grouped by variant: each XCD slice then runs few kernel variants,
+  // which keeps the instruction cache warm
+  order.resize(n);
+  for (int k = 0; k < n; ++k) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return variant[x] < variant[y]; });
+  for (int k : order) {
+    int nt = solves[k].tiles_x * solves[k].tiles_y;
+    for (int t = 0; t < nt; ++t) items.push_back({k, t});
   }
   BCK(hipMemcpyAsync(mem + variant_off, variant.data(), n * sizeof(int), hipMemcpyHostToDevice, stream));
   BCK(hipMemcpyAsync(mem + items_off, items.data(), items.size() * sizeof(BatchItem), hipMemcpyHostToDevice,
                      stream));
   BCK(hipMemcpyAsync(mem + refs_off, refs.data(), n * sizeof(BatchSolveRef), hipMemcpyHostToDevice, stream));
   BCK(hipMemcpyAsync(mem + count_off, &nitems, sizeof(int), hipMemcpyHostToDevice, stream));
+  BCK(hipMemsetAsync(mem + cursor_off, 0, (size_t)nstages * kNumXcd * sizeof(int), stream));
   hipDeviceProp_t prop;
   int dev = 0;
   BCK(hipGetDevice(&dev));
@@ -144,11 +155,13 @@ int BatchEngine::launch_step(bool first) {
     b.variant = reinterpret_cast<const int *>(mem + variant_off);
     b.items = reinterpret_cast<const BatchItem *>(mem + items_off);
     b.count = reinterpret_cast<const int *>(mem + count_off);
+    b.cursor = reinterpret_cast<int *>(mem + cursor_off) + (size_t)s * kNumXcd;
     launcher(dim3((unsigned)grid), stream, b);
     BCK(hipGetLastError());
   }
   hipLaunchKernelGGL(batch_finalize_kernel, dim3((unsigned)n), dim3(kThreads), 0, stream,
-                     reinterpret_cast<const BatchSolveRef *>(mem + refs_off), 2);
+                     reinterpret_cast<const BatchSolveRef *>(mem + refs_off), 2, reinterpret_cast<int *>(mem + cursor_off),
+                     nstages * kNumXcd);
   BCK(hipGetLastError());
   return CATO_OK;
 }
@@ -164,7 +177,7 @@ int BatchEngine::compact(const std::vector<LoopState> &H) {
   if (live == live_solves) return CATO_OK;
   live_solves = live;
   live_items.clear();
-  for (int k = 0; k < n; ++k) {
+  for (int k : order) {
     if (H[k].done) continue;
     int nt = solves[k].tiles_x * solves[k].tiles_y;
     for (int t = 0; t < nt; ++t) live_items.push_back({k, t});
@@ -191,6 +204,7 @@ int BatchEngine::run(const double *max_time, const double *contour_dt, long long
     L[k].max_iterations = max_iterations;
   }
   BCK(hipMemcpyAsync(mem + loops_off, L.data(), n * sizeof(LoopState), hipMemcpyHostToDevice, stream));
+  BCK(hipMemsetAsync(mem + cursor_off, 0, (size_t)nstages * kNumXcd * sizeof(int), stream));  // outside any graph
   int rc = upload_args();  // synchronises, so L may go
   if (rc) return rc;
   if ((rc = compact(L))) return rc;  // every solve live again (a batch may be run more than once)

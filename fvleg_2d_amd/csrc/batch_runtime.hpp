@@ -16,8 +16,9 @@ struct BatchEngine {
   hipStream_t stream = nullptr;
   char *mem = nullptr;
   size_t total_bytes = 0;
-  size_t loops_off = 0, args_off = 0, variant_off = 0, items_off = 0, refs_off = 0, count_off = 0;
+  size_t loops_off = 0, args_off = 0, variant_off = 0, items_off = 0, refs_off = 0, count_off = 0, cursor_off = 0;
   std::vector<BatchItem> live_items;  // host copy of the current work list
+  std::vector<int> order;             // solves ordered by kernel variant
   int live_solves = 0, live_count = 0;
   int precision = 0, tile = 0, nstages = 0, nitems = 0, grid = 0;
   BatchLauncher launcher = nullptr;

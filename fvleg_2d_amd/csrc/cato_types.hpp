@@ -14,7 +14,8 @@ enum FluxId {
   FLUX_ROE = CATO_FLUX_ROE,
   FLUX_AUSM_PLUS = CATO_FLUX_AUSM_PLUS,
   FLUX_HLLC = CATO_FLUX_HLLC,
-  FLUX_COUNT = 8
+  FLUX_M_AUSMPW_PLUS = CATO_FLUX_M_AUSMPW_PLUS,
+  FLUX_COUNT = 9
 };
 
 enum LimId {

@@ -17,6 +17,7 @@
 
 #include <cfloat>
 #include <cstdint>
+#include <type_traits>
 
 #include "cato_types.hpp"
 
@@ -115,6 +116,28 @@ __device__ inline Prim<double> cons_to_prim(double r, double ru, double rv, doub
   if (!(absd(v) >= lim) && absd(op_div(v, cs)) < MF) v = 0.0;
   if (!(absd(u) >= lim) && absd(op_div(u, cs)) < MF) u = 0.0;
   if (cs_out) *cs_out = cs;
+  return Prim<double>{r, u, v, p};
+}
+
+// 1/x in FP64 to about 1 ulp: hardware estimate plus two Newton steps (the
+// precision form's epilogue; the FP64 form divides).
+__device__ inline double rcp64(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  return __builtin_fma(r, e, r);
+}
+
+// Primitive recovery of the precision form: the same formulas in FP64 through
+// one reciprocal of rho, without the field-operator flushes and the Mach
+// filter (both act far below FP32 resolution); cs only when asked.
+__device__ inline Prim<double> cons_to_prim_fast(double r, double ru, double rv, double rE, double gamma, double gm1,
+                                                 int want_cs, double *cs_out) {
+  const double ir = rcp64(r);
+  const double u = ru * ir, v = rv * ir;
+  const double p = (r * gm1) * (rE * ir - 0.5 * (u * u + v * v));
+  *cs_out = want_cs ? __builtin_sqrt(gamma * p * ir) : 0.0;
   return Prim<double>{r, u, v, p};
 }
 
@@ -545,9 +568,119 @@ __device__ inline void flux_hllc(R gamma, const EdgeState<R> &s, R out[4]) {
   }
 }
 
+// M-AUSMPW+: m_ausmpw_plus_solver.fypp:437-738 (the path the reference runs: AUSMPW+ pressure
+// splitting without the alpha term, w from the normal pressure ratio), get_f 829-862 with its
+// 1e-10 flushes, phi_L_half / phi_R_half 790-827 blending in the superbee states (sb), and the
+// 1e3*epsilon flushes of the assembled flux 715-735.
+template <typename R> __device__ inline R m_phi_L_half(R phi_L, R phi_R, R phi_L_sb, R a) {
+  if (absd(phi_R - phi_L) < R(1) || absd(phi_L_sb - phi_L) < Lim<R>::eps) return phi_L;
+  return phi_L + fmax2(R(0), (phi_R - phi_L) * (phi_L_sb - phi_L)) / ((phi_R - phi_L) * absd(phi_L_sb - phi_L)) *
+                     fmin2(fmin2(a, R(0.5) * absd(phi_R - phi_L)), absd(phi_L_sb - phi_L));
+}
+template <typename R> __device__ inline R m_phi_R_half(R phi_L, R phi_R, R phi_R_sb, R a) {
+  if (absd(phi_L - phi_R) < Lim<R>::eps || absd(phi_R_sb - phi_R) < R(1)) return phi_R;
+  return phi_R + fmax2(R(0), (phi_L - phi_R) * (phi_R_sb - phi_R)) / ((phi_L - phi_R) * absd(phi_R_sb - phi_R)) *
+                     fmin2(fmin2(a, R(0.5) * absd(phi_L - phi_R)), absd(phi_R_sb - phi_R));
+}
+
+template <typename R, int DIR>
+__device__ inline void flux_m_ausmpw(R gamma, int low_mach_fix, const EdgeState<R> &s, const EdgeState<R> &sb,
+                                     const PressNbr<R> &pn, R out[4]) {
+  const R n_x = DIR == 0 ? R(1) : R(0), n_y = DIR == 0 ? R(0) : R(1);
+  const R LOW_MACH = R(0.1);
+  const R FEPS = R(Lim<double>::eps * 1e3);
+  R rho_L = s.rL, u_LHS = s.uL, v_LHS = s.vL, p_L = s.pL;
+  R rho_R = s.rR, u_RHS = s.uR, v_RHS = s.vR, p_R = s.pR;
+  R U_L = DIR == 0 ? u_LHS : v_LHS, U_R = DIR == 0 ? u_RHS : v_RHS;
+  R V_L = DIR == 0 ? v_LHS : u_LHS, V_R = DIR == 0 ? v_RHS : u_RHS;
+  R cs_L = sqrtr(gamma * p_L / rho_L);
+  R cs_R = sqrtr(gamma * p_R / rho_R);
+  R M_LHS = sqrtr(sq(u_LHS) + sq(v_LHS)) / cs_L;
+  R M_RHS = sqrtr(sq(u_RHS) + sq(v_RHS)) / cs_R;
+  if (low_mach_fix && (absd(M_LHS) < LOW_MACH || absd(M_RHS) < LOW_MACH)) {
+    R f_M = fmin2(R(1), fmax2(M_LHS, M_RHS));
+    R ulf = R(0.5) * (u_LHS + u_RHS) + f_M * R(0.5) * (u_LHS - u_RHS);
+    R urf = R(0.5) * (u_LHS + u_RHS) - f_M * R(0.5) * (u_LHS - u_RHS);
+    R vlf = R(0.5) * (v_LHS + v_RHS) + f_M * R(0.5) * (v_LHS - v_RHS);
+    R vrf = R(0.5) * (v_LHS + v_RHS) - f_M * R(0.5) * (v_LHS - v_RHS);
+    u_LHS = ulf; u_RHS = urf; v_LHS = vlf; v_RHS = vrf;
+  }
+  R H_L = (gamma / (gamma - R(1))) * (p_L / rho_L) + R(0.5) * (sq(u_LHS) + sq(v_LHS));
+  R H_R = (gamma / (gamma - R(1))) * (p_R / rho_R) + R(0.5) * (sq(u_RHS) + sq(v_RHS));
+  R H_normal = fmin2(H_L - R(0.5) * sq(V_L), H_R - R(0.5) * sq(V_R));
+  R c_s = sqrtr(R(2) * ((gamma - R(1)) / (gamma + R(1))) * H_normal);
+  R c_half = (R(0.5) * (U_L + U_R) < R(0)) ? sq(c_s) / fmax2(absd(U_R), c_s) : sq(c_s) / fmax2(absd(U_L), c_s);
+  R M_L = U_L / c_half;
+  R M_R = U_R / c_half;
+  R M_L_plus = (absd(M_L) > R(1)) ? R(0.5) * (M_L + absd(M_L)) : R(0.25) * sq(M_L + R(1));
+  R M_R_minus = (absd(M_R) > R(1)) ? R(0.5) * (M_R - absd(M_R)) : -(R(0.25) * sq(M_R - R(1)));
+  R P_R_minus = (absd(M_R) > R(1)) ? R(0.5) * (R(1) - copysignr(R(1), M_R)) : R(0.25) * sq(M_R - R(1)) * (R(2) + M_R);
+  R P_L_plus = (absd(M_L) > R(1)) ? R(0.5) * (R(1) + copysignr(R(1), M_L)) : R(0.25) * sq(M_L + R(1)) * (R(2) - M_L);
+  R p_s = p_L * P_L_plus + p_R * P_R_minus;
+  R mt = fmin2((p_L / p_R), (p_R / p_L));
+  R w = R(1) - (mt * mt * mt);
+  // get_f neighbours as AUSMPW+: i-dir p1 = (0,-1),(1,-1), p2 = (0,+1),(1,+1); j-dir the transverse +1 then -1
+  R p1L = DIR == 0 ? pn.p0m : pn.p0p, p1R = DIR == 0 ? pn.p1m : pn.p1p;
+  R p2L = DIR == 0 ? pn.p0p : pn.p0m, p2R = DIR == 0 ? pn.p1p : pn.p1m;
+  R f_L, f_R;
+  if (absd(p_s) < Lim<R>::tiny) {
+    f_L = R(0);
+    f_R = R(0);
+  } else {
+    const R FE = R(1e-10);
+    R mn4 = fmin2(fmin2(fmin2(p1L, p1R), p2L), p2R);
+    R min_term = sq(fmin2(R(1), mn4 / fmin2(p_L, p_R)));
+    if (absd(R(1) - min_term) < FE) min_term = R(1);
+    R fl = (p_L / p_s) - R(1);
+    if (absd(fl) < FE) fl = R(0);
+    f_L = fl * min_term;
+    R fr = (p_R / p_s) - R(1);
+    if (absd(fr) < FE) fr = R(0);
+    f_R = fr * min_term;
+  }
+  R Mb_L, Mb_R;
+  if (M_L_plus + M_R_minus < R(0)) {
+    Mb_L = M_L_plus * w * (R(1) + f_L);
+    Mb_R = M_R_minus + M_L_plus * ((R(1) - w) * (R(1) + f_L) - f_R);
+  } else {
+    Mb_L = M_L_plus + M_R_minus * ((R(1) - w) * (R(1) + f_R) - f_L);
+    Mb_R = M_R_minus * w * (R(1) + f_R);
+  }
+  R a = R(1) - sq(fmin2(R(1), fmax2(absd(M_L), absd(M_R))));
+  R rho_L_half = m_phi_L_half(rho_L, rho_R, sb.rL, a);
+  R rho_R_half = m_phi_R_half(rho_L, rho_R, sb.rR, a);
+  R u_L_half = m_phi_L_half(u_LHS, u_RHS, sb.uL, a);
+  R u_R_half = m_phi_R_half(u_LHS, u_RHS, sb.uR, a);
+  R v_L_half = m_phi_L_half(v_LHS, v_RHS, sb.vL, a);
+  R v_R_half = m_phi_R_half(v_LHS, v_RHS, sb.vR, a);
+  R p_L_half = m_phi_L_half(p_L, p_R, sb.pL, a);
+  R p_R_half = m_phi_R_half(p_L, p_R, sb.pR, a);
+  R H_L_half = (gamma / (gamma - R(1))) * (p_L_half / rho_L_half) + R(0.5) * (sq(u_L_half) + sq(v_L_half));
+  R H_R_half = (gamma / (gamma - R(1))) * (p_R_half / rho_R_half) + R(0.5) * (sq(u_R_half) + sq(v_R_half));
+  R mfL = Mb_L * c_half * rho_L_half;
+  R mfR = Mb_R * c_half * rho_R_half;
+  if (absd(mfL + mfR) < FEPS) {
+    mfL = R(0);
+    mfR = R(0);
+  }
+  out[0] = mfL + mfR;
+  R pu = (P_L_plus * n_x * p_L) + (P_R_minus * n_x * p_R);
+  R mu = (mfL * u_L_half) + (mfR * u_R_half);
+  out[1] = (absd(mu) < FEPS) ? pu : mu + pu;
+  R pv = (P_L_plus * n_y * p_L) + (P_R_minus * n_y * p_R);
+  R mv = (mfL * v_L_half) + (mfR * v_R_half);
+  out[2] = (absd(mv) < FEPS) ? pv : mv + pv;
+  R e = (mfL * H_L_half) + (mfR * H_R_half);
+  out[3] = (absd(e) < FEPS) ? R(0) : e;
+}
+
+template <int FLUX> constexpr bool needs_superbee_states() { return FLUX == FLUX_M_AUSMPW_PLUS; }
+
 template <typename R, int FLUX, int DIR>
-__device__ inline void edge_flux(R gamma, int low_mach_fix, const EdgeState<R> &s, const PressNbr<R> &pn, R out[4]) {
+__device__ inline void edge_flux(R gamma, int low_mach_fix, const EdgeState<R> &s, const PressNbr<R> &pn, R out[4],
+                                 const EdgeState<R> *sb = nullptr) {
   if constexpr (FLUX == FLUX_AUSMPW_PLUS) flux_ausmpw<R, DIR>(gamma, low_mach_fix, s, pn, out);
+  else if constexpr (FLUX == FLUX_M_AUSMPW_PLUS) flux_m_ausmpw<R, DIR>(gamma, low_mach_fix, s, *sb, pn, out);
   else if constexpr (FLUX == FLUX_SLAU || FLUX == FLUX_SLAU2 || FLUX == FLUX_SD_SLAU || FLUX == FLUX_SD_SLAU2)
     flux_slau<R, FLUX, DIR>(gamma, s, pn, out);
   else if constexpr (FLUX == FLUX_ROE) flux_roe<R, DIR>(gamma, s, pn, out);
@@ -556,7 +689,8 @@ __device__ inline void edge_flux(R gamma, int low_mach_fix, const EdgeState<R> &
 }
 
 template <int FLUX> constexpr bool needs_pressure_nbrs() {
-  return FLUX == FLUX_AUSMPW_PLUS || FLUX == FLUX_SD_SLAU || FLUX == FLUX_SD_SLAU2 || FLUX == FLUX_ROE;
+  return FLUX == FLUX_AUSMPW_PLUS || FLUX == FLUX_SD_SLAU || FLUX == FLUX_SD_SLAU2 || FLUX == FLUX_ROE ||
+         FLUX == FLUX_M_AUSMPW_PLUS;
 }
 
 // qa = a / m and qb = b / m for m = max(|a|, |b|, 1) != 1.  A finite m is |a|

@@ -844,10 +844,11 @@ int cato_edge_fluxes(cato_ctx *c, const double *edges, const double *p, int ld, 
   if (ld < nx) return fail(CATO_ERR_INVALID, "ld < ni + 2*nh");
   size_t n = (size_t)ld * ny, ne = (size_t)2 * (s.ni + 1) * (s.nj + 1);
   size_t nfi = (size_t)4 * (s.ni + 1) * s.nj, nfj = (size_t)4 * s.ni * (s.nj + 1);
+  const size_t nedge = (s.cfg.flux == FLUX_M_AUSMPW_PLUS ? 16 : 8) * ne;  // + superbee states
   DevBuf d;
-  HIPCK(hipMalloc(&d.p, (8 * ne + n + nfi + nfj) * 8));
-  double *E = (double *)d.p, *P = E + 8 * ne, *FI = P + n, *FJ = FI + nfi;
-  HIPCK(hipMemcpyAsync(E, edges, 8 * ne * 8, hipMemcpyHostToDevice, c->stream));
+  HIPCK(hipMalloc(&d.p, (nedge + n + nfi + nfj) * 8));
+  double *E = (double *)d.p, *P = E + nedge, *FI = P + n, *FJ = FI + nfi;
+  HIPCK(hipMemcpyAsync(E, edges, nedge * 8, hipMemcpyHostToDevice, c->stream));
   HIPCK(hipMemcpyAsync(P, p, n * 8, hipMemcpyHostToDevice, c->stream));
   FieldGeom g = s.fgeom();
   int nt = (s.ni + 1) * s.nj + s.ni * (s.nj + 1);
@@ -861,6 +862,7 @@ int cato_edge_fluxes(cato_ctx *c, const double *edges, const double *p, int ld, 
     case FLUX_SD_SLAU2: launch_flux<FLUX_SD_SLAU2>(grid, c->stream, pr, g, ld, E, P, lm, FI, FJ); break;
     case FLUX_ROE: launch_flux<FLUX_ROE>(grid, c->stream, pr, g, ld, E, P, lm, FI, FJ); break;
     case FLUX_AUSM_PLUS: launch_flux<FLUX_AUSM_PLUS>(grid, c->stream, pr, g, ld, E, P, lm, FI, FJ); break;
+    case FLUX_M_AUSMPW_PLUS: launch_flux<FLUX_M_AUSMPW_PLUS>(grid, c->stream, pr, g, ld, E, P, lm, FI, FJ); break;
     default: launch_flux<FLUX_HLLC>(grid, c->stream, pr, g, ld, E, P, lm, FI, FJ); break;
   }
   HIPCK(hipGetLastError());

@@ -95,7 +95,8 @@ struct cato_server {
   hipStream_t stream = nullptr;
   char *arena_mem = nullptr;
   char *table_mem = nullptr;
-  size_t table_bytes = 0, loops_off = 0, args_off = 0, variant_off = 0, refs_off = 0, items_off = 0, count_off = 0;
+  size_t table_bytes = 0, loops_off = 0, args_off = 0, variant_off = 0, refs_off = 0, items_off = 0, count_off = 0,
+         cursor_off = 0;
   std::vector<BatchItem> items;  // host copy of the work list (kept until the next tick)
   int item_count = 0;
   int max_items = 0, resident_items = 0;
@@ -130,8 +131,13 @@ int server_tables(cato_server *S) {
   off += a((size_t)S->max_items * sizeof(BatchItem));
   S->count_off = off;
   off += 256;
+  S->cursor_off = off;
+  off += a((size_t)S->nstages * kNumXcd * sizeof(int));
   S->table_bytes = off;
-  return hip_check(hipMalloc(&S->table_mem, S->table_bytes), "server tables");
+  int rc = hip_check(hipMalloc(&S->table_mem, S->table_bytes), "server tables");
+  if (rc) return rc;
+  return hip_check(hipMemset(S->table_mem + S->cursor_off, 0, (size_t)S->nstages * kNumXcd * sizeof(int)),
+                   "server cursors");
 }
 
 // Rebuild the work list and per-slot stage arguments of the resident set.
@@ -162,9 +168,16 @@ int server_upload(cato_server *S, int &nitems, std::vector<int> &active_ids) {
     tab[(size_t)S->nstages * n + k] = first[0];
     variant[k] = sv.cfg.flux * LIM_COUNT + sv.cfg.limiter;
     int nt = sv.tiles_x * sv.tiles_y;
-    for (int t = 0; t < nt; ++t) items.push_back({k, t});
     refs.push_back(BatchSolveRef{sv.block_min, sv.block_err, sv.loop, nt, 0});
     active_ids.push_back(k);
+  }
+  // work list grouped by kernel variant (instruction-cache locality per XCD slice)
+  std::vector<int> order(active_ids);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return variant[x] < variant[y]; });
+  for (int k : order) {
+    const Solve &sv = *S->slots[k].s;
+    int nt = sv.tiles_x * sv.tiles_y;
+    for (int t = 0; t < nt; ++t) items.push_back({k, t});
   }
   nitems = (int)items.size();
   S->item_count = nitems;
@@ -417,6 +430,10 @@ int cato_server_tick(cato_server *S, int *n_resident, int *n_queued, int *n_comp
   }
   if (resident > 0) {
     const int n = S->max_slots;
+    // work cursors start at zero (the finalize kernel resets them after every step)
+    if ((rc = hip_check(hipMemsetAsync(S->table_mem + S->cursor_off, 0, (size_t)S->nstages * kNumXcd * sizeof(int),
+                                       S->stream), "cursors")))
+      return rc;
     const StageArgs *args = reinterpret_cast<const StageArgs *>(S->table_mem + S->args_off);
     // slot `nstages` of the table holds stage 0 as the first step of this tick
     // runs it: fresh solves read their uploaded primitives, the others their state
@@ -431,10 +448,12 @@ int cato_server_tick(cato_server *S, int *n_resident, int *n_queued, int *n_comp
         b.variant = reinterpret_cast<const int *>(S->table_mem + S->variant_off);
         b.items = reinterpret_cast<const BatchItem *>(S->table_mem + S->items_off);
         b.count = reinterpret_cast<const int *>(S->table_mem + S->count_off);
+        b.cursor = reinterpret_cast<int *>(S->table_mem + S->cursor_off) + (size_t)s * kNumXcd;
         S->launcher(dim3((unsigned)grid), S->stream, b);
       }
       hipLaunchKernelGGL(batch_finalize_kernel, dim3((unsigned)resident), dim3(kThreads), 0, S->stream,
-                         reinterpret_cast<const BatchSolveRef *>(S->table_mem + S->refs_off), 2);
+                         reinterpret_cast<const BatchSolveRef *>(S->table_mem + S->refs_off), 2,
+                         reinterpret_cast<int *>(S->table_mem + S->cursor_off), S->nstages * kNumXcd);
       if ((rc = hip_check(hipGetLastError(), "server step"))) return rc;
     }
     for (int k : ids) {

@@ -25,6 +25,7 @@ StageLauncher stage_launcher_f4(int lim, int prec, int tile);
 StageLauncher stage_launcher_f5(int lim, int prec, int tile);
 StageLauncher stage_launcher_f6(int lim, int prec, int tile);
 StageLauncher stage_launcher_f7(int lim, int prec, int tile);
+StageLauncher stage_launcher_f8(int lim, int prec, int tile);
 
 inline StageLauncher stage_launcher(int flux, int lim, int prec, int tile) {
   switch (flux) {
@@ -36,6 +37,7 @@ inline StageLauncher stage_launcher(int flux, int lim, int prec, int tile) {
     case 5: return stage_launcher_f5(lim, prec, tile);
     case 6: return stage_launcher_f6(lim, prec, tile);
     case 7: return stage_launcher_f7(lim, prec, tile);
+    case 8: return stage_launcher_f8(lim, prec, tile);
     default: return nullptr;
   }
 }

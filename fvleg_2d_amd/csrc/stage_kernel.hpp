@@ -128,6 +128,9 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
   constexpr int H = halo_of<LIMC>();
   constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
   constexpr int NIE = (TX + 1) * TY, NJE = TX * (TY + 1);
+  // FP64 form: the reference's arithmetic bit for bit; precision form (f32x):
+  // a lighter FP64 epilogue (reciprocal-based primitive recovery and flux balance)
+  constexpr bool kExact = !std::is_same<R, f32x>::value;
   auto &sp = sm.sp;
   auto &sfi = sm.sfi;
   auto &sfj = sm.sfj;
@@ -181,6 +184,13 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
     recon_var<R, LIMC>(&sp[1][ly][lx], sn, st, s.uL, s.uR);
     recon_var<R, LIMC>(&sp[2][ly][lx], sn, st, s.vL, s.vR);
     recon_var<R, LIMC>(&sp[3][ly][lx], sn, st, s.pL, s.pR);
+    EdgeState<R> sb;  // superbee reconstruction of the same edge (M-AUSMPW+ only)
+    if constexpr (needs_superbee_states<FLUX>()) {
+      recon_var<R, LIM_SUPERBEE>(&sp[0][ly][lx], sn, st, sb.rL, sb.rR);
+      recon_var<R, LIM_SUPERBEE>(&sp[1][ly][lx], sn, st, sb.uL, sb.uR);
+      recon_var<R, LIM_SUPERBEE>(&sp[2][ly][lx], sn, st, sb.vL, sb.vR);
+      recon_var<R, LIM_SUPERBEE>(&sp[3][ly][lx], sn, st, sb.pL, sb.pR);
+    }
     PressNbr<R> pn;
     if constexpr (needs_pressure_nbrs<FLUX>()) {
       const R *pc = &sp[3][ly][lx];
@@ -195,11 +205,11 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
     }
     R f[4];
     if (idir) {
-      edge_flux<R, FLUX, 0>(gamma, a.low_mach_fix, s, pn, f);
+      edge_flux<R, FLUX, 0>(gamma, a.low_mach_fix, s, pn, f, &sb);
 #pragma unroll
       for (int v = 0; v < 4; ++v) sfi[v][ey][ex] = f[v];
     } else {
-      edge_flux<R, FLUX, 1>(gamma, a.low_mach_fix, s, pn, f);
+      edge_flux<R, FLUX, 1>(gamma, a.low_mach_fix, s, pn, f, &sb);
 #pragma unroll
       for (int v = 0; v < 4; ++v) sfj[v][ey][ex] = f[v];
     }
@@ -222,14 +232,21 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
     double vol = dxv * dyv;
     uint64_t vb = (uint64_t)__double_as_longlong(vol);
     bool pow2 = (vb & 0x000FFFFFFFFFFFFFull) == 0 && ((vb >> 52) & 0x7FF) > 1 && ((vb >> 52) & 0x7FF) < 2046;
-    double inv_vol = pow2 ? 1.0 / vol : 0.0;
+    double inv_vol = kExact ? (pow2 ? 1.0 / vol : 0.0) : rcp64(vol);
     size_t off = (size_t)(gj - 1) * a.pitch + (size_t)(gi - 1);
     double nv[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       double Fi = (double)sfi[v][cy][cx + 1], Fim = (double)sfi[v][cy][cx];
       double Gj = (double)sfj[v][cy + 1][cx], Gjm = (double)sfj[v][cy][cx];
-      double L = divergence(Fi, Fim, Gj, Gjm, lb, lr, lt, ll, vol, pow2, inv_vol);
+      double L;
+      if constexpr (kExact) {
+        L = divergence(Fi, Fim, Gj, Gjm, lb, lr, lt, ll, vol, pow2, inv_vol);
+      } else {
+        // precision form: the same FP64 flux balance without the reference's
+        // max-normalisation and flushes (they only shape rounding at 1e-16)
+        L = -(((Fi * lr) - (Fim * ll)) + ((Gj * lt) - (Gjm * lb))) * inv_vol;
+      }
       double r;
       if (a.mode == MODE_RESIDUAL) {
         r = L;
@@ -248,7 +265,8 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
     // primitives of the new state for the next stage (assign_fluid ->
     // calculate_derived_quantities, mod_fluid.f90:564-591) and its NaN check
     double cs;
-    Prim<double> q = cons_to_prim(nv[0], nv[1], nv[2], nv[3], a.gamma, a.gamma - 1.0, &cs);
+    Prim<double> q = kExact ? cons_to_prim(nv[0], nv[1], nv[2], nv[3], a.gamma, a.gamma - 1.0, &cs)
+                            : cons_to_prim_fast(nv[0], nv[1], nv[2], nv[3], a.gamma, a.gamma - 1.0, a.last_stage, &cs);
     a.prim_out[0][off] = q.rho;
     a.prim_out[1][off] = q.u;
     a.prim_out[2][off] = q.v;

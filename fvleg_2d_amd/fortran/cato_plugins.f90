@@ -174,9 +174,10 @@ contains
   function config_from_input(input) result(cfg)
     class(cato_input_t), intent(in) :: input
     type(cato_config_t) :: cfg
-    character(len=16), parameter :: fluxes(11) = [character(len=16) :: 'AUSMPW+', 'SLAU', 'SLAU2', 'SD-SLAU', &
-                                                   'SD-SLAU2', 'Roe', 'S-Roe', 'SP-Roe', 'AUSM+', 'AUSM+-u', 'HLLC']
-    integer, parameter :: flux_ids(11) = [0, 1, 2, 3, 4, 5, 5, 5, 6, 6, 7]
+    character(len=16), parameter :: fluxes(12) = [character(len=16) :: 'AUSMPW+', 'SLAU', 'SLAU2', 'SD-SLAU', &
+                                                   'SD-SLAU2', 'Roe', 'S-Roe', 'SP-Roe', 'AUSM+', 'AUSM+-u', 'HLLC', &
+                                                   'M-AUSMPW+']
+    integer, parameter :: flux_ids(12) = [0, 1, 2, 3, 4, 5, 5, 5, 6, 6, 7, 8]
     character(len=16), parameter :: limiters(6) = [character(len=16) :: 'minmod', 'superbee', 'van_leer', &
                                                     'vanleer', 'MLP3', 'MLP5']
     integer, parameter :: limiter_ids(6) = [0, 1, 2, 2, 3, 4]

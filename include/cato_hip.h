@@ -47,7 +47,8 @@ enum {
   CATO_FLUX_SD_SLAU2 = 4,    /* 'SD-SLAU2'                                  */
   CATO_FLUX_ROE = 5,         /* 'Roe', 'S-Roe', 'SP-Roe' (reference variant) */
   CATO_FLUX_AUSM_PLUS = 6,   /* 'AUSM+' / 'AUSM+-u'                         */
-  CATO_FLUX_HLLC = 7         /* 'HLLC' (Toro, PVRS wave speeds)             */
+  CATO_FLUX_HLLC = 7,        /* 'HLLC' (Toro, PVRS wave speeds)             */
+  CATO_FLUX_M_AUSMPW_PLUS = 8 /* 'M-AUSMPW+' (m_ausmpw_plus_solver.fypp)   */
 };
 /* limiter names (muscl_factory.f90:69-90) */
 enum {
@@ -153,7 +154,8 @@ int cato_apply_bc(cato_ctx *ctx, double *rho, double *u, double *v, double *p, i
  * (2, 0:ni, 0:nj) arrays of (L, R) states. */
 int cato_interpolate_edges(cato_ctx *ctx, const double *q, int ld, double *i_edges, double *j_edges);
 /* get_<dir>flux: edges = 8 consecutive (2, 0:ni, 0:nj) arrays (rho_i, rho_j, u_i, u_j,
- * v_i, v_j, p_i, p_j); p = halo'd cell pressure; iflux (4, 0:ni, 1:nj), jflux (4, 1:ni, 0:nj). */
+ * v_i, v_j, p_i, p_j), for M-AUSMPW+ followed by the same 8 of the superbee reconstruction;
+ * p = halo'd cell pressure; iflux (4, 0:ni, 1:nj), jflux (4, 1:ni, 0:nj). */
 int cato_edge_fluxes(cato_ctx *ctx, const double *edges, const double *p, int ld, double *iflux, double *jflux);
 /* flux_solver_t%solve: BCs are applied to rho, u, v, p in place (intent(inout)),
  * d_* receive d(rho, rho u, rho v, rho E)/dt with zeroed halos. */

@@ -549,6 +549,111 @@ static void flux_ausmpw(double gamma, int low_mach_fix, int dir, const edge_stat
   out[3] = (mfL * H_L) + (mfR * H_R);
 }
 
+/* M-AUSMPW+: m_ausmpw_plus_solver.fypp:437-738 (the code path the reference runs: AUSMPW+
+ * pressure splitting without the alpha term, w from the normal pressure ratio), get_f 829-862 with
+ * its 1e-10 flushes, phi_L_half / phi_R_half 790-827 blending the limiter-of-choice states with the
+ * superbee states, and the 1e3*epsilon flushes of the assembled flux 715-735. */
+static double m_phi_L_half(double phi_L, double phi_R, double phi_L_sb, double a) {
+  if (fabs(phi_R - phi_L) < 1.0 || fabs(phi_L_sb - phi_L) < EPS) return phi_L;
+  return phi_L + fmax2(0.0, (phi_R - phi_L) * (phi_L_sb - phi_L)) / ((phi_R - phi_L) * fabs(phi_L_sb - phi_L)) *
+                     fmin3(a, 0.5 * fabs(phi_R - phi_L), fabs(phi_L_sb - phi_L));
+}
+
+static double m_phi_R_half(double phi_L, double phi_R, double phi_R_sb, double a) {
+  if (fabs(phi_L - phi_R) < EPS || fabs(phi_R_sb - phi_R) < 1.0) return phi_R;
+  return phi_R + fmax2(0.0, (phi_L - phi_R) * (phi_R_sb - phi_R)) / ((phi_L - phi_R) * fabs(phi_R_sb - phi_R)) *
+                     fmin3(a, 0.5 * fabs(phi_L - phi_R), fabs(phi_R_sb - phi_R));
+}
+
+static void flux_m_ausmpw(double gamma, int low_mach_fix, int dir, const edge_state *s, const edge_state *sb,
+                          double n_x, double n_y, const double *p4 /* p1_L,p1_R,p2_L,p2_R */, double out[4]) {
+  const double LOW_MACH = 0.1;
+  const double FEPS = EPS * 1e3;
+  double rho_L = s->rL, u_LHS = s->uL, v_LHS = s->vL, p_L = s->pL;
+  double rho_R = s->rR, u_RHS = s->uR, v_RHS = s->vR, p_R = s->pR;
+  double U_L, U_R, V_L, V_R;
+  if (dir == 0) { U_L = u_LHS; U_R = u_RHS; V_L = v_LHS; V_R = v_RHS; }
+  else          { U_L = v_LHS; U_R = v_RHS; V_L = u_LHS; V_R = u_RHS; }
+  double cs_L = sqrt(gamma * p_L / rho_L);
+  double cs_R = sqrt(gamma * p_R / rho_R);
+  double M_LHS = sqrt(sq(u_LHS) + sq(v_LHS)) / cs_L;
+  double M_RHS = sqrt(sq(u_RHS) + sq(v_RHS)) / cs_R;
+  if (low_mach_fix && (fabs(M_LHS) < LOW_MACH || fabs(M_RHS) < LOW_MACH)) {
+    double f_M = fmin2(1.0, fmax2(M_LHS, M_RHS));
+    double ulf = 0.5 * (u_LHS + u_RHS) + f_M * 0.5 * (u_LHS - u_RHS);
+    double urf = 0.5 * (u_LHS + u_RHS) - f_M * 0.5 * (u_LHS - u_RHS);
+    double vlf = 0.5 * (v_LHS + v_RHS) + f_M * 0.5 * (v_LHS - v_RHS);
+    double vrf = 0.5 * (v_LHS + v_RHS) - f_M * 0.5 * (v_LHS - v_RHS);
+    u_LHS = ulf; u_RHS = urf; v_LHS = vlf; v_RHS = vrf;
+  }
+  double H_L = (gamma / (gamma - 1.0)) * (p_L / rho_L) + 0.5 * (sq(u_LHS) + sq(v_LHS));
+  double H_R = (gamma / (gamma - 1.0)) * (p_R / rho_R) + 0.5 * (sq(u_RHS) + sq(v_RHS));
+  double H_normal = fmin2(H_L - 0.5 * sq(V_L), H_R - 0.5 * sq(V_R));
+  double c_s = sqrt(2.0 * ((gamma - 1.0) / (gamma + 1.0)) * H_normal);
+  double c_half;
+  if (0.5 * (U_L + U_R) < 0.0) c_half = sq(c_s) / fmax2(fabs(U_R), c_s);
+  else c_half = sq(c_s) / fmax2(fabs(U_L), c_s);
+  double M_L = U_L / c_half;
+  double M_R = U_R / c_half;
+  double M_L_plus = (fabs(M_L) > 1.0) ? 0.5 * (M_L + fabs(M_L)) : 0.25 * sq(M_L + 1.0);
+  double M_R_minus = (fabs(M_R) > 1.0) ? 0.5 * (M_R - fabs(M_R)) : -(0.25 * sq(M_R - 1.0));
+  double P_R_minus = (fabs(M_R) > 1.0) ? 0.5 * (1.0 - copysign(1.0, M_R)) : 0.25 * sq(M_R - 1.0) * (2.0 + M_R);
+  double P_L_plus = (fabs(M_L) > 1.0) ? 0.5 * (1.0 + copysign(1.0, M_L)) : 0.25 * sq(M_L + 1.0) * (2.0 - M_L);
+  double p_s = p_L * P_L_plus + p_R * P_R_minus;
+  double mt = fmin2((p_L / p_R), (p_R / p_L));
+  double w = 1.0 - (mt * mt * mt);
+  double f_L, f_R;
+  if (fabs(p_s) < TINY) {
+    f_L = 0.0;
+    f_R = 0.0;
+  } else {
+    const double FE = 1e-10;
+    double mn4 = fmin2(fmin2(fmin2(p4[0], p4[1]), p4[2]), p4[3]);
+    double min_term = sq(fmin2(1.0, mn4 / fmin2(p_L, p_R)));
+    if (fabs(1.0 - min_term) < FE) min_term = 1.0;
+    double fl = (p_L / p_s) - 1.0;
+    if (fabs(fl) < FE) fl = 0.0;
+    f_L = fl * min_term;
+    double fr = (p_R / p_s) - 1.0;
+    if (fabs(fr) < FE) fr = 0.0;
+    f_R = fr * min_term;
+  }
+  double Mb_L, Mb_R;
+  if (M_L_plus + M_R_minus < 0.0) {
+    Mb_L = M_L_plus * w * (1.0 + f_L);
+    Mb_R = M_R_minus + M_L_plus * ((1.0 - w) * (1.0 + f_L) - f_R);
+  } else {
+    Mb_L = M_L_plus + M_R_minus * ((1.0 - w) * (1.0 + f_R) - f_L);
+    Mb_R = M_R_minus * w * (1.0 + f_R);
+  }
+  double a = 1.0 - sq(fmin2(1.0, fmax2(fabs(M_L), fabs(M_R))));
+  double rho_L_half = m_phi_L_half(rho_L, rho_R, sb->rL, a);
+  double rho_R_half = m_phi_R_half(rho_L, rho_R, sb->rR, a);
+  double u_L_half = m_phi_L_half(u_LHS, u_RHS, sb->uL, a);
+  double u_R_half = m_phi_R_half(u_LHS, u_RHS, sb->uR, a);
+  double v_L_half = m_phi_L_half(v_LHS, v_RHS, sb->vL, a);
+  double v_R_half = m_phi_R_half(v_LHS, v_RHS, sb->vR, a);
+  double p_L_half = m_phi_L_half(p_L, p_R, sb->pL, a);
+  double p_R_half = m_phi_R_half(p_L, p_R, sb->pR, a);
+  double H_L_half = (gamma / (gamma - 1.0)) * (p_L_half / rho_L_half) + 0.5 * (sq(u_L_half) + sq(v_L_half));
+  double H_R_half = (gamma / (gamma - 1.0)) * (p_R_half / rho_R_half) + 0.5 * (sq(u_R_half) + sq(v_R_half));
+  double mfL = Mb_L * c_half * rho_L_half;
+  double mfR = Mb_R * c_half * rho_R_half;
+  if (fabs(mfL + mfR) < FEPS) {
+    mfL = 0.0;
+    mfR = 0.0;
+  }
+  out[0] = mfL + mfR;
+  double pu = (P_L_plus * n_x * p_L) + (P_R_minus * n_x * p_R);
+  double mu = (mfL * u_L_half) + (mfR * u_R_half);
+  out[1] = (fabs(mu) < FEPS) ? pu : mu + pu;
+  double pv = (P_L_plus * n_y * p_L) + (P_R_minus * n_y * p_R);
+  double mv = (mfL * v_L_half) + (mfR * v_R_half);
+  out[2] = (fabs(mv) < FEPS) ? pv : mv + pv;
+  out[3] = (mfL * H_L_half) + (mfR * H_R_half);
+  if (fabs(out[3]) < FEPS) out[3] = 0.0;
+}
+
 /* Roe (reference variant): roe_solver.fypp:420-511, f_h 519-527 */
 static double roe_f_h(double pL, double pR) {
   double ratio = fabs(pR - pL) / (fmin2(pR, pL) + 1e-16);
@@ -719,6 +824,13 @@ static void flux_hllc(double gamma, const edge_state *s, double n_x, double n_y,
 
 void or_edge_fluxes(const or_config *c, const or_geometry *g, const double *rho, const double *u, const double *v,
                     const double *p, const double *edges, double *iflux, double *jflux) {
+  const size_t ne = (size_t)2 * (c->ni + 1) * (c->nj + 1);
+  or_edge_fluxes_sb(c, g, rho, u, v, p, edges, c->flux == OR_FLUX_M_AUSMPW_PLUS ? edges + 8 * ne : NULL, iflux,
+                    jflux);
+}
+
+void or_edge_fluxes_sb(const or_config *c, const or_geometry *g, const double *rho, const double *u, const double *v,
+                       const double *p, const double *edges, const double *edges_sb, double *iflux, double *jflux) {
   const int ni = c->ni, nj = c->nj, nh = c->nh;
   const size_t ne = (size_t)2 * (ni + 1) * (nj + 1);
   const double *rho_i = edges + 0 * ne, *rho_j = edges + 1 * ne;
@@ -738,6 +850,15 @@ void or_edge_fluxes(const or_config *c, const or_geometry *g, const double *rho,
       for (int i = i0; i <= ni; ++i) {
         edge_state s = {er[EI(0, i, j)], eu[EI(0, i, j)], evv[EI(0, i, j)], ep[EI(0, i, j)],
                         er[EI(1, i, j)], eu[EI(1, i, j)], evv[EI(1, i, j)], ep[EI(1, i, j)]};
+        edge_state sb = s;
+        if (edges_sb) {
+          const double *b = edges_sb;
+          const size_t o0 = (size_t)(dir == 0 ? 0 : 1) * ne;  /* rho_i | rho_j, then u, v, p pairs */
+          sb.rL = b[o0 + 0 * 2 * ne + EI(0, i, j)]; sb.rR = b[o0 + 0 * 2 * ne + EI(1, i, j)];
+          sb.uL = b[o0 + 1 * 2 * ne + EI(0, i, j)]; sb.uR = b[o0 + 1 * 2 * ne + EI(1, i, j)];
+          sb.vL = b[o0 + 2 * 2 * ne + EI(0, i, j)]; sb.vR = b[o0 + 2 * 2 * ne + EI(1, i, j)];
+          sb.pL = b[o0 + 3 * 2 * ne + EI(0, i, j)]; sb.pR = b[o0 + 3 * 2 * ne + EI(1, i, j)];
+        }
         int e = dir == 0 ? 1 : 2; /* RIGHT_IDX / TOP_IDX */
         size_t cc = CI(i, j);
         double nx = g->edge_norm[8 * cc + 2 * e + 0];
@@ -790,6 +911,13 @@ void or_edge_fluxes(const or_config *c, const or_geometry *g, const double *rho,
               pf[8] = P(i, j + 1); pf[9] = P(i + 1, j + 1);
             }
             flux_roe(c->gamma, &s, nx, ny, pf, out);
+            break;
+          }
+          case OR_FLUX_M_AUSMPW_PLUS: {
+            double p4[4];
+            if (dir == 0) { p4[0] = P(i, j - 1); p4[1] = P(i + 1, j - 1); p4[2] = P(i, j + 1); p4[3] = P(i + 1, j + 1); }
+            else          { p4[0] = P(i + 1, j); p4[1] = P(i + 1, j + 1); p4[2] = P(i - 1, j); p4[3] = P(i - 1, j + 1); }
+            flux_m_ausmpw(c->gamma, c->apply_low_mach_fix, dir, &s, &sb, nx, ny, p4, out);
             break;
           }
           case OR_FLUX_AUSM_PLUS: flux_ausm_plus(c->gamma, &s, nx, ny, out); break;
@@ -852,7 +980,19 @@ void or_solve(const or_config *c, const or_geometry *g, double *rho, double *u, 
   or_interpolate_edges(c, p, edges + 6 * ne, edges + 7 * ne);
   double *iflux = (double *)calloc((size_t)4 * (ni + 1) * nj, sizeof(double));
   double *jflux = (double *)calloc((size_t)4 * ni * (nj + 1), sizeof(double));
-  or_edge_fluxes(c, g, rho, u, v, p, edges, iflux, jflux);
+  double *edges_sb = NULL;
+  if (c->flux == OR_FLUX_M_AUSMPW_PLUS) {
+    /* second reconstruction with the superbee limiter (m_ausmpw_plus_solver.fypp:206-211) */
+    or_config csb = *c;
+    csb.limiter = OR_LIM_SUPERBEE;
+    edges_sb = (double *)calloc(8 * ne, sizeof(double));
+    or_interpolate_edges(&csb, rho, edges_sb + 0 * ne, edges_sb + 1 * ne);
+    or_interpolate_edges(&csb, u, edges_sb + 2 * ne, edges_sb + 3 * ne);
+    or_interpolate_edges(&csb, v, edges_sb + 4 * ne, edges_sb + 5 * ne);
+    or_interpolate_edges(&csb, p, edges_sb + 6 * ne, edges_sb + 7 * ne);
+  }
+  or_edge_fluxes_sb(c, g, rho, u, v, p, edges, edges_sb, iflux, jflux);
+  free(edges_sb);
   or_flux_split_edges(c, g, iflux, jflux, d_rho, d_rho_u, d_rho_v, d_rho_E);
   free(edges);
   free(iflux);

@@ -27,7 +27,8 @@ enum {
   OR_FLUX_SD_SLAU2 = 4,    /* 'SD-SLAU2'                                        */
   OR_FLUX_ROE = 5,         /* 'Roe' / 'S-Roe' / 'SP-Roe' roe_solver.fypp        */
   OR_FLUX_AUSM_PLUS = 6,   /* 'AUSM+' (= AUSM+-u flags)  ausm_plus_solver.f90    */
-  OR_FLUX_HLLC = 7         /* 'HLLC' (textbook Toro; reference body incomplete) */
+  OR_FLUX_HLLC = 7,        /* 'HLLC' (textbook Toro; reference body incomplete) */
+  OR_FLUX_M_AUSMPW_PLUS = 8 /* 'M-AUSMPW+'                m_ausmpw_plus_solver.fypp */
 };
 
 /* Reconstruction ids (muscl_factory.f90:69-90). */
@@ -91,6 +92,11 @@ void or_interpolate_edges(const or_config *c, const double *q, double *i_edges, 
 void or_edge_fluxes(const or_config *c, const or_geometry *g, const double *rho, const double *u, const double *v,
                     const double *p, const double *edges /* 8 arrays: rho_i,rho_j,u_i,u_j,v_i,v_j,p_i,p_j */,
                     double *iflux, double *jflux);
+
+/* The same with the superbee edge states M-AUSMPW+ blends in (8 more arrays in the same order;
+ * NULL for the other fluxes).  or_edge_fluxes takes them from edges + 8 arrays for M-AUSMPW+. */
+void or_edge_fluxes_sb(const or_config *c, const or_geometry *g, const double *rho, const double *u, const double *v,
+                       const double *p, const double *edges, const double *edges_sb, double *iflux, double *jflux);
 
 /* flux_solver.f90:187-459. d_dt arrays are halo'd; halo zeroed. */
 void or_flux_split_edges(const or_config *c, const or_geometry *g, const double *iflux, const double *jflux,

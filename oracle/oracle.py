@@ -24,7 +24,7 @@ LIB_PATH = os.path.join(HERE, "liboracle_omp.so" if _OMP else "liboracle.so")
 
 FLUX_IDS = {
     "AUSMPW+": 0, "SLAU": 1, "SLAU2": 2, "SD-SLAU": 3, "SD-SLAU2": 4,
-    "Roe": 5, "S-Roe": 5, "SP-Roe": 5, "AUSM+": 6, "AUSM+-u": 6, "HLLC": 7,
+    "Roe": 5, "S-Roe": 5, "SP-Roe": 5, "AUSM+": 6, "AUSM+-u": 6, "HLLC": 7, "M-AUSMPW+": 8,
 }
 LIMITER_IDS = {"minmod": 0, "superbee": 1, "van_leer": 2, "vanleer": 2, "MLP3": 3, "MLP5": 4}
 BC_IDS = {"periodic": 0, "zero_gradient": 1, "symmetry": 2, "mirror": 2, "reflection": 2}
@@ -183,7 +183,8 @@ def interpolate_edges(cfg: Config, q):
 
 
 def edge_fluxes(cfg: Config, geom: Geometry, rho, u, v, p, edges):
-    """edges: list of 8 arrays (rho_i, rho_j, u_i, u_j, v_i, v_j, p_i, p_j), each (2, ni+1, nj+1)."""
+    """edges: list of 8 arrays (rho_i, rho_j, u_i, u_j, v_i, v_j, p_i, p_j), each (2, ni+1, nj+1);
+    for M-AUSMPW+ 16: the same 8 followed by the 8 superbee reconstructions."""
     ni, nj = cfg.ni, cfg.nj
     packed = np.concatenate([np.asfortranarray(e).ravel(order="F") for e in edges])
     iflux = np.zeros((4, ni + 1, nj), order="F")

@@ -54,11 +54,11 @@ def test_host_rejects_unsupported_scheme(tmp_path):
 
     pb = sod_problem(ni=40, nj=4)
     ini = write_deck(pb, str(tmp_path), "sod")
-    text = open(ini).read().replace("flux_solver = 'AUSM+'", "flux_solver = 'M-AUSMPW+'")
+    text = open(ini).read().replace("flux_solver = 'AUSM+'", "flux_solver = 'FVLEG'")
     open(ini, "w").write(text)
     r = subprocess.run([host_or_fail(), ini, "--dry-run"], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0
-    assert "unsupported flux_solver 'M-AUSMPW+'" in r.stdout
+    assert "unsupported flux_solver 'FVLEG'" in r.stdout
 
 
 @pytest.mark.gpu

@@ -32,7 +32,7 @@ def solve_alone(pb, max_time):
 
 def mixed_batch():
     pbs = batch_riemann(n_solves=10, seed=77, side_range=(20, 70), max_time=0.05)
-    fl = ["AUSMPW+", "SLAU", "SLAU2", "SD-SLAU", "SD-SLAU2", "Roe", "AUSM+", "HLLC", "SLAU", "Roe"]
+    fl = ["AUSMPW+", "SLAU", "SLAU2", "SD-SLAU", "SD-SLAU2", "Roe", "AUSM+", "HLLC", "M-AUSMPW+", "Roe"]
     lims = ["minmod", "superbee", "van_leer", "MLP3", "MLP5", "minmod", "MLP5", "MLP3", "van_leer", "superbee"]
     out = []
     for k, pb in enumerate(pbs):

@@ -14,7 +14,7 @@ from fvleg_2d_amd.problems import kelvin_helmholtz, riemann2d, sod, uniform_node
 
 pytestmark = pytest.mark.gpu
 
-FLUXES = ["AUSMPW+", "SLAU", "SLAU2", "SD-SLAU", "SD-SLAU2", "Roe", "AUSM+", "HLLC"]
+FLUXES = ["AUSMPW+", "SLAU", "SLAU2", "SD-SLAU", "SD-SLAU2", "Roe", "AUSM+", "HLLC", "M-AUSMPW+"]
 LIMITERS = ["minmod", "superbee", "van_leer", "MLP3", "MLP5"]
 TOL = 1e-12
 
@@ -112,6 +112,10 @@ def test_edge_flux_parity(oracle, flux):
     edges = []
     for q in (rho, u, v, p):
         edges += list(oracle.interpolate_edges(oc, q))
+    if flux == "M-AUSMPW+":  # plus the superbee reconstruction it blends in
+        osb = oracle_cfg(oracle, SchemeConfig(flux=flux, limiter="superbee"), ni, nj)
+        for q in (rho, u, v, p):
+            edges += list(oracle.interpolate_edges(osb, q))
     og = oracle.Geometry(oc, nx, ny)
     oi, oj = oracle.edge_fluxes(oc, og, rho, u, v, p, edges)
     with Solver(cfg, nx, ny) as s:

@@ -18,7 +18,7 @@ from conftest import ROOT
 from exact_riemann import sample
 
 GOLD = os.path.join(ROOT, "tests", "golden")
-FLUXES = ["AUSMPW+", "SLAU", "SLAU2", "SD-SLAU", "SD-SLAU2", "Roe", "AUSM+", "HLLC"]
+FLUXES = ["AUSMPW+", "SLAU", "SLAU2", "SD-SLAU", "SD-SLAU2", "Roe", "AUSM+", "HLLC", "M-AUSMPW+"]
 LIMITERS = ["minmod", "superbee", "van_leer", "MLP3", "MLP5"]
 
 
@@ -143,6 +143,8 @@ def test_flux_consistency(oracle, flux):
         for val in st:
             e = np.full((2, 5, 5), val, order="F")
             edges += [e, e]
+        if flux == "M-AUSMPW+":
+            edges = edges + edges  # superbee states of a uniform field are the same values
         p = cfg.field()
         p[:] = st[3]
         fi, fj = oracle.edge_fluxes(cfg, g, p, p, p, p, edges)
@@ -239,7 +241,7 @@ def test_conservation_on_periodic_grid(oracle, flux):
         assert abs(tot) <= 1e-13 * mag + 1e-300
 
 
-@pytest.mark.parametrize("flux", ["AUSM+", "SLAU", "AUSMPW+", "Roe", "HLLC"])
+@pytest.mark.parametrize("flux", ["AUSM+", "SLAU", "AUSMPW+", "Roe", "HLLC", "M-AUSMPW+"])
 def test_sod_against_exact_riemann(oracle, flux):
     """Accuracy (not parity): density L1 error of Sod at t = 0.2 on 400 cells."""
     ni, nj = 400, 4
@@ -347,3 +349,27 @@ def test_timestep_is_the_cfl_minimum(oracle):
     cs = np.sqrt(1.4 * p[inner] / rho[inner])
     per_cell = 0.37 / ((np.abs(u[inner]) + cs) / dx + (np.abs(v[inner]) + cs) / dy)
     assert abs(f.get_timestep() - per_cell.min()) <= 1e-14 * per_cell.min()
+
+
+def test_m_ausmpw_strong_shock_tube_against_exact_riemann(oracle):
+    """M-AUSMPW+ blends in the superbee states where a jump exceeds 1 (phi_L_half / phi_R_half,
+    m_ausmpw_plus_solver.fypp:790-827): a shock tube with a pressure ratio of 100 exercises that branch;
+    density L1 error against the exact solution at t = 0.012 on 400 cells, and the 1-D invariance."""
+    ni, nj = 400, 4
+    cfg = oracle.Config(ni, nj, flux="M-AUSMPW+", limiter="minmod", integrator="ssp_rk2",
+                        bcs=("zero_gradient", "zero_gradient", "periodic", "periodic"), cfl=0.3)
+    nx, ny = oracle.uniform_nodes(ni, nj, cfg.nh, (0, 1), (0, 0.01))
+    g = oracle.Geometry(cfg, nx, ny)
+    xc = 0.5 * (nx[:-1, :-1] + nx[1:, :-1])
+    rho = np.asfortranarray(np.where(xc < 0.5, 1.0, 0.125))
+    p = np.asfortranarray(np.where(xc < 0.5, 10.0, 0.1))
+    z = cfg.field()
+    f = oracle.Fluid(cfg, g, rho, z, z.copy(order="F"), p)
+    n, t, err = f.run(0.012, contour_interval_dt=0.012)
+    assert err == 0 and n > 50
+    xi = xc[cfg.nh:-cfg.nh, cfg.nh]
+    ex = sample(xi, t, 0.5, 1.0, 0.0, 10.0, 0.125, 0.0, 0.1)
+    r = f.interior("rho")
+    l1 = np.mean(np.abs(r[:, 0] - ex[0]))
+    assert l1 < 0.03, l1
+    assert np.abs(r - r[:, :1]).max() < 1e-8
