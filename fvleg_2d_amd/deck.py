@@ -52,3 +52,16 @@ def read_state(path: str) -> dict:
     out["time"] = float(np.asarray(out["time"]).reshape(-1)[0])
     out["iteration"] = int(np.asarray(out["iteration"]).reshape(-1)[0])
     return out
+
+
+def read_contour(path: str) -> dict:
+    """A contour file of the Fortran host (step_NNNNNNN.h5): scalars and halo'd fields."""
+    out = {}
+    with h5io.H5File(path, "r") as f:
+        for k in ("iteration", "n_ghost_layers"):
+            out[k] = int(f.read("/" + k, dtype=np.int32))
+        for k in ("time", "delta_t"):
+            out[k] = float(f.read("/" + k))
+        for k in ("x", "y", "density", "x_velocity", "y_velocity", "pressure", "sound_speed"):
+            out[k] = f.read("/" + k)
+    return out

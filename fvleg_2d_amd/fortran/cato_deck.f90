@@ -12,7 +12,7 @@ module cato_deck
   private
 
   integer, parameter :: rk = c_double
-  public :: cato_input_t, read_input_ini, read_initial_conditions, write_state
+  public :: cato_input_t, read_input_ini, read_initial_conditions, write_state, write_contour
 
   !> Deck entries of the hot path (names as in input.ini)
   type :: cato_input_t
@@ -49,6 +49,8 @@ module cato_deck
   integer(c_int64_t), parameter :: H5P_DEFAULT = 0, H5S_ALL = 0
   integer(c_int64_t), bind(C, name="H5T_NATIVE_DOUBLE_g") :: h5t_native_double
   integer(c_int64_t), bind(C, name="H5T_NATIVE_INT_g") :: h5t_native_int
+  integer(c_int64_t), bind(C, name="H5T_C_S1_g") :: h5t_c_s1
+  integer(c_int), parameter :: H5S_SCALAR = 0
 
   interface
     function H5open() bind(C, name="H5open") result(r)
@@ -125,6 +127,27 @@ module cato_deck
       integer(c_int64_t), value :: loc, dtype, space, lcpl, dcpl, dapl
       character(kind=c_char), intent(in) :: name(*)
       integer(c_int64_t) :: h
+    end function
+    function H5Screate(kind) bind(C, name="H5Screate") result(h)
+      import :: c_int, c_int64_t
+      integer(c_int), value :: kind
+      integer(c_int64_t) :: h
+    end function
+    function H5Tcopy(t) bind(C, name="H5Tcopy") result(h)
+      import :: c_int64_t
+      integer(c_int64_t), value :: t
+      integer(c_int64_t) :: h
+    end function
+    function H5Tset_size(t, n) bind(C, name="H5Tset_size") result(r)
+      import :: c_int, c_int64_t, c_size_t
+      integer(c_int64_t), value :: t
+      integer(c_size_t), value :: n
+      integer(c_int) :: r
+    end function
+    function H5Tclose(t) bind(C, name="H5Tclose") result(r)
+      import :: c_int, c_int64_t
+      integer(c_int64_t), value :: t
+      integer(c_int) :: r
     end function
     function H5Dwrite(d, mtype, mspace, fspace, plist, buf) bind(C, name="H5Dwrite") result(r)
       import :: c_int, c_int64_t, c_ptr
@@ -300,6 +323,70 @@ contains
     real(rk), target :: a(1, 1)
     a(1, 1) = val
     call write_2d(f, name, a)
+  end subroutine
+
+  subroutine write_scalar_value(f, name, mtype, buf)
+    integer(c_int64_t), intent(in) :: f, mtype
+    character(len=*), intent(in) :: name
+    type(c_ptr), intent(in) :: buf
+    integer(c_int64_t) :: sp, d
+    integer(c_int) :: r
+    sp = H5Screate(H5S_SCALAR)
+    d = H5Dcreate2(f, cstr(name), mtype, sp, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT)
+    if (d < 0) error stop 'contour: cannot create dataset '//trim(name)
+    r = H5Dwrite(d, mtype, H5S_ALL, H5S_ALL, H5P_DEFAULT, buf)
+    r = H5Dclose(d)
+    r = H5Sclose(sp)
+  end subroutine
+
+  subroutine write_string(f, name, text)
+    integer(c_int64_t), intent(in) :: f
+    character(len=*), intent(in) :: name, text
+    character(kind=c_char, len=max(len_trim(text), 1)), target :: buf
+    integer(c_int64_t) :: t
+    integer(c_int) :: r
+    buf = trim(text)
+    if (len_trim(text) == 0) buf = ' '
+    t = H5Tcopy(h5t_c_s1)
+    r = H5Tset_size(t, int(len(buf), c_size_t))
+    call write_scalar_value(f, name, t, c_loc(buf))
+    r = H5Tclose(t)
+  end subroutine
+
+  !> contour_writer_t%write_hdf5 (src/lib/io/contour_writer.f90:193-384): results/step_NNNNNNN.h5 with the
+  !> scalars and the halo'd node and cell fields (ghost cells included, as plot_ghost_cells defaults to true).
+  !> Units attributes and the xdmf companion are not written.
+  subroutine write_contour(folder, title, iteration, time, delta_t, n_ghost, x, y, rho, u, v, p, cs)
+    character(len=*), intent(in) :: folder, title
+    integer, intent(in) :: iteration, n_ghost
+    real(rk), intent(in) :: time, delta_t
+    real(rk), intent(in) :: x(:, :), y(:, :), rho(:, :), u(:, :), v(:, :), p(:, :), cs(:, :)
+    character(len=512) :: path
+    integer(c_int64_t) :: f
+    integer(c_int), target :: ival
+    real(rk), target :: rval
+    integer(c_int) :: r
+    write(path, '(a, a, i0.7, a)') trim(folder), '/step_', iteration, '.h5'
+    r = H5open()
+    f = H5Fcreate(cstr(path), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT)
+    if (f < 0) error stop 'contour: cannot create '//trim(path)
+    call write_string(f, '/title', title)
+    ival = iteration
+    call write_scalar_value(f, '/iteration', h5t_native_int, c_loc(ival))
+    rval = time
+    call write_scalar_value(f, '/time', h5t_native_double, c_loc(rval))
+    rval = delta_t
+    call write_scalar_value(f, '/delta_t', h5t_native_double, c_loc(rval))
+    ival = n_ghost
+    call write_scalar_value(f, '/n_ghost_layers', h5t_native_int, c_loc(ival))
+    call write_2d(f, '/x', x)
+    call write_2d(f, '/y', y)
+    call write_2d(f, '/density', rho)
+    call write_2d(f, '/x_velocity', u)
+    call write_2d(f, '/y_velocity', v)
+    call write_2d(f, '/pressure', p)
+    call write_2d(f, '/sound_speed', cs)
+    r = H5Fclose(f)
   end subroutine
 
   !> Interior state of a solve: conserved and primitive fields, time and iteration count.

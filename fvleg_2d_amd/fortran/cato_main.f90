@@ -2,11 +2,14 @@
 !> input.ini and pycato HDF5 deck, runs the time loop of src/main.f90:117-198
 !> through the plugin objects, and writes the final state.
 !>
-!>   cato_hip_host input.ini [output.h5] [--device-loop] [--dry-run]
+!>   cato_hip_host input.ini [output.h5] [--device-loop] [--dry-run] [--contours DIR]
 !>
 !> --device-loop runs the same loop on the device (cato_run) instead of one
 !> host call per step; the results are identical.  --dry-run reads and checks
 !> the deck, prints the configuration and stops before touching a device.
+!> --contours DIR writes DIR/step_NNNNNNN.h5 at the contour cadence of
+!> main.f90 (iteration 0, every contour_interval_dt, and the last step), as the
+!> reference's contour writer does (host loop only).
 program cato_hip_host
   use, intrinsic :: iso_c_binding
   use cato_hip_api
@@ -23,6 +26,10 @@ program cato_hip_host
   real(rk), allocatable :: r(:, :), ru(:, :), rv(:, :), rE(:, :), cs(:, :)
   character(len=512) :: ini_file, out_file, arg
   logical :: device_loop, dry_run
+  character(len=512) :: contour_dir
+  type(hip_boundary_conditions_t) :: bcs
+  integer :: last_io_iteration
+  real(rk) :: last_dt
   integer :: nh, nargs, k, error_code, iteration
   real(rk) :: time, new_time, delta_t, next_output_time, contour_interval_dt, master_time
   integer(c_long_long) :: iters
@@ -37,15 +44,21 @@ program cato_hip_host
   out_file = 'final_state.h5'
   device_loop = .false.
   dry_run = .false.
-  do k = 2, nargs
+  contour_dir = ''
+  k = 2
+  do while (k <= nargs)
     call get_command_argument(k, arg)
     if (trim(arg) == '--device-loop') then
       device_loop = .true.
     else if (trim(arg) == '--dry-run') then
       dry_run = .true.
+    else if (trim(arg) == '--contours') then
+      k = k + 1
+      call get_command_argument(k, contour_dir)
     else
       out_file = arg
     end if
+    k = k + 1
   end do
 
   call read_input_ini(trim(ini_file), input)
@@ -86,6 +99,15 @@ program cato_hip_host
   end if
   call dev%create(cfg, grid)
   call fluid%initialize(dev, rho, u, v, p)
+  bcs%dev => dev
+  bcs%name = 'deck boundary conditions'
+  if (len_trim(contour_dir) > 0) then
+    if (device_loop) then
+      write(*, '(a)') 'Error: --contours needs the host-side loop (drop --device-loop)'
+      error stop 1
+    end if
+    call execute_command_line('mkdir -p '//trim(contour_dir))
+  end if
   write(*, '(a, i0, a, i0, a)') 'CATO (HIP hot path): ', grid%ni, ' x ', grid%nj, ' cells, '// &
     trim(input%flux_solver)//' / '//trim(input%limiter)//' / '//trim(input%integration_strategy)
 
@@ -103,10 +125,13 @@ program cato_hip_host
     iteration = int(iters)
     error_code = e
   else
-    ! main.f90:138-198 with the contour-interval clip of the driver time
+    ! main.f90:100-198: contour at iteration 0, then the loop with the contour-interval clip of the driver time
     next_output_time = 0.0_rk + contour_interval_dt
     master_time = 0.0_rk
     error_code = ALL_OK
+    last_dt = 0.0_rk
+    last_io_iteration = 0
+    if (len_trim(contour_dir) > 0) call contour(0, master_time, last_dt)
     do while (time < input%max_time .and. iteration < input%max_iterations)
       if (input%use_constant_delta_t .or. (iteration == 0 .and. input%initial_delta_t > 0.0_rk)) then
         delta_t = input%initial_delta_t
@@ -115,9 +140,14 @@ program cato_hip_host
       end if
       call fluid%integrate(delta_t, error_code)
       master_time = master_time + delta_t  ! master_puppeteer%time
+      last_dt = delta_t
       new_time = master_time
       if (error_code /= ALL_OK) exit
-      if (abs(time - next_output_time) < epsilon(1.0_rk)) next_output_time = next_output_time + contour_interval_dt
+      if (abs(time - next_output_time) < epsilon(1.0_rk)) then
+        next_output_time = next_output_time + contour_interval_dt
+        if (len_trim(contour_dir) > 0) call contour(iteration, master_time, last_dt)
+        last_io_iteration = iteration
+      end if
       if (new_time > next_output_time) then
         delta_t = abs(next_output_time - time)
         new_time = time + delta_t
@@ -125,6 +155,7 @@ program cato_hip_host
       time = new_time
       iteration = iteration + 1
     end do
+    if (len_trim(contour_dir) > 0 .and. iteration > last_io_iteration) call contour(iteration, master_time, last_dt)
   end if
   if (error_code /= ALL_OK) write(*, '(a, i0)') 'Something went wrong in the time integration, error code ', error_code
 
@@ -140,6 +171,29 @@ program cato_hip_host
   if (error_code /= ALL_OK) error stop 1
 
 contains
+
+  !> contour_writer_t%write_contour: primitives of the current state with BC-filled ghost cells
+  subroutine contour(it, t, dt_last)
+    integer, intent(in) :: it
+    real(rk), intent(in) :: t, dt_last
+    real(rk), allocatable :: hr(:, :), hu(:, :), hv(:, :), hp(:, :), hc(:, :)
+    real(rk), allocatable :: ir(:, :), iu(:, :), iv(:, :), ip(:, :), ic(:, :)
+    integer :: n1, n2
+    n1 = grid%ni + 2 * nh
+    n2 = grid%nj + 2 * nh
+    allocate(ir(grid%ni, grid%nj), iu(grid%ni, grid%nj), iv(grid%ni, grid%nj), ip(grid%ni, grid%nj))
+    allocate(ic(grid%ni, grid%nj))
+    call fluid%get_primitives(ir, iu, iv, ip, ic)
+    allocate(hr(n1, n2), hu(n1, n2), hv(n1, n2), hp(n1, n2), hc(n1, n2), source=0.0_rk)
+    hr(nh + 1:nh + grid%ni, nh + 1:nh + grid%nj) = ir
+    hu(nh + 1:nh + grid%ni, nh + 1:nh + grid%nj) = iu
+    hv(nh + 1:nh + grid%ni, nh + 1:nh + grid%nj) = iv
+    hp(nh + 1:nh + grid%ni, nh + 1:nh + grid%nj) = ip
+    call bcs%apply(hr, hu, hv, hp)
+    where (hr > 0.0_rk .and. hp >= 0.0_rk) hc = sqrt(cfg%gamma * hp / hr)
+    hc(nh + 1:nh + grid%ni, nh + 1:nh + grid%nj) = ic
+    call write_contour(trim(contour_dir), trim(input%title), it, t, dt_last, nh, x, y, hr, hu, hv, hp, hc)
+  end subroutine
 
   subroutine check_rc(rc, what)
     integer(c_int), intent(in) :: rc

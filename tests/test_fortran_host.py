@@ -88,3 +88,37 @@ def test_host_sod_matches_oracle(tmp_path, oracle, device_loop):
     assert got["iteration"] == n and got["time"] == t
     for key, name in (("density", "rho"), ("rho_u", "rho_u"), ("rho_v", "rho_v"), ("rho_E", "rho_E")):
         assert np.array_equal(got[key], f.interior(name)), key
+
+
+@pytest.mark.gpu
+def test_host_writes_contours_on_the_reference_cadence(tmp_path):
+    """main.f90:100-198 + contour_writer.f90:143-384: step_0000000.h5, one file per contour interval, and
+    the last step; the last contour holds the final state with BC-filled ghost cells."""
+    import glob
+
+    from fvleg_2d_amd.deck import read_contour, read_state, write_deck
+
+    pb = sod_problem(ni=100, nj=4, flux="AUSM+", limiter="minmod", integrator="ssp_rk2", cfl=0.4, max_time=0.2)
+    pb.contour_interval_dt = 0.05
+    ini = write_deck(pb, str(tmp_path), "sod")
+    out = os.path.join(str(tmp_path), "final.h5")
+    cdir = os.path.join(str(tmp_path), "results")
+    r = subprocess.run([host_or_fail(), ini, out, "--contours", cdir], capture_output=True, text=True, timeout=120)
+    print(r.stdout, flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    files = sorted(glob.glob(os.path.join(cdir, "step_*.h5")))
+    assert os.path.basename(files[0]) == "step_0000000.h5"
+    assert len(files) >= 5, files
+    c0 = read_contour(files[0])
+    assert c0["iteration"] == 0 and c0["time"] == 0.0 and c0["n_ghost_layers"] == 2
+    nh = 2
+    assert np.array_equal(c0["density"][nh:-nh, nh:-nh], pb.rho)
+    final = read_state(out)
+    last = read_contour(files[-1])
+    assert last["iteration"] == final["iteration"]
+    assert np.array_equal(last["density"][nh:-nh, nh:-nh], final["density"])
+    d = last["density"]
+    assert np.array_equal(d[:nh, nh:-nh], np.repeat(d[nh:nh + 1, nh:-nh], nh, axis=0))  # zero-gradient -x
+    assert np.array_equal(d[nh:-nh, :nh], d[nh:-nh, -2 * nh:-nh])  # periodic y
+    times = [read_contour(f)["time"] for f in files]
+    assert all(b > a for a, b in zip(times, times[1:]))
