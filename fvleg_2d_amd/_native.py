@@ -51,6 +51,7 @@ SIGNATURES = {
     "cato_set_primitive": (C.c_int, [_CTX, _DP, _DP, _DP, _DP, C.c_int, C.c_int]),
     "cato_set_conserved": (C.c_int, [_CTX, _DP, _DP, _DP, _DP, C.c_int, C.c_int]),
     "cato_get_conserved": (C.c_int, [_CTX, _DP, _DP, _DP, _DP, C.c_int, C.c_int]),
+    "cato_set_energy_source": (C.c_int, [_CTX, _DP, C.c_int, C.c_int]),
     "cato_get_primitive": (C.c_int, [_CTX, _DP, _DP, _DP, _DP, _DP, C.c_int, C.c_int]),
     "cato_compute_dt": (C.c_int, [_CTX, _DP]),
     "cato_step": (C.c_int, [_CTX, C.c_double, C.POINTER(C.c_int)]),

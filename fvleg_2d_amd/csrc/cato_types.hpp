@@ -73,6 +73,7 @@ struct StageArgs {
   const double *prim[4];     // rho, u, v, p of U_k (read with halo through the BC map)
   double *out[4];            // U_{k+1}
   double *prim_out[4];       // rho, u, v, p of U_{k+1} (null in MODE_RESIDUAL)
+  const double *src_E;       // energy source added to d(rho E)/dt (null: none; always null in MODE_RESIDUAL)
   const double *dxi, *lxi;   // per halo'd i: x spacing and x-edge length
   const double *dyj, *lyj;   // per halo'd j
   const LoopState *loop;     // dt source and done flag (may be null -> dt_value)

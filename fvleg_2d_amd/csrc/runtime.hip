@@ -239,6 +239,7 @@ StageArgs Solve::stage_args(int mode, double a, double b, double c, int in_buf, 
     s.out[v] = U[out_buf][v];
     s.prim_out[v] = p_out >= 0 ? P[p_out][v] : nullptr;
   }
+  s.src_E = mode == MODE_RESIDUAL ? nullptr : src_E;
   s.dxi = dxi;
   s.lxi = lxi;
   s.dyj = dyj;
@@ -590,6 +591,7 @@ int cato_ctx_destroy(cato_ctx *c) {
     (void)hipEventDestroy(p.first.second);
   }
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->s.src_E) (void)hipFree(c->s.src_E);
   if (c->mem) (void)hipFree(c->mem);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -627,6 +629,26 @@ int cato_set_conserved(cato_ctx *c, const double *r, const double *ru, const dou
   if ((rc = derive_primitives(c->stream, s))) return rc;
   s.prims_valid = false;
   return hip_check(hipStreamSynchronize(c->stream), "set_conserved");
+}
+
+int cato_set_energy_source(cato_ctx *c, const double *q, int ld, int halo) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  Solve &s = c->s;
+  if (!q) {
+    // the stages already enqueued keep reading the old field: free it after them
+    if (s.src_E) {
+      if ((rc = hip_check(hipStreamSynchronize(c->stream), "set_energy_source"))) return rc;
+      (void)hipFree(s.src_E);
+      s.src_E = nullptr;
+    }
+    return CATO_OK;
+  }
+  if (halo < 0 || ld < s.ni + 2 * halo) return fail(CATO_ERR_INVALID, "bad leading dimension / halo");
+  if (!s.src_E && (rc = hip_check(hipMalloc((void **)&s.src_E, s.field_elems() * sizeof(double)), "alloc source")))
+    return rc;
+  if ((rc = upload_field(c, s.src_E, q, ld, halo, s.U[(s.cur + 1) % 3][0]))) return rc;
+  return hip_check(hipStreamSynchronize(c->stream), "set_energy_source");
 }
 
 int cato_get_conserved(cato_ctx *c, double *r, double *ru, double *rv, double *rE, int ld, int halo) {

@@ -38,7 +38,8 @@ struct Solve {
   double *result = nullptr;
   int *block_err = nullptr;
   LoopState *loop = nullptr;
-  int cur = 0;              // which U[] holds U^n
+  double *src_E = nullptr;  // energy source field (pitched like U; owned by the caller of bind())
+  int cur = 0;             // which U[] holds U^n
   bool prims_valid = false; // P[0] holds the caller's primitives (not yet checked by a stage)
 
   int init(const cato_config &c, const cato_geom &g);

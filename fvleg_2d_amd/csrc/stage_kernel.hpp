@@ -247,6 +247,8 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
         // max-normalisation and flushes (they only shape rounding at 1e-16)
         L = -(((Fi * lr) - (Fim * ll)) + ((Gj * lt) - (Gjm * lb))) * inv_vol;
       }
+      // time_derivative's energy source (mod_fluid.f90:500-509): a plain sum
+      if (v == 3 && a.src_E) L = L + a.src_E[off];
       double r;
       if (a.mode == MODE_RESIDUAL) {
         r = L;

@@ -38,7 +38,7 @@ module cato_hip_api
   end type
 
   public :: cato_abi_version, cato_ctx_create, cato_ctx_destroy, cato_set_primitive, cato_set_conserved, &
-            cato_get_conserved, cato_get_primitive, cato_compute_dt, cato_step, cato_run, cato_apply_bc, &
+            cato_get_conserved, cato_get_primitive, cato_set_energy_source, cato_compute_dt, cato_step, cato_run, cato_apply_bc, &
             cato_interpolate_edges, cato_edge_fluxes, cato_residual, cato_error_message
 
   interface
@@ -79,6 +79,14 @@ module cato_hip_api
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx
       real(c_double), intent(in) :: rho(*), rho_u(*), rho_v(*), rho_E(*)
+      integer(c_int), value :: ld, halo
+      integer(c_int) :: rc
+    end function
+
+    !> q_dot: c_loc of a contiguous (ld, *) field, or c_null_ptr to clear the source
+    function cato_set_energy_source(ctx, q_dot, ld, halo) bind(C, name="cato_set_energy_source") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, q_dot
       integer(c_int), value :: ld, halo
       integer(c_int) :: rc
     end function

@@ -139,6 +139,7 @@ module cato_hip_plugins
     procedure :: initialize => fluid_initialize
     procedure :: get_timestep => fluid_get_timestep
     procedure :: integrate => fluid_integrate
+    procedure :: set_energy_source => fluid_set_energy_source
     procedure :: get_conserved => fluid_get_conserved
     procedure :: get_primitives => fluid_get_primitives
     procedure :: finalize => fluid_finalize
@@ -331,6 +332,20 @@ contains
     error_code = e
     self%time = self%time + dt
     self%iteration = self%iteration + 1
+  end subroutine
+
+  !> source_term of fluid_t%integrate (mod_fluid.f90:411-416, added to d(rho E)/dt by
+  !> time_derivative, mod_fluid.f90:500-509): held for every following step; absent clears it
+  subroutine fluid_set_energy_source(self, q_dot)
+    class(hip_fluid_t), intent(inout) :: self
+    real(rk), intent(in), target, contiguous, optional :: q_dot(:, :)
+    if (present(q_dot)) then
+      call require_halo(self%dev, q_dot, 'q_dot')
+      call check(cato_set_energy_source(self%dev%ctx, c_loc(q_dot), int(size(q_dot, 1), c_int), &
+                                        int(self%dev%nh, c_int)), 'fluid%set_energy_source')
+    else
+      call check(cato_set_energy_source(self%dev%ctx, c_null_ptr, 0_c_int, 0_c_int), 'fluid%set_energy_source')
+    end if
   end subroutine
 
   subroutine fluid_get_conserved(self, rho, rho_u, rho_v, rho_E)

@@ -105,6 +105,16 @@ class Solver:
         ld, halo = self._ld_halo(arrs[0])
         N.check(self.lib.cato_set_conserved(self.h, *[_p(a) for a in arrs], ld, halo), "cato_set_conserved")
 
+    def set_energy_source(self, q_dot):
+        """Energy source added to d(rho E)/dt in every stage of the following steps
+        (fluid_t%time_derivative, mod_fluid.f90:500-509); None clears it."""
+        if q_dot is None:
+            N.check(self.lib.cato_set_energy_source(self.h, None, self.ni, 0), "cato_set_energy_source")
+            return
+        q = _f64f(q_dot)
+        ld, halo = self._ld_halo(q)
+        N.check(self.lib.cato_set_energy_source(self.h, _p(q), ld, halo), "cato_set_energy_source")
+
     def conserved(self):
         out = [np.zeros((self.ni, self.nj), order="F") for _ in range(4)]
         N.check(self.lib.cato_get_conserved(self.h, *[_p(a) for a in out], self.ni, 0), "cato_get_conserved")

@@ -137,6 +137,12 @@ int cato_step(cato_ctx *ctx, double dt, int *error_code);
  * first error code.  The host only polls every few steps. */
 int cato_run(cato_ctx *ctx, double max_time, long long max_iterations, double contour_interval_dt, double constant_dt,
              long long *iterations, double *time, int *error_code);
+/* Energy source of fluid_t%time_derivative (mod_fluid.f90:500-509, with the field
+ * that source_t%integrate returns for the step, ssp_rk_3_3 mod_fluid.f90:767-770):
+ * q_dot (interior ni x nj, host layout ld/halo) is added to d(rho E)/dt in every
+ * stage of every following cato_step / cato_run / cato_advance until it is
+ * replaced or cleared (q_dot = NULL).  cato_residual (solver%solve) never adds it. */
+int cato_set_energy_source(cato_ctx *ctx, const double *q_dot, int ld, int halo);
 /* Enqueue exactly n steps with CFL dt and no host synchronisation (benchmarks). */
 int cato_advance(cato_ctx *ctx, long long n);
 int cato_synchronize(cato_ctx *ctx);

@@ -1068,9 +1068,16 @@ static int neg_check(const or_config *c, const double *x) {
   return found;
 }
 
-/* L(U) into d[0..3] (time_derivative, mod_fluid.f90:460-516): prims of U get BC-filled in place */
-static void time_derivative(const or_config *c, const or_geometry *g, stage_buf *U, double **d) {
+/* L(U) into d[0..3] (time_derivative, mod_fluid.f90:460-516): prims of U get BC-filled in place;
+ * the energy source field, when given, is added to d(rho E)/dt (mod_fluid.f90:500-509: a plain
+ * array sum, not a field operator). */
+static void time_derivative(const or_config *c, const or_geometry *g, stage_buf *U, double **d, const double *src) {
   or_solve(c, g, U->a[0], U->a[4], U->a[5], U->a[6], d[0], d[1], d[2], d[3]);
+  if (src) {
+    const int ni = c->ni, nj = c->nj, nh = c->nh;
+    for (int j = 1; j <= nj; ++j)
+      for (int i = 1; i <= ni; ++i) d[3][CI(i, j)] = d[3][CI(i, j)] + src[CI(i, j)];
+  }
 }
 
 /* out = base + L*s (fluid + fluid*real), interior, add-with-flush */
@@ -1122,26 +1129,26 @@ int or_integrate(const or_config *c, const or_geometry *g, or_fluid *f, double d
   if (c->integrator == OR_RK_SSP2) {
     /* ssp_rk_2_2 (mod_fluid.f90:1115-1129) */
     fluid_to_stage(f, &U1, n);
-    time_derivative(c, g, &U, L);
+    time_derivative(c, g, &U, L, f->src_E);
     combine1(c, &U, L, dt, &U1);
     err |= assign_stage(c, &U1);
     fluid_to_stage(f, &U2, n);
-    time_derivative(c, g, &U1, L);
+    time_derivative(c, g, &U1, L, f->src_E);
     combine2(c, &U, 0.5, &U1, 0.5, L, 0.5 * dt, &U2);
     err |= assign_stage(c, &U2);
     fin = &U2;
   } else if (c->integrator == OR_RK_SSP3) {
     /* ssp_rk_3_3 (mod_fluid.f90:773-802) */
     fluid_to_stage(f, &U1, n);
-    time_derivative(c, g, &U, L);
+    time_derivative(c, g, &U, L, f->src_E);
     combine1(c, &U, L, dt, &U1);
     err |= assign_stage(c, &U1);
     fluid_to_stage(f, &U2, n);
-    time_derivative(c, g, &U1, L);
+    time_derivative(c, g, &U1, L, f->src_E);
     combine2(c, &U, 3.0 / 4.0, &U1, 1.0 / 4.0, L, (1.0 / 4.0) * dt, &U2);
     err |= assign_stage(c, &U2);
     fluid_to_stage(f, &U3, n);
-    time_derivative(c, g, &U2, L);
+    time_derivative(c, g, &U2, L, f->src_E);
     combine2(c, &U, 1.0 / 3.0, &U2, 2.0 / 3.0, L, (2.0 / 3.0) * dt, &U3);
     err |= assign_stage(c, &U3);
     fin = &U3;
@@ -1164,7 +1171,7 @@ int or_integrate(const or_config *c, const or_geometry *g, or_fluid *f, double d
     fluid_to_stage(f, &U1, n); /* U_prev */
     fluid_to_stage(f, &U2, n); /* U_curr */
     for (int m = 0; m < ns; ++m) {
-      time_derivative(c, g, &U1, L);
+      time_derivative(c, g, &U1, L, f->src_E);
       combine1(c, &U, L, dt * alpha[m], &U2);
       err |= assign_stage(c, &U2);
       for (int k = 0; k < 8; ++k) memcpy(U1.a[k], U2.a[k], n * sizeof(double));

@@ -53,7 +53,8 @@ class _Geometry(C.Structure):
 
 
 class _Fluid(C.Structure):
-    _fields_ = [(n, C.POINTER(C.c_double)) for n in ("rho", "rho_u", "rho_v", "rho_E", "u", "v", "p", "cs")]
+    _fields_ = [(n, C.POINTER(C.c_double)) for n in ("rho", "rho_u", "rho_v", "rho_E", "u", "v", "p", "cs")] + \
+        [("src_E", C.POINTER(C.c_double))]
 
 
 def build(force: bool = False) -> str:
@@ -224,9 +225,20 @@ class Fluid:
         sl = (slice(nh, nh + ni), slice(nh, nh + nj))
         for name, src in (("rho", rho), ("u", u), ("v", v), ("p", p)):
             getattr(self, name)[sl] = np.asarray(src, dtype=np.float64)[sl] if src.shape == cfg.shape else src
-        self._f = _Fluid(*[_p(getattr(self, n)) for n in self.NAMES])
+        self._f = _Fluid(*[_p(getattr(self, n)) for n in self.NAMES], None)
+        self._src = None
         c = cfg.c()
         lib().or_fluid_init_from_prims(C.byref(c), C.byref(self._f))
+
+    def set_energy_source(self, src):
+        """Halo'd field added to d(rho E)/dt in every stage (None removes it)."""
+        if src is None:
+            self._src = None
+            self._f.src_E = None
+        else:
+            self._src = np.asfortranarray(src, dtype=np.float64)
+            assert self._src.shape == self.cfg.shape
+            self._f.src_E = _p(self._src)
 
     def get_timestep(self) -> float:
         c = self.cfg.c()

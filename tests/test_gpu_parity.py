@@ -195,6 +195,47 @@ def test_step_parity_integrators(oracle, integrator, flux):
         assert np.array_equal(g[k], o[k]), (k, rel_linf(g[k], o[k]))
 
 
+@pytest.mark.parametrize("integrator", ["ssp_rk2", "ssp_rk3", "rk(2,4)"])
+def test_energy_source_parity(oracle, integrator):
+    """time_derivative's energy source (mod_fluid.f90:500-509): a non-uniform q_dot held over several steps,
+    then cleared; residual() (solver%solve) must ignore it."""
+    say("energy source", integrator)
+    ni, nj = 41, 27
+    cfg = SchemeConfig(flux="SLAU2", limiter="MLP5", integrator=integrator,
+                       bcs=("zero_gradient", "symmetry", "periodic", "periodic"), cfl=0.35)
+    nh = cfg.n_ghost
+    nx, ny = uniform_nodes(ni, nj, nh, (0.0, 1.0), (0.0, 0.7))
+    prims = shock_state(ni, nj, 11)
+    rng = np.random.default_rng(12)
+    X, Y = np.meshgrid((np.arange(ni) + 0.5) / ni, (np.arange(nj) + 0.5) / nj, indexing="ij")
+    q = np.asfortranarray(40.0 * np.exp(-((X - 0.4) ** 2 + (Y - 0.3) ** 2) / 0.02) + rng.uniform(0, 1, (ni, nj)))
+    oc = oracle_cfg(oracle, cfg, ni, nj)
+    og = oracle.Geometry(oc, nx, ny)
+    f = oracle.Fluid(oc, og, *[pad(a, nh) for a in prims])
+    f.set_energy_source(pad(q, nh))
+    with Solver(cfg, nx, ny) as s:
+        s.set_primitive(*prims)
+        s.set_energy_source(pad(q, nh))  # halo'd layout; the interior is what counts
+        for k in range(6):
+            if k == 4:
+                f.set_energy_source(None)
+                s.set_energy_source(None)
+            dt = s.compute_dt()
+            assert dt == f.get_timestep(), k
+            assert s.step(dt) == 0 and f.integrate(dt) == 0
+        s.set_energy_source(q)  # interior layout
+        g = s.conserved()
+        gp = s.primitive()
+        rp = [pad(a, nh) for a in gp[:4]]
+        gd = s.residual(*[a.copy(order="F") for a in rp])
+    o = [f.interior(n) for n in ("rho", "rho_u", "rho_v", "rho_E")]
+    for k in range(4):
+        assert np.array_equal(g[k], o[k]), (k, rel_linf(g[k], o[k]))
+    od = oracle.solve(oc, og, *[a.copy(order="F") for a in rp])
+    for k in range(4):
+        assert np.array_equal(gd[k], od[k]), ("residual", k)
+
+
 def run_both(oracle, pb, max_iterations=0):
     nh = pb.cfg.n_ghost
     oc = oracle_cfg(oracle, pb.cfg, pb.ni, pb.nj)

@@ -373,3 +373,26 @@ def test_m_ausmpw_strong_shock_tube_against_exact_riemann(oracle):
     l1 = np.mean(np.abs(r[:, 0] - ex[0]))
     assert l1 < 0.03, l1
     assert np.abs(r - r[:, :1]).max() < 1e-8
+
+
+@pytest.mark.parametrize("integrator", ["ssp_rk2", "ssp_rk3", "rk(2,4)"])
+def test_energy_source_adds_s_dt_to_rho_E(oracle, integrator):
+    """mod_fluid.f90:500-509: d(rho E)/dt gains the source field in every stage, so on a uniform periodic
+    state one step of any consistent scheme adds S*dt to rho E and leaves rho, rho u, rho v alone."""
+    n = 8
+    cfg = oracle.Config(n, n, flux="SLAU", limiter="minmod", integrator=integrator, bcs=("periodic",) * 4)
+    g = oracle.Geometry(cfg, *oracle.uniform_nodes(n, n, cfg.nh))
+    rho, u, v, p = (cfg.field() for _ in range(4))
+    rho[:] = 1.3
+    u[:] = 0.2
+    v[:] = -0.1
+    p[:] = 0.9
+    f = oracle.Fluid(cfg, g, rho, u, v, p)
+    before = {k: f.interior(k) for k in ("rho", "rho_u", "rho_v", "rho_E")}
+    S = 0.37
+    f.set_energy_source(np.full(cfg.shape, S, order="F"))
+    dt = 0.01
+    assert f.integrate(dt) == 0
+    assert np.allclose(f.interior("rho_E") - before["rho_E"], S * dt, rtol=0, atol=1e-13)
+    for k in ("rho", "rho_u", "rho_v"):
+        assert np.allclose(f.interior(k), before[k], rtol=1e-15, atol=0)
