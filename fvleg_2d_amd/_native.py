@@ -58,6 +58,9 @@ SIGNATURES = {
     "cato_run": (C.c_int, [_CTX, C.c_double, C.c_longlong, C.c_double, C.c_double, C.POINTER(C.c_longlong),
                            _DP, C.POINTER(C.c_int)]),
     "cato_advance": (C.c_int, [_CTX, C.c_longlong]),
+    "cato_set_diagnostics": (C.c_int, [_CTX, C.c_int]),
+    "cato_get_convergence": (C.c_int, [_CTX, _DP]),
+    "cato_get_history": (C.c_int, [_CTX, _DP, C.c_longlong, C.POINTER(C.c_longlong)]),
     "cato_synchronize": (C.c_int, [_CTX]),
     "cato_set_timing": (C.c_int, [_CTX, C.c_int]),
     "cato_get_timing": (C.c_int, [_CTX, _DP, C.POINTER(C.c_longlong), _DP, C.POINTER(C.c_longlong)]),

@@ -166,12 +166,43 @@ __device__ inline void loop_update(LoopState *L, double m, int e, int mode) {
 // Reduce per-workgroup partials into out[0] = min dt, out[1] = flags; clears flags.
 // mode 0: plain reduction into out; mode 1: initialise the loop (first dt);
 // mode 2: end of a step of the main.f90 time loop.
+// With cv.sums set it also forms the convergence norms of the step: the block
+// sums of squares are added in a fixed order (threads, then waves), so the
+// norms are the same on every run; NORM2's own order agrees to a few ulps.
 static __global__ __launch_bounds__(1024) void k_reduce_finalize(int nb, double *__restrict__ block_min,
                                                           int *__restrict__ block_err, double *__restrict__ out,
-                                                          LoopState *__restrict__ L, int mode) {
+                                                          LoopState *__restrict__ L, int mode, ConvOut cv) {
   __shared__ double sm[16];
   __shared__ int se[16];
+  __shared__ double ss[16][8];
   if (mode == 2 && L->done) return;
+  double nrm[4] = {0.0, 0.0, 0.0, 0.0};
+  if (cv.sums) {
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      double t = 0.0;
+      for (int b = threadIdx.x; b < nb; b += blockDim.x) t = t + cv.sums[(size_t)v * nb + b];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) t = t + __shfl_xor(t, o, 64);
+      if ((threadIdx.x & 63) == 0) ss[threadIdx.x >> 6][v] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int nw = (blockDim.x + 63) / 64;
+      double t[8];
+      for (int v = 0; v < 8; ++v) {
+        t[v] = ss[0][v];
+        for (int k = 1; k < nw; ++k) t[v] = t[v] + ss[k][v];
+      }
+      // norm2(last - first) / norm2(first); the momenta divide by max(norm2(first), 1)
+      for (int v = 0; v < 4; ++v) {
+        double den = sqrt(t[4 + v]);
+        if (v == 1 || v == 2) den = den > 1.0 ? den : 1.0;
+        nrm[v] = sqrt(t[v]) / den;
+        cv.norms[v] = nrm[v];
+      }
+    }
+  }
   double m = __builtin_inf();
   int e = 0;
   for (int b = threadIdx.x; b < nb; b += blockDim.x) {
@@ -199,7 +230,16 @@ static __global__ __launch_bounds__(1024) void k_reduce_finalize(int nb, double 
     out[0] = m;
     out[1] = (double)e;
   }
+  const double step_dt = (mode == 2) ? L->dt : 0.0;
   loop_update(L, m, e, mode);
+  if (mode == 2 && cv.sums && cv.hist) {
+    // write_residual_history (mod_fluid.f90:1207-1223) row of the step just closed
+    double *row = cv.hist + (size_t)((L->iteration - 1) % cv.hist_cap) * kHistRow;
+    row[0] = (double)L->iteration;
+    row[1] = L->master_time;
+    row[2] = step_dt;
+    for (int v = 0; v < 4; ++v) row[3 + v] = nrm[v];
+  }
 }
 
 static __global__ void k_set_scalar(double *dst, double value) { *dst = value; }

@@ -218,7 +218,7 @@ int BatchEngine::run(const double *max_time, const double *contour_dt, long long
     hipLaunchKernelGGL(k_cfl_partial, dim3((unsigned)nb), dim3(kThreads), 0, stream, g, 1, a[0],
                        a[1], a[2], a[3], sv.block_min, sv.block_err);
     hipLaunchKernelGGL(k_reduce_finalize, dim3(1), dim3(1024), 0, stream, nb, sv.block_min, sv.block_err, sv.result,
-                       sv.loop, 1);
+                       sv.loop, 1, ConvOut{});
     BCK(hipGetLastError());
   }
   // first step eagerly (its first stage reads the caller's primitives)

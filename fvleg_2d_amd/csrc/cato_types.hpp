@@ -53,6 +53,17 @@ struct LoopState {
   int pad[2];
 };
 
+// Convergence diagnostics of the step a finalize closes (calculate_convergence,
+// mod_fluid.f90:1162-1205): the per-block sums the last stage wrote, the four
+// norms, and an optional device ring of history rows for the device-side loop.
+constexpr int kHistRow = 7;  // iteration, time, dt, L2 norms of rho, rho u, rho v, rho E
+struct ConvOut {
+  const double *sums;  // 8 arrays of nb block sums (null: diagnostics off)
+  double *norms;       // 4 norms of the last step
+  double *hist;        // hist_cap rows of kHistRow, row k of a run at k % hist_cap (null: none)
+  int hist_cap;
+};
+
 // Per-field pointers of one conserved state, interior only, pitched rows.
 struct State4 {
   double *f[4];
@@ -80,6 +91,8 @@ struct StageArgs {
   double dt_value;
   double *block_min;         // per-block CFL minimum (last stage)
   int *block_err;            // per-block sanity flags
+  const double *conv_first[4];  // last stage: the state calculate_convergence compares against (null: off)
+  double *block_conv;        // per-block sums for the convergence norms, 8 arrays of tiles_x*tiles_y
 };
 
 }  // namespace cato

@@ -317,6 +317,15 @@ std::vector<StageArgs> Solve::step_stages(const LoopState *lp, double dt_value) 
       }
     }
   }
+  if (conv_sums) {
+    // calculate_convergence compares the last stage with U_1 (ssp_rk_2_2:1132,
+    // ssp_rk_3_3:805; still in buffer A) or with U_0 (rk_multistage:970; read
+    // at the cell before the last stage overwrites it)
+    const int first = (cfg.integrator == CATO_RK_SSP2 || cfg.integrator == CATO_RK_SSP3) ? A : n;
+    StageArgs &l = st.back();
+    for (int v = 0; v < 4; ++v) l.conv_first[v] = U[first][v];
+    l.block_conv = conv_sums;
+  }
   return st;
 }
 
@@ -409,6 +418,12 @@ struct cato_ctx {
   bool timing = false;
   double stage_ms = 0.0, other_ms = 0.0;
   long long stage_n = 0, other_n = 0;
+  // convergence diagnostics: one device allocation (block sums, norms, history
+  // ring), the rows collected so far and the step counters of the rows
+  double *diag = nullptr;
+  std::vector<double> history;
+  long long diag_iter = 0;
+  double diag_time = 0.0;
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<std::pair<hipEvent_t, hipEvent_t>, int>> pending;  // (start, stop), is_stage
 };
@@ -476,12 +491,14 @@ int launch_stages(cato_ctx *c, const std::vector<StageArgs> &st) {
   return CATO_OK;
 }
 
-int launch_reduce(cato_ctx *c, int mode) {
+int launch_reduce(cato_ctx *c, int mode, bool with_conv) {
   Solve &s = c->s;
   int nb = s.tiles_x * s.tiles_y;
   return timed(c, false, [&] {
+    // the convergence sums belong to a step (modes 0 and 2 after stages), not to a CFL pass
+    ConvOut cv = with_conv ? s.conv_out() : ConvOut{};
     hipLaunchKernelGGL(k_reduce_finalize, dim3(1), dim3(1024), 0, c->stream, nb, s.block_min, s.block_err, s.result,
-                       s.loop, mode);
+                       s.loop, mode, cv);
   });
 }
 
@@ -496,7 +513,7 @@ int launch_cfl(cato_ctx *c, int loop_mode) {
                        s.P[0][2], s.P[0][3], s.block_min, s.block_err);
   });
   if (rc) return rc;
-  return launch_reduce(c, loop_mode);
+  return launch_reduce(c, loop_mode, false);
 }
 
 int check_ctx(cato_ctx *c) { return c ? CATO_OK : fail(CATO_ERR_INVALID, "null context"); }
@@ -592,6 +609,7 @@ int cato_ctx_destroy(cato_ctx *c) {
   }
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->s.src_E) (void)hipFree(c->s.src_E);
+  if (c->diag) (void)hipFree(c->diag);
   if (c->mem) (void)hipFree(c->mem);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -613,6 +631,8 @@ int cato_set_primitive(cato_ctx *c, const double *rho, const double *u, const do
                      s.U[s.cur][0], s.U[s.cur][1], s.U[s.cur][2], s.U[s.cur][3]);
   HIPCK(hipGetLastError());
   s.prims_valid = true;
+  c->diag_iter = 0;
+  c->diag_time = 0.0;
   return hip_check(hipStreamSynchronize(c->stream), "set_primitive");
 }
 
@@ -628,6 +648,8 @@ int cato_set_conserved(cato_ctx *c, const double *r, const double *ru, const dou
     return rc;
   if ((rc = derive_primitives(c->stream, s))) return rc;
   s.prims_valid = false;
+  c->diag_iter = 0;
+  c->diag_time = 0.0;
   return hip_check(hipStreamSynchronize(c->stream), "set_conserved");
 }
 
@@ -700,10 +722,18 @@ int cato_step(cato_ctx *c, double dt, int *error_code) {
     return fail(CATO_ERR_INVALID, "The timestep dt is < tiny(1.0_rk)");  // slau_solver.fypp:162-166
   if ((rc = launch_stages(c, s.step_stages(nullptr, dt)))) return rc;
   s.prims_valid = false;
-  if ((rc = launch_reduce(c, 0))) return rc;
-  double r[2];
+  if ((rc = launch_reduce(c, 0, true))) return rc;
+  double r[2], nrm[4];
   HIPCK(hipMemcpyAsync(r, s.result, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+  if (c->diag) HIPCK(hipMemcpyAsync(nrm, s.conv_norms, sizeof(nrm), hipMemcpyDeviceToHost, c->stream));
   HIPCK(hipStreamSynchronize(c->stream));
+  if (c->diag) {
+    // fluid_t%integrate: iteration + 1 (mod_fluid.f90:422), time + dt (ssp_rk_3_3:844)
+    c->diag_iter += 1;
+    c->diag_time = c->diag_time + dt;
+    const double row[kHistRow] = {(double)c->diag_iter, c->diag_time, dt, nrm[0], nrm[1], nrm[2], nrm[3]};
+    c->history.insert(c->history.end(), row, row + kHistRow);
+  }
   if (c->timing) collect_timing(c);
   if (error_code) {
     int f = (int)r[1];
@@ -729,22 +759,39 @@ static int run_impl(cato_ctx *c, double max_time, long long max_iter, double con
   if (rc) return rc;
   long long launched = 0;
   const long long chunk = 32;
+  static_assert(32 < Solve::kHistCap, "a poll must find every row of its chunk in the ring");
   LoopState H{};
   while (true) {
     long long n = poll ? chunk : fixed_steps;
     for (long long k = 0; k < n; ++k) {
       if ((rc = launch_stages(c, s.step_stages(s.loop, 0.0)))) return rc;
       s.prims_valid = false;
-      if ((rc = launch_reduce(c, 2))) return rc;
+      if ((rc = launch_reduce(c, 2, true))) return rc;
     }
     launched += n;
     if (!poll) break;
+    long long seen = H.iteration;
     HIPCK(hipMemcpyAsync(&H, s.loop, sizeof(H), hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> ring;
+    if (c->diag) {
+      ring.resize((size_t)Solve::kHistCap * kHistRow);
+      HIPCK(hipMemcpyAsync(ring.data(), s.conv_hist, ring.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           c->stream));
+    }
     HIPCK(hipStreamSynchronize(c->stream));
     if (c->timing) collect_timing(c);
+    // rows of the steps since the last poll (at most `chunk` < kHistCap of them)
+    for (long long it = seen + 1; c->diag && it <= H.iteration; ++it) {
+      const double *row = &ring[(size_t)((it - 1) % Solve::kHistCap) * kHistRow];
+      c->history.push_back((double)(c->diag_iter + it));
+      c->history.push_back(c->diag_time + row[1]);
+      c->history.insert(c->history.end(), row + 2, row + kHistRow);
+    }
     if (H.done) break;
   }
   if (!poll) return CATO_OK;
+  c->diag_iter += H.iteration;
+  c->diag_time = c->diag_time + H.master_time;
   if (iters) *iters = H.iteration;
   if (time) *time = H.time;
   if (err) *err = H.error;
@@ -773,6 +820,55 @@ int cato_synchronize(cato_ctx *c) {
   if (rc) return rc;
   HIPCK(hipStreamSynchronize(c->stream));
   if (c->timing) collect_timing(c);
+  return CATO_OK;
+}
+
+int cato_set_diagnostics(cato_ctx *c, int enable) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  Solve &s = c->s;
+  HIPCK(hipStreamSynchronize(c->stream));
+  c->history.clear();
+  c->diag_iter = 0;
+  c->diag_time = 0.0;
+  if (!enable) {
+    if (c->diag) (void)hipFree(c->diag);
+    c->diag = nullptr;
+    s.conv_sums = s.conv_norms = s.conv_hist = nullptr;
+    return CATO_OK;
+  }
+  if (c->diag) return CATO_OK;
+  const size_t nb = (size_t)s.tiles_x * s.tiles_y;
+  const size_t n = 8 * nb + 4 + (size_t)Solve::kHistCap * kHistRow;
+  HIPCK(hipMalloc((void **)&c->diag, n * sizeof(double)));
+  HIPCK(hipMemsetAsync(c->diag, 0, n * sizeof(double), c->stream));
+  s.conv_sums = c->diag;
+  s.conv_norms = c->diag + 8 * nb;
+  s.conv_hist = s.conv_norms + 4;
+  return hip_check(hipStreamSynchronize(c->stream), "set_diagnostics");
+}
+
+int cato_get_convergence(cato_ctx *c, double *norms) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!norms) return fail(CATO_ERR_INVALID, "null output pointer");
+  if (!c->diag) return fail(CATO_ERR_INVALID, "convergence diagnostics are off (cato_set_diagnostics)");
+  HIPCK(hipMemcpyAsync(norms, c->s.conv_norms, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  return hip_check(hipStreamSynchronize(c->stream), "get_convergence");
+}
+
+int cato_get_history(cato_ctx *c, double *rows, long long max_rows, long long *n_rows) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  const long long pending = (long long)(c->history.size() / kHistRow);
+  if (!rows) {
+    if (n_rows) *n_rows = pending;
+    return CATO_OK;
+  }
+  const long long n = std::min(pending, std::max(max_rows, 0LL));
+  std::copy(c->history.begin(), c->history.begin() + n * kHistRow, rows);
+  c->history.erase(c->history.begin(), c->history.begin() + n * kHistRow);
+  if (n_rows) *n_rows = n;
   return CATO_OK;
 }
 

@@ -39,6 +39,10 @@ struct Solve {
   int *block_err = nullptr;
   LoopState *loop = nullptr;
   double *src_E = nullptr;  // energy source field (pitched like U; owned by the caller of bind())
+  // convergence diagnostics (owned by the caller of bind()): 8 arrays of block
+  // sums, the 4 norms of the last step, and a ring of history rows
+  double *conv_sums = nullptr, *conv_norms = nullptr, *conv_hist = nullptr;
+  static constexpr int kHistCap = 64;
   int cur = 0;             // which U[] holds U^n
   bool prims_valid = false; // P[0] holds the caller's primitives (not yet checked by a stage)
 
@@ -51,6 +55,7 @@ struct Solve {
   StageArgs stage_args(int mode, double a, double b, double c, int in_buf, int u0_buf, int out_buf, int p_in,
                        int p_out, bool last, const LoopState *lp, double dt_value, bool check_input) const;
   std::vector<StageArgs> step_stages(const LoopState *lp, double dt_value) const;
+  ConvOut conv_out() const { return ConvOut{conv_sums, conv_norms, conv_hist, kHistCap}; }
 };
 
 // Host <-> device copy of one interior field with host layout (ld, halo).

@@ -271,7 +271,7 @@ int admit_into(cato_server *S, Request &R, Solve &solve, int k, size_t off) {
   hipLaunchKernelGGL(k_cfl_partial, dim3((unsigned)nb), dim3(kThreads), 0, st, fg, 1, sv->P[0][0], sv->P[0][1],
                      sv->P[0][2], sv->P[0][3], sv->block_min, sv->block_err);
   hipLaunchKernelGGL(k_reduce_finalize, dim3(1), dim3(1024), 0, st, nb, sv->block_min, sv->block_err, sv->result,
-                     sv->loop, 1);
+                     sv->loop, 1, ConvOut{});
   if ((rc = hip_check(hipGetLastError(), "admit kernels"))) return rc;
   return hip_check(hipStreamSynchronize(st), "admit");  // L lives on this stack frame
 }

@@ -62,6 +62,11 @@ template <typename T> __device__ inline T wave_min(T v) {
   }
   return v;
 }
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, 64);
+  return v;
+}
 __device__ inline int wave_or(int v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
@@ -114,10 +119,11 @@ template <int LIMC, typename R, int TX, int TY> struct TileSmem {
   R sfj[4][TY + 1][TX];
 };
 
-// Block-wide scratch for the CFL / sanity reduction.
+// Block-wide scratch for the CFL / sanity / convergence reductions.
 struct RedSmem {
   double m[kThreads / 64];
   int e[kThreads / 64];
+  double s[kThreads / 64][8];
 };
 
 // One tile of one stage.  Writes the block's CFL partial and sanity flags to
@@ -218,6 +224,12 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
 
   // ---- 3. divergence + stage combination ---------------------------------
   double lmin = __builtin_inf();
+  // calculate_convergence sums (last stage, when requested): squares of
+  // (new - first stage) in [0..3] and of the first stage in [4..7]
+  const bool conv = a.last_stage && a.conv_first[0] != nullptr;
+  double csq[8];
+#pragma unroll
+  for (int v = 0; v < 8; ++v) csq[v] = 0.0;
   const double EPSD = Lim<double>::eps;
   for (int c = tid; c < TX * TY; c += kThreads) {
     int cy = c / TX, cx = c - cy * TX;
@@ -261,6 +273,11 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
         r = op_add(t, L * (a.c * dt));
       }
       nv[v] = r;
+      if (conv) {
+        double f = a.conv_first[v][off], d = r - f;
+        csq[v] = csq[v] + d * d;
+        csq[4 + v] = csq[4 + v] + f * f;
+      }
       a.out[v][off] = r;
     }
     if (a.mode == MODE_RESIDUAL) continue;
@@ -280,6 +297,22 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
       // get_timestep (mod_fluid.f90:554-558); min_dx/min_dy = |dx|, |dy| on rectilinear cells
       double val = a.cfl / (((absd(q.u) + cs) / absd(dxv)) + ((absd(q.v) + cs) / absd(dyv)));
       lmin = (val < lmin) ? val : lmin;
+    }
+  }
+
+  if (conv) {
+    const int w = tid >> 6;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      double t = wave_sum(csq[v]);
+      if ((tid & 63) == 0) red.s[w][v] = t;
+    }
+    __syncthreads();
+    if (tid < 8) {
+      double t = red.s[0][tid];
+#pragma unroll
+      for (int k = 1; k < kThreads / 64; ++k) t = t + red.s[k][tid];
+      a.block_conv[(size_t)tid * (size_t)(a.tiles_x * a.tiles_y) + tile] = t;
     }
   }
 

@@ -24,6 +24,11 @@ module cato_deck
     real(rk) :: cfl = 0.1_rk
     character(len=32) :: integration_strategy = ''
     integer :: max_iterations = huge(1)
+    logical :: enable_startup_time = .false.
+    real(rk) :: startup_period = 0.0_rk, startup_delta_t = 0.0_rk
+    logical :: allow_subcycle = .false.
+    ! [source_terms]
+    logical :: enable_source_terms = .false.
     ! [reference_state]
     logical :: non_dimensionalize = .true.
     real(rk) :: reference_length = 1.0_rk, reference_density = 1.0_rk, reference_velocity = 1.0_rk
@@ -228,6 +233,11 @@ contains
       case ('time.cfl'); inp%cfl = to_real(val, key)
       case ('time.integration_strategy'); inp%integration_strategy = val
       case ('time.max_iterations'); inp%max_iterations = int(to_real(val, key))
+      case ('time.enable_startup_time'); inp%enable_startup_time = to_logical(val)
+      case ('time.startup_period'); inp%startup_period = to_real(val, key)
+      case ('time.startup_delta_t'); inp%startup_delta_t = to_real(val, key)
+      case ('time.allow_subcycle'); inp%allow_subcycle = to_logical(val)
+      case ('source_terms.enable_source_terms'); inp%enable_source_terms = to_logical(val)
       case ('reference_state.non_dimensionalize'); inp%non_dimensionalize = to_logical(val)
       case ('reference_state.reference_length'); inp%reference_length = to_real(val, key)
       case ('reference_state.reference_density'); inp%reference_density = to_real(val, key)

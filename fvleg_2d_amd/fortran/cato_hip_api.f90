@@ -38,7 +38,8 @@ module cato_hip_api
   end type
 
   public :: cato_abi_version, cato_ctx_create, cato_ctx_destroy, cato_set_primitive, cato_set_conserved, &
-            cato_get_conserved, cato_get_primitive, cato_set_energy_source, cato_compute_dt, cato_step, cato_run, cato_apply_bc, &
+            cato_get_conserved, cato_get_primitive, cato_set_energy_source, cato_set_diagnostics, &
+            cato_get_convergence, cato_get_history, cato_compute_dt, cato_step, cato_run, cato_apply_bc, &
             cato_interpolate_edges, cato_edge_fluxes, cato_residual, cato_error_message
 
   interface
@@ -80,6 +81,30 @@ module cato_hip_api
       type(c_ptr), value :: ctx
       real(c_double), intent(in) :: rho(*), rho_u(*), rho_v(*), rho_E(*)
       integer(c_int), value :: ld, halo
+      integer(c_int) :: rc
+    end function
+
+    function cato_set_diagnostics(ctx, enable) bind(C, name="cato_set_diagnostics") result(rc)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: enable
+      integer(c_int) :: rc
+    end function
+
+    function cato_get_convergence(ctx, norms) bind(C, name="cato_get_convergence") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: norms(4)
+      integer(c_int) :: rc
+    end function
+
+    !> rows(7, max_rows): iteration, time, dt, norms(4) per step, oldest first
+    function cato_get_history(ctx, rows, max_rows, n_rows) bind(C, name="cato_get_history") result(rc)
+      import :: c_int, c_ptr, c_double, c_long_long
+      type(c_ptr), value :: ctx
+      real(c_double), intent(inout) :: rows(*)
+      integer(c_long_long), value :: max_rows
+      integer(c_long_long), intent(out) :: n_rows
       integer(c_int) :: rc
     end function
 

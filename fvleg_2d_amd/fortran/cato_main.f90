@@ -9,7 +9,9 @@
 !> the deck, prints the configuration and stops before touching a device.
 !> --contours DIR writes DIR/step_NNNNNNN.h5 at the contour cadence of
 !> main.f90 (iteration 0, every contour_interval_dt, and the last step), as the
-!> reference's contour writer does (host loop only).
+!> reference's contour writer does (host loop only).  residual_hist.csv is
+!> written in the working directory as fluid_t does (mod_fluid.f90:244-246,
+!> 1207-1223), one row per step; --no-residual-hist skips it.
 program cato_hip_host
   use, intrinsic :: iso_c_binding
   use cato_hip_api
@@ -25,7 +27,8 @@ program cato_hip_host
   real(rk), allocatable :: x(:, :), y(:, :), rho(:, :), u(:, :), v(:, :), p(:, :)
   real(rk), allocatable :: r(:, :), ru(:, :), rv(:, :), rE(:, :), cs(:, :)
   character(len=512) :: ini_file, out_file, arg
-  logical :: device_loop, dry_run
+  logical :: device_loop, dry_run, residual_hist, constant_dt
+  integer :: hist_unit
   character(len=512) :: contour_dir
   type(hip_boundary_conditions_t) :: bcs
   integer :: last_io_iteration
@@ -44,6 +47,7 @@ program cato_hip_host
   out_file = 'final_state.h5'
   device_loop = .false.
   dry_run = .false.
+  residual_hist = .true.
   contour_dir = ''
   k = 2
   do while (k <= nargs)
@@ -52,6 +56,8 @@ program cato_hip_host
       device_loop = .true.
     else if (trim(arg) == '--dry-run') then
       dry_run = .true.
+    else if (trim(arg) == '--no-residual-hist') then
+      residual_hist = .false.
     else if (trim(arg) == '--contours') then
       k = k + 1
       call get_command_argument(k, contour_dir)
@@ -69,6 +75,15 @@ program cato_hip_host
   if (input%non_dimensionalize .and. (input%reference_length /= 1.0_rk .or. input%reference_density /= 1.0_rk &
                                       .or. input%reference_velocity /= 1.0_rk)) then
     write(*, '(a)') 'Error: reference_state scaling other than 1 is not supported by this host'
+    error stop 1
+  end if
+  if (input%allow_subcycle) then
+    write(*, '(a)') 'Error: allow_subcycle = true is not supported (no reference deck sets it)'
+    error stop 1
+  end if
+  if (input%enable_source_terms) then
+    write(*, '(a)') 'Error: [source_terms] from the deck are not supported; the library takes the step''s '// &
+      'source field through fluid%set_energy_source'
     error stop 1
   end if
   cfg = config_from_input(input)
@@ -114,9 +129,16 @@ program cato_hip_host
   contour_interval_dt = input%contour_interval_dt
   time = 0.0_rk
   iteration = 0
-  if (device_loop .and. .not. input%use_constant_delta_t .and. input%initial_delta_t > 0.0_rk) then
-    write(*, '(a)') 'Error: --device-loop does not support a given first time step (initial_delta_t)'
+  if (device_loop .and. (input%enable_startup_time .or. &
+                         (.not. input%use_constant_delta_t .and. input%initial_delta_t > 0.0_rk))) then
+    write(*, '(a)') 'Error: --device-loop does not support a given first time step (initial_delta_t) '// &
+      'or a startup period'
     error stop 1
+  end if
+  if (residual_hist) then
+    call check_rc(cato_set_diagnostics(dev%ctx, 1_c_int), 'cato_set_diagnostics')
+    open(newunit=hist_unit, file='residual_hist.csv', status='replace')
+    write(hist_unit, '(a)') 'iteration,time,dt,rho,rho_u,rho_v,rho_E'
   end if
   if (device_loop) then
     call check_rc(cato_run(dev%ctx, input%max_time, int(input%max_iterations, c_long_long), contour_interval_dt, &
@@ -124,6 +146,7 @@ program cato_hip_host
                   'cato_run')
     iteration = int(iters)
     error_code = e
+    if (residual_hist) call drain_history()
   else
     ! main.f90:100-198: contour at iteration 0, then the loop with the contour-interval clip of the driver time
     next_output_time = 0.0_rk + contour_interval_dt
@@ -133,12 +156,20 @@ program cato_hip_host
     last_io_iteration = 0
     if (len_trim(contour_dir) > 0) call contour(0, master_time, last_dt)
     do while (time < input%max_time .and. iteration < input%max_iterations)
-      if (input%use_constant_delta_t .or. (iteration == 0 .and. input%initial_delta_t > 0.0_rk)) then
+      ! main.f90:140-158: a startup period of constant dt, else a constant or given first dt, else CFL
+      constant_dt = .false.
+      if (input%enable_startup_time) then
+        if (time <= input%startup_period) then
+          constant_dt = .true.
+          delta_t = input%startup_delta_t
+        end if
+      else if (input%use_constant_delta_t .or. (iteration == 0 .and. input%initial_delta_t > 0.0_rk)) then
+        constant_dt = .true.
         delta_t = input%initial_delta_t
-      else
-        delta_t = fluid%get_timestep()
       end if
+      if (.not. constant_dt) delta_t = fluid%get_timestep()
       call fluid%integrate(delta_t, error_code)
+      if (residual_hist) call history_row(fluid%iteration, fluid%time, delta_t)
       master_time = master_time + delta_t  ! master_puppeteer%time
       last_dt = delta_t
       new_time = master_time
@@ -157,6 +188,7 @@ program cato_hip_host
     end do
     if (len_trim(contour_dir) > 0 .and. iteration > last_io_iteration) call contour(iteration, master_time, last_dt)
   end if
+  if (residual_hist) close(hist_unit)
   if (error_code /= ALL_OK) write(*, '(a, i0)') 'Something went wrong in the time integration, error code ', error_code
 
   allocate(r(grid%ni, grid%nj), ru(grid%ni, grid%nj), rv(grid%ni, grid%nj), rE(grid%ni, grid%nj))
@@ -193,6 +225,29 @@ contains
     where (hr > 0.0_rk .and. hp >= 0.0_rk) hc = sqrt(cfg%gamma * hp / hr)
     hc(nh + 1:nh + grid%ni, nh + 1:nh + grid%nj) = ic
     call write_contour(trim(contour_dir), trim(input%title), it, t, dt_last, nh, x, y, hr, hu, hv, hp, hc)
+  end subroutine
+
+  !> write_residual_history (mod_fluid.f90:1207-1223) for the step just taken
+  subroutine history_row(it, t, dt)
+    integer, intent(in) :: it
+    real(rk), intent(in) :: t, dt
+    real(rk) :: norms(4)
+    call fluid%convergence(norms)
+    write(hist_unit, '(i0, ",", 6(es16.6, ","))') it, t, dt, norms
+  end subroutine
+
+  !> the rows the device-side loop collected, oldest first
+  subroutine drain_history()
+    real(c_double) :: rows(7, 256)
+    integer(c_long_long) :: n
+    integer :: r
+    do
+      call check_rc(cato_get_history(dev%ctx, rows, 256_c_long_long, n), 'cato_get_history')
+      do r = 1, int(n)
+        write(hist_unit, '(i0, ",", 6(es16.6, ","))') nint(rows(1, r)), rows(2:7, r)
+      end do
+      if (n < 256) exit
+    end do
   end subroutine
 
   subroutine check_rc(rc, what)

@@ -140,6 +140,7 @@ module cato_hip_plugins
     procedure :: get_timestep => fluid_get_timestep
     procedure :: integrate => fluid_integrate
     procedure :: set_energy_source => fluid_set_energy_source
+    procedure :: convergence => fluid_convergence
     procedure :: get_conserved => fluid_get_conserved
     procedure :: get_primitives => fluid_get_primitives
     procedure :: finalize => fluid_finalize
@@ -346,6 +347,13 @@ contains
     else
       call check(cato_set_energy_source(self%dev%ctx, c_null_ptr, 0_c_int, 0_c_int), 'fluid%set_energy_source')
     end if
+  end subroutine
+
+  !> calculate_convergence of the last step (mod_fluid.f90:1162-1205); needs cato_set_diagnostics
+  subroutine fluid_convergence(self, norms)
+    class(hip_fluid_t), intent(inout) :: self
+    real(rk), intent(out) :: norms(4)
+    call check(cato_get_convergence(self%dev%ctx, norms), 'fluid%convergence')
   end subroutine
 
   subroutine fluid_get_conserved(self, rho, rho_u, rho_v, rho_E)

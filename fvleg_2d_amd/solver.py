@@ -149,6 +149,25 @@ class Solver:
     def synchronize(self):
         N.check(self.lib.cato_synchronize(self.h), "cato_synchronize")
 
+    # ---- step bookkeeping (calculate_convergence / write_residual_history) ------
+    def set_diagnostics(self, on: bool):
+        N.check(self.lib.cato_set_diagnostics(self.h, int(on)), "cato_set_diagnostics")
+
+    def convergence(self):
+        """L2-norm convergence [rho, rho u, rho v, rho E] of the last step."""
+        out = (C.c_double * 4)()
+        N.check(self.lib.cato_get_convergence(self.h, out), "cato_get_convergence")
+        return list(out)
+
+    def history(self) -> np.ndarray:
+        """Collected residual_hist rows (iteration, time, dt, 4 norms), oldest first; drains them."""
+        n = C.c_longlong()
+        N.check(self.lib.cato_get_history(self.h, None, 0, C.byref(n)), "cato_get_history")
+        rows = np.zeros((n.value, 7))
+        if n.value:
+            N.check(self.lib.cato_get_history(self.h, _p(rows), n.value, C.byref(n)), "cato_get_history")
+        return rows[: n.value]
+
     def set_timing(self, on: bool):
         N.check(self.lib.cato_set_timing(self.h, int(on)), "cato_set_timing")
 

@@ -147,6 +147,24 @@ int cato_set_energy_source(cato_ctx *ctx, const double *q_dot, int ld, int halo)
 int cato_advance(cato_ctx *ctx, long long n);
 int cato_synchronize(cato_ctx *ctx);
 
+/* Step bookkeeping of fluid_t%integrate: calculate_convergence (mod_fluid.f90:1162-1205),
+ * the relative L2 change [rho, rho u, rho v, rho E] between the last stage and U_1
+ * (SSP-RK2/RK3, mod_fluid.f90:805,1132) or U_0 (rk(k,n), mod_fluid.f90:970), with
+ * the momenta divided by max(norm2(first), 1); and write_residual_history
+ * (mod_fluid.f90:1207-1223).  Off by default.  While on, the last stage of every step
+ * also reads the compared state and leaves per-workgroup sums that the step's
+ * finalize reduces in a fixed order (norms agree with NORM2 to ~1e-15 relative).
+ * Enabling (or disabling) clears the collected rows; the row counters restart at
+ * 0 here and at cato_set_primitive / cato_set_conserved. */
+int cato_set_diagnostics(cato_ctx *ctx, int enable);
+/* The four norms of the last step (error if diagnostics are off). */
+int cato_get_convergence(cato_ctx *ctx, double norms[4]);
+/* Moves up to max_rows collected rows (oldest first) into rows, 7 doubles each:
+ * iteration, time, dt, norms[4] -- the columns of residual_hist.csv.  cato_step and
+ * cato_run record one row per step; cato_advance records none.  rows = NULL only
+ * reports the number of rows waiting in *n_rows. */
+int cato_get_history(cato_ctx *ctx, double *rows, long long max_rows, long long *n_rows);
+
 /* Per-launch timing of the fused stage kernel, measured with HIP events on the
  * context's stream while enabled. */
 int cato_set_timing(cato_ctx *ctx, int enable);

@@ -1114,6 +1114,42 @@ static int assign_stage(const or_config *c, stage_buf *s) {
   return 0;
 }
 
+/* NORM2 of (a - b) over the interior, or of a when b is NULL, in array element order (i fastest).
+ * NORM2 is a gfortran intrinsic, not reference code: this follows the scaled sum of squares that
+ * libgfortran and the gfortran front end use (scale = 1, ssq = 0; a new maximum rescales the sum),
+ * which the reference's own norm2 calls inherit; any other order agrees to a few ulps. */
+static double norm2_interior(const or_config *c, const double *a, const double *b) {
+  const int ni = c->ni, nj = c->nj, nh = c->nh;
+  double scale = 1.0, ssq = 0.0;
+  for (int j = 1; j <= nj; ++j)
+    for (int i = 1; i <= ni; ++i) {
+      double x = b ? a[CI(i, j)] - b[CI(i, j)] : a[CI(i, j)];
+      if (x != 0.0) {
+        double ax = fabs(x);
+        if (scale < ax) {
+          double r = scale / ax;
+          ssq = 1.0 + ssq * r * r;
+          scale = ax;
+        } else {
+          double r = ax / scale;
+          ssq = ssq + r * r;
+        }
+      }
+    }
+  return scale * sqrt(ssq);
+}
+
+/* calculate_convergence (mod_fluid.f90:1162-1205): relative L2 change from first to last; the
+ * momentum norms are divided by max(norm2(first), 1). */
+static void convergence(const or_config *c, const stage_buf *first, const stage_buf *last, double *out) {
+  for (int v = 0; v < 4; ++v) {
+    double num = norm2_interior(c, last->a[v], first->a[v]);
+    double den = norm2_interior(c, first->a[v], NULL);
+    if (v == 1 || v == 2) den = den > 1.0 ? den : 1.0;
+    out[v] = num / den;
+  }
+}
+
 int or_integrate(const or_config *c, const or_geometry *g, or_fluid *f, double dt) {
   size_t n = ncell(c);
   stage_buf U, U1, U2, U3;
@@ -1136,6 +1172,7 @@ int or_integrate(const or_config *c, const or_geometry *g, or_fluid *f, double d
     time_derivative(c, g, &U1, L, f->src_E);
     combine2(c, &U, 0.5, &U1, 0.5, L, 0.5 * dt, &U2);
     err |= assign_stage(c, &U2);
+    convergence(c, &U1, &U2, f->conv); /* mod_fluid.f90:1132 */
     fin = &U2;
   } else if (c->integrator == OR_RK_SSP3) {
     /* ssp_rk_3_3 (mod_fluid.f90:773-802) */
@@ -1151,6 +1188,7 @@ int or_integrate(const or_config *c, const or_geometry *g, or_fluid *f, double d
     time_derivative(c, g, &U2, L, f->src_E);
     combine2(c, &U, 1.0 / 3.0, &U2, 2.0 / 3.0, L, (2.0 / 3.0) * dt, &U3);
     err |= assign_stage(c, &U3);
+    convergence(c, &U1, &U3, f->conv); /* mod_fluid.f90:805 */
     fin = &U3;
   } else {
     /* rk_multistage (mod_fluid.f90:889-1007) */
@@ -1176,6 +1214,7 @@ int or_integrate(const or_config *c, const or_geometry *g, or_fluid *f, double d
       err |= assign_stage(c, &U2);
       for (int k = 0; k < 8; ++k) memcpy(U1.a[k], U2.a[k], n * sizeof(double));
     }
+    convergence(c, &U, &U2, f->conv); /* mod_fluid.f90:970: U_0 against the last stage */
     fin = &U2;
   }
   /* copy-back of the conserved fields, then sanity check and derived quantities */

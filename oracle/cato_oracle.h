@@ -107,11 +107,14 @@ void or_solve(const or_config *c, const or_geometry *g, double *rho, double *u, 
               double *d_rho, double *d_rho_u, double *d_rho_v, double *d_rho_E);
 
 /* Fluid state: 4 conserved + u,v,p,cs (all halo'd); src_E: optional energy source field
- * (halo'd, NULL for none) added to d(rho E)/dt in every stage (mod_fluid.f90:500-509). */
+ * (halo'd, NULL for none) added to d(rho E)/dt in every stage (mod_fluid.f90:500-509);
+ * conv: the L2-norm convergence of the last or_integrate (rho, rho u, rho v, rho E;
+ * calculate_convergence, mod_fluid.f90:1162-1205). */
 typedef struct {
   double *rho, *rho_u, *rho_v, *rho_E;
   double *u, *v, *p, *cs;
   const double *src_E;
+  double conv[4];
 } or_fluid;
 
 /* fluid_t%initialize from primitive interior values (mod_fluid.f90:248-252, 339-363). */

@@ -54,7 +54,7 @@ class _Geometry(C.Structure):
 
 class _Fluid(C.Structure):
     _fields_ = [(n, C.POINTER(C.c_double)) for n in ("rho", "rho_u", "rho_v", "rho_E", "u", "v", "p", "cs")] + \
-        [("src_E", C.POINTER(C.c_double))]
+        [("src_E", C.POINTER(C.c_double)), ("conv", C.c_double * 4)]
 
 
 def build(force: bool = False) -> str:
@@ -239,6 +239,10 @@ class Fluid:
             self._src = np.asfortranarray(src, dtype=np.float64)
             assert self._src.shape == self.cfg.shape
             self._f.src_E = _p(self._src)
+
+    def convergence(self):
+        """L2-norm convergence [rho, rho u, rho v, rho E] of the last integrate()."""
+        return list(self._f.conv)
 
     def get_timestep(self) -> float:
         c = self.cfg.c()

@@ -34,7 +34,8 @@ def test_host_reads_deck_without_device(tmp_path):
 
     pb = sod_problem(ni=400, nj=4, flux="AUSM+", limiter="minmod", integrator="ssp_rk2", cfl=0.4, max_time=0.2)
     ini = write_deck(pb, str(tmp_path), "sod")
-    r = subprocess.run([host_or_fail(), ini, "--dry-run"], capture_output=True, text=True, timeout=60)
+    r = subprocess.run([host_or_fail(), ini, "--dry-run"], capture_output=True, text=True, timeout=60,
+                       cwd=str(tmp_path))
     assert r.returncode == 0, r.stdout + r.stderr
     out = r.stdout
     assert "ni = 400, nj = 4, n_ghost_layers = 2" in out
@@ -56,9 +57,28 @@ def test_host_rejects_unsupported_scheme(tmp_path):
     ini = write_deck(pb, str(tmp_path), "sod")
     text = open(ini).read().replace("flux_solver = 'AUSM+'", "flux_solver = 'FVLEG'")
     open(ini, "w").write(text)
-    r = subprocess.run([host_or_fail(), ini, "--dry-run"], capture_output=True, text=True, timeout=60)
+    r = subprocess.run([host_or_fail(), ini, "--dry-run"], capture_output=True, text=True, timeout=60,
+                       cwd=str(tmp_path))
     assert r.returncode != 0
     assert "unsupported flux_solver 'FVLEG'" in r.stdout
+
+
+@pytest.mark.parametrize("extra, message", [
+    ("[time]\nallow_subcycle = true\n", "allow_subcycle = true is not supported"),
+    ("[source_terms]\nenable_source_terms = true\n", "[source_terms] from the deck are not supported"),
+])
+def test_host_rejects_unsupported_deck_options(tmp_path, extra, message):
+    """Deck options whose effect the host does not reproduce stop the run instead of being ignored."""
+    from fvleg_2d_amd.deck import write_deck
+
+    pb = sod_problem(ni=40, nj=4)
+    ini = write_deck(pb, str(tmp_path), "sod")
+    with open(ini, "a") as fh:
+        fh.write("\n" + extra)
+    r = subprocess.run([host_or_fail(), ini, "--dry-run"], capture_output=True, text=True, timeout=60,
+                       cwd=str(tmp_path))
+    assert r.returncode != 0
+    assert message in r.stdout
 
 
 @pytest.mark.gpu
@@ -72,8 +92,8 @@ def test_host_sod_matches_oracle(tmp_path, oracle, device_loop):
     out = os.path.join(str(tmp_path), "final.h5")
     cmd = [host_or_fail(), ini, out] + (["--device-loop"] if device_loop else [])
     print("host:", " ".join(cmd), flush=True)
-    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
-    print(r.stdout, flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    print(r.stdout[-2000:], flush=True)
     assert r.returncode == 0, r.stdout + r.stderr
     got = read_state(out)
 
@@ -103,8 +123,10 @@ def test_host_writes_contours_on_the_reference_cadence(tmp_path):
     ini = write_deck(pb, str(tmp_path), "sod")
     out = os.path.join(str(tmp_path), "final.h5")
     cdir = os.path.join(str(tmp_path), "results")
-    r = subprocess.run([host_or_fail(), ini, out, "--contours", cdir], capture_output=True, text=True, timeout=120)
-    print(r.stdout, flush=True)
+    r = subprocess.run([host_or_fail(), ini, out, "--contours", cdir, "--no-residual-hist"], capture_output=True,
+                       text=True, timeout=120, cwd=str(tmp_path))
+    print(r.stdout[-2000:], flush=True)
+    assert not os.path.exists(os.path.join(str(tmp_path), "residual_hist.csv"))
     assert r.returncode == 0, r.stdout + r.stderr
     files = sorted(glob.glob(os.path.join(cdir, "step_*.h5")))
     assert os.path.basename(files[0]) == "step_0000000.h5"
@@ -122,3 +144,62 @@ def test_host_writes_contours_on_the_reference_cadence(tmp_path):
     assert np.array_equal(d[nh:-nh, :nh], d[nh:-nh, -2 * nh:-nh])  # periodic y
     times = [read_contour(f)["time"] for f in files]
     assert all(b > a for a, b in zip(times, times[1:]))
+
+
+def read_residual_hist(path):
+    lines = open(path).read().splitlines()
+    assert lines[0] == "iteration,time,dt,rho,rho_u,rho_v,rho_E"
+    rows = []
+    for line in lines[1:]:
+        f = line.split(",")
+        assert f[-1] == "" and len(f) == 8, line  # every value is followed by a comma (mod_fluid.f90:1217)
+        assert all(len(x) == 16 for x in f[1:7]), line  # es16.6
+        rows.append([float(x) for x in f[:7]])
+    return np.array(rows)
+
+
+@pytest.mark.gpu
+def test_host_residual_history_matches_oracle(tmp_path, oracle):
+    """residual_hist.csv (mod_fluid.f90:244-246,1207-1223) from the host-side and the device-side loop: the same
+    text, one row per step, iteration/time/dt of the oracle's run and its convergence norms to es16.6 precision."""
+    from fvleg_2d_amd.deck import write_deck
+
+    pb = sod_problem(ni=200, nj=4, flux="SLAU", limiter="MLP3", integrator="ssp_rk3", cfl=0.4, max_time=0.05)
+    ini = write_deck(pb, str(tmp_path), "sod")
+    text = {}
+    for mode in ("host", "device"):
+        d = tmp_path / mode
+        d.mkdir()
+        cmd = [host_or_fail(), ini, str(d / "final.h5")] + (["--device-loop"] if mode == "device" else [])
+        print("host:", " ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=120, cwd=str(d))
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+        text[mode] = open(d / "residual_hist.csv").read()
+    assert text["host"] == text["device"]
+    got = read_residual_hist(str(tmp_path / "host" / "residual_hist.csv"))
+
+    cfg = pb.cfg
+    nh = cfg.n_ghost
+    oc = oracle.Config(pb.ni, pb.nj, flux=cfg.flux, limiter=cfg.limiter, integrator=cfg.integrator, bcs=cfg.bcs,
+                       gamma=cfg.gamma, cfl=cfg.cfl)
+    og = oracle.Geometry(oc, pb.node_x, pb.node_y)
+    f = oracle.Fluid(oc, og, *[np.pad(a, nh) for a in (pb.rho, pb.u, pb.v, pb.p)])
+    # main.f90:100-198 with contour_interval_dt = max_time (the deck's), including the clip of the driver time
+    eps = np.finfo(float).eps
+    want, master, time, it, next_out = [], 0.0, 0.0, 0, pb.max_time
+    while time < pb.max_time:
+        dt = f.get_timestep()
+        assert f.integrate(dt) == 0
+        master = master + dt
+        it += 1
+        want.append([it, master, dt] + f.convergence())
+        new_time = master
+        if abs(time - next_out) < eps:
+            next_out = next_out + pb.max_time
+        if new_time > next_out:
+            new_time = time + abs(next_out - time)
+        time = new_time
+    want = np.array(want)
+    assert got.shape == want.shape
+    assert np.array_equal(got[:, 0], want[:, 0])
+    assert np.allclose(got[:, 1:], want[:, 1:], rtol=6e-7, atol=0)

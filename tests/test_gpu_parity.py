@@ -236,6 +236,58 @@ def test_energy_source_parity(oracle, integrator):
         assert np.array_equal(gd[k], od[k]), ("residual", k)
 
 
+@pytest.mark.parametrize("integrator", ["ssp_rk2", "ssp_rk3", "rk(2,4)"])
+def test_convergence_history_parity(oracle, integrator):
+    """calculate_convergence + write_residual_history (mod_fluid.f90:1162-1223): per-step norms within 1e-12 of
+    the oracle's NORM2 (a different summation order), rows from cato_step and from the device loop, and the
+    state still bit-identical with the diagnostics on."""
+    say("convergence", integrator)
+    ni, nj = 53, 31
+    cfg = SchemeConfig(flux="AUSMPW+", limiter="superbee", integrator=integrator,
+                       bcs=("zero_gradient", "zero_gradient", "periodic", "periodic"), cfl=0.4)
+    nh = cfg.n_ghost
+    nx, ny = uniform_nodes(ni, nj, nh, (0.0, 1.0), (0.0, 0.6))
+    prims = shock_state(ni, nj, 21)
+    oc = oracle_cfg(oracle, cfg, ni, nj)
+    og = oracle.Geometry(oc, nx, ny)
+
+    def oracle_rows(n):
+        f = oracle.Fluid(oc, og, *[pad(a, nh) for a in prims])
+        rows, t = [], 0.0
+        for k in range(n):
+            dt = f.get_timestep()
+            assert f.integrate(dt) == 0
+            t = t + dt
+            rows.append([k + 1, t, dt] + f.convergence())
+        return np.array(rows), f
+
+    nsteps = 40  # crosses one 32-step poll of the device loop
+    want, f = oracle_rows(nsteps)
+    with Solver(cfg, nx, ny) as s:
+        s.set_primitive(*prims)
+        s.set_diagnostics(True)
+        for k in range(5):
+            s.step(s.compute_dt())
+            assert np.allclose(s.convergence(), want[k, 3:], rtol=1e-12, atol=0), k
+        got = s.history()
+        assert got.shape == (5, 7)
+        assert np.array_equal(got[:, :3], want[:5, :3])
+        assert np.allclose(got[:, 3:], want[:5, 3:], rtol=1e-12, atol=0)
+        assert s.history().shape == (0, 7)  # drained
+
+        s.set_primitive(*prims)
+        it, _, err = s.run(1e30, nsteps)
+        assert (it, err) == (nsteps, 0)
+        got = s.history()
+        g = s.conserved()
+    assert got.shape == (nsteps, 7)
+    assert np.array_equal(got[:, :3], want[:, :3])
+    assert np.allclose(got[:, 3:], want[:, 3:], rtol=1e-12, atol=0)
+    assert np.all(want[:, 3:] > 0)
+    for k, name in enumerate(("rho", "rho_u", "rho_v", "rho_E")):
+        assert np.array_equal(g[k], f.interior(name)), name
+
+
 def run_both(oracle, pb, max_iterations=0):
     nh = pb.cfg.n_ghost
     oc = oracle_cfg(oracle, pb.cfg, pb.ni, pb.nj)

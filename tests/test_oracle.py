@@ -396,3 +396,57 @@ def test_energy_source_adds_s_dt_to_rho_E(oracle, integrator):
     assert np.allclose(f.interior("rho_E") - before["rho_E"], S * dt, rtol=0, atol=1e-13)
     for k in ("rho", "rho_u", "rho_v"):
         assert np.allclose(f.interior(k), before[k], rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("integrator", ["ssp_rk2", "ssp_rk3", "rk(2,4)"])
+def test_convergence_norms_closed_form(oracle, integrator):
+    """calculate_convergence (mod_fluid.f90:1162-1205) on a uniform periodic state heated by a constant source S:
+    every stage of SSP-RK2/RK3 after the first lands on U + S dt again, so the norm of (last - first stage) is
+    round-off; rk_multistage compares against U_0 (mod_fluid.f90:970), giving S dt / (rho E) exactly in rho E."""
+    n = 12
+    cfg = oracle.Config(n, n, flux="AUSMPW+", limiter="van_leer", integrator=integrator, bcs=("periodic",) * 4)
+    g = oracle.Geometry(cfg, *oracle.uniform_nodes(n, n, cfg.nh))
+    rho, u, v, p = (cfg.field() for _ in range(4))
+    rho[:], u[:], v[:], p[:] = 0.8, 0.0, 0.0, 1.1
+    f = oracle.Fluid(cfg, g, rho, u, v, p)
+    rho_E0 = f.interior("rho_E")[0, 0]
+    S, dt = 2.5, 0.004
+    f.set_energy_source(np.full(cfg.shape, S, order="F"))
+    assert f.integrate(dt) == 0
+    conv = f.convergence()
+    assert conv[:3] == [0.0, 0.0, 0.0]
+    if integrator == "rk(2,4)":
+        assert abs(conv[3] - S * dt / rho_E0) <= 1e-12 * conv[3]
+    else:
+        assert conv[3] < 1e-14
+
+
+@pytest.mark.parametrize("integrator", ["ssp_rk2", "ssp_rk3", "rk(1,3)"])
+def test_convergence_norms_against_numpy(oracle, integrator):
+    """The same norms from numpy.linalg.norm on the states themselves: U_1 = U + L(U) dt built from the oracle's
+    solve (flushes at 2.2e-16 change nothing at this tolerance), the momentum norms divided by max(norm, 1)."""
+    ni, nj = 24, 17
+    cfg = oracle.Config(ni, nj, flux="SLAU", limiter="MLP3", integrator=integrator, bcs=("zero_gradient",) * 4)
+    g = oracle.Geometry(cfg, *oracle.uniform_nodes(ni, nj, cfg.nh, (0.0, 1.0), (0.0, 0.7)))
+    rng = np.random.default_rng(3)
+    rho, u, v, p = (cfg.field() for _ in range(4))
+    rho[:] = rng.uniform(0.5, 2.0, rho.shape)
+    u[:] = rng.uniform(-0.4, 0.4, u.shape)
+    v[:] = rng.uniform(-0.01, 0.01, v.shape)  # norm2(rho v) < 1: the max(., 1) branch; rho u takes the other
+    p[:] = rng.uniform(0.5, 1.5, p.shape)
+    f = oracle.Fluid(cfg, g, rho, u, v, p)
+    U0 = [f.interior(k) for k in ("rho", "rho_u", "rho_v", "rho_E")]
+    L0 = oracle.solve(cfg, g, *[getattr(f, k).copy(order="F") for k in ("rho", "u", "v", "p")])
+    nh = cfg.nh
+    dt = 0.5 * f.get_timestep()
+    assert f.integrate(dt) == 0
+    U = [f.interior(k) for k in ("rho", "rho_u", "rho_v", "rho_E")]
+    first = U0 if integrator.startswith("rk(") else [a + b[nh:-nh, nh:-nh] * dt for a, b in zip(U0, L0)]
+    want = []
+    for k in range(4):
+        den = np.linalg.norm(first[k])
+        want.append(np.linalg.norm(U[k] - first[k]) / (max(den, 1.0) if k in (1, 2) else den))
+    got = f.convergence()
+    assert np.linalg.norm(first[2]) < 1.0 < np.linalg.norm(first[1])
+    assert min(want) > 1e-6
+    assert np.allclose(got, want, rtol=1e-12, atol=0), (got, want)
