@@ -14,6 +14,8 @@ The JSON line carries, beside the headline throughput:
   precision_form  the same workload with FP32 reconstruction/flux and FP64
                   state: throughput, roofline and its L1/Linf difference from
                   the FP64 state after the same steps
+  diagnostics     ms/step of the device loop with the per-step convergence
+                  norms and history rows off and on
   riemann_1024    four-quadrant Riemann, 1024^2, HLLC + MLP5, SSP-RK3 to
                   t = 0.25, FP64 and FP32-flux forms, errors of the latter
   batched         256 independent Riemann solves (side log-uniform in
@@ -133,6 +135,27 @@ def timed_kh(n, precision, steps, warmup, tile, profile_only=False):
     return pb, wall, tm["stage_ms"] / max(tm["stage_launches"], 1), tm["stage_launches"], state
 
 
+def diagnostics_cost(pb, steps=500):
+    """ms/step of the device-side loop (cato_run) on the same workload with the per-step convergence
+    norms and history rows (calculate_convergence / write_residual_history) off and on."""
+    from fvleg_2d_amd import Solver
+
+    out = {"steps": steps, "loop": "cato_run, host poll every 32 steps"}
+    with Solver(pb.cfg, pb.node_x, pb.node_y) as s:
+        for on in (False, True, False, True):  # second pair after warm clocks; the last of each is kept
+            s.set_diagnostics(on)
+            s.set_primitive(pb.rho, pb.u, pb.v, pb.p)
+            t0 = time.perf_counter()
+            it, _, err = s.run(1e30, steps)
+            wall = time.perf_counter() - t0
+            assert (it, err) == (steps, 0)
+            out["ms_per_step_on" if on else "ms_per_step_off"] = wall / steps * 1e3
+            if on:
+                assert len(s.history()) == steps
+    out["overhead"] = out["ms_per_step_on"] / out["ms_per_step_off"] - 1.0
+    return out
+
+
 def roofline(pb, stage_avg_ms, launches, precision, n):
     integ = pb.cfg.integrator
     cells = pb.ni * pb.nj
@@ -250,6 +273,8 @@ def main():
             "roofline": roofline(pb32, ms32, l32, "fp32_flux", args.n),
             "difference_from_fp64_state": rel_errors(state, state32),
         }
+        say("[bench] convergence diagnostics on/off (device loop)")
+        out["diagnostics"] = diagnostics_cost(pb)
         say("[bench] riemann 1024^2 HLLC/MLP5, FP64 and FP32-flux")
         out["riemann_1024"] = riemann_forms()
         from fvleg_2d_amd.benchforms import measure_batched, measure_served

@@ -17,7 +17,7 @@ CATO_OK = 0
 FLUX = {"AUSMPW+": 0, "SLAU": 1, "SLAU2": 2, "SD-SLAU": 3, "SD-SLAU2": 4,
         "Roe": 5, "S-Roe": 5, "SP-Roe": 5, "AUSM+": 6, "AUSM+-u": 6, "HLLC": 7, "M-AUSMPW+": 8}
 LIMITER = {"minmod": 0, "superbee": 1, "van_leer": 2, "vanleer": 2, "MLP3": 3, "MLP5": 4}
-BC = {"periodic": 0, "zero_gradient": 1, "symmetry": 2, "mirror": 2, "reflection": 2}
+BC = {"periodic": 0, "zero_gradient": 1, "symmetry": 2, "mirror": 2, "reflection": 2, "outflow": 3, "outlet": 3}
 INTEGRATOR = {"ssp_rk2": 0, "ssp_rk3": 1, "ssp_rk33": 1, "rk(1,3)": 2, "rk(1,4)": 3, "rk(1,5)": 4,
               "rk(2,3)": 5, "rk(2,4)": 6, "rk(2,5)": 7}
 PRECISION = {"fp64": 0, "fp32_flux": 1}
@@ -28,7 +28,7 @@ class CatoConfig(C.Structure):
         ("flux", C.c_int), ("limiter", C.c_int), ("integrator", C.c_int), ("precision", C.c_int),
         ("bc_plus_x", C.c_int), ("bc_minus_x", C.c_int), ("bc_plus_y", C.c_int), ("bc_minus_y", C.c_int),
         ("n_ghost_layers", C.c_int), ("apply_low_mach_fix", C.c_int),
-        ("gamma", C.c_double), ("cfl", C.c_double), ("tile", C.c_int),
+        ("gamma", C.c_double), ("cfl", C.c_double), ("tile", C.c_int), ("bc_pressure", C.c_double),
     ]
 
 

@@ -14,6 +14,7 @@ struct FieldGeom {
   int ni, nj, nh, pitch;
   int bc[4];
   double gamma, cfl;
+  double bc_pressure;
   const double *dxi, *dyj;
 };
 
@@ -176,63 +177,89 @@ static __global__ __launch_bounds__(1024) void k_reduce_finalize(int nb, double 
   __shared__ int se[16];
   __shared__ double ss[16][8];
   if (mode == 2 && L->done) return;
-  double nrm[4] = {0.0, 0.0, 0.0, 0.0};
-  if (cv.sums) {
-#pragma unroll
-    for (int v = 0; v < 8; ++v) {
-      double t = 0.0;
-      for (int b = threadIdx.x; b < nb; b += blockDim.x) t = t + cv.sums[(size_t)v * nb + b];
-#pragma unroll
-      for (int o = 32; o > 0; o >>= 1) t = t + __shfl_xor(t, o, 64);
-      if ((threadIdx.x & 63) == 0) ss[threadIdx.x >> 6][v] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      const int nw = (blockDim.x + 63) / 64;
-      double t[8];
-      for (int v = 0; v < 8; ++v) {
-        t[v] = ss[0][v];
-        for (int k = 1; k < nw; ++k) t[v] = t[v] + ss[k][v];
-      }
-      // norm2(last - first) / norm2(first); the momenta divide by max(norm2(first), 1)
-      for (int v = 0; v < 4; ++v) {
-        double den = sqrt(t[4 + v]);
-        if (v == 1 || v == 2) den = den > 1.0 ? den : 1.0;
-        nrm[v] = sqrt(t[v]) / den;
-        cv.norms[v] = nrm[v];
-      }
-    }
-  }
+  // One workgroup reads every partial: the loop is latency-bound, so each
+  // thread issues kU partials (and their 8 convergence sums) before using any.
+  constexpr int kU = 4;
+  const bool conv = cv.sums != nullptr;
   double m = __builtin_inf();
   int e = 0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-    double x = block_min[b];
-    m = (x < m) ? x : m;
-    e |= block_err[b];
-    block_err[b] = 0;
+  double t[8];
+#pragma unroll
+  for (int v = 0; v < 8; ++v) t[v] = 0.0;
+  for (int b0 = threadIdx.x; b0 < nb; b0 += kU * blockDim.x) {
+    double x[kU];
+    int f[kU];
+#pragma unroll
+    for (int k = 0; k < kU; ++k) {
+      const int b = b0 + k * (int)blockDim.x;
+      x[k] = b < nb ? block_min[b] : __builtin_inf();
+      f[k] = b < nb ? block_err[b] : 0;
+    }
+    if (conv) {
+      double c[kU][8];
+#pragma unroll
+      for (int k = 0; k < kU; ++k) {
+        const int b = b0 + k * (int)blockDim.x;
+#pragma unroll
+        for (int v = 0; v < 8; ++v) c[k][v] = b < nb ? cv.sums[(size_t)v * nb + b] : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < kU; ++k)
+#pragma unroll
+        for (int v = 0; v < 8; ++v) t[v] = t[v] + c[k][v];
+    }
+#pragma unroll
+    for (int k = 0; k < kU; ++k) {
+      const int b = b0 + k * (int)blockDim.x;
+      m = (x[k] < m) ? x[k] : m;
+      e |= f[k];
+      if (b < nb && f[k]) block_err[b] = 0;
+    }
   }
   m = wave_min(m);
   e = wave_or(e);
+  const int w = threadIdx.x >> 6, nw = (blockDim.x + 63) / 64;
   if ((threadIdx.x & 63) == 0) {
-    sm[threadIdx.x >> 6] = m;
-    se[threadIdx.x >> 6] = e;
+    sm[w] = m;
+    se[w] = e;
+  }
+  if (conv) {
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      double y = t[v];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) y = y + __shfl_xor(y, o, 64);
+      if ((threadIdx.x & 63) == 0) ss[w][v] = y;
+    }
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
-  int nw = (blockDim.x + 63) / 64;
-  for (int k = 1; k < nw; ++k) {
+  for (int k = 0; k < nw; ++k) {
     m = (sm[k] < m) ? sm[k] : m;
     e |= se[k];
   }
-  m = (sm[0] < m) ? sm[0] : m;
-  e |= se[0];
+  double nrm[4] = {0.0, 0.0, 0.0, 0.0};
+  if (conv) {
+    double sum[8];
+    for (int v = 0; v < 8; ++v) {
+      sum[v] = ss[0][v];
+      for (int k = 1; k < nw; ++k) sum[v] = sum[v] + ss[k][v];
+    }
+    // norm2(last - first) / norm2(first); the momenta divide by max(norm2(first), 1)
+    for (int v = 0; v < 4; ++v) {
+      double den = sqrt(sum[4 + v]);
+      if (v == 1 || v == 2) den = den > 1.0 ? den : 1.0;
+      nrm[v] = sqrt(sum[v]) / den;
+      cv.norms[v] = nrm[v];
+    }
+  }
   if (out) {
     out[0] = m;
     out[1] = (double)e;
   }
   const double step_dt = (mode == 2) ? L->dt : 0.0;
   loop_update(L, m, e, mode);
-  if (mode == 2 && cv.sums && cv.hist) {
+  if (mode == 2 && conv && cv.hist) {
     // write_residual_history (mod_fluid.f90:1207-1223) row of the step just closed
     double *row = cv.hist + (size_t)((L->iteration - 1) % cv.hist_cap) * kHistRow;
     row[0] = (double)L->iteration;
@@ -243,6 +270,20 @@ static __global__ __launch_bounds__(1024) void k_reduce_finalize(int nb, double 
 }
 
 static __global__ void k_set_scalar(double *dst, double value) { *dst = value; }
+
+// +x outflow ghost column of a primitive set: column ni+1 of every row takes
+// outflow_ghost of column ni (the stage kernel keeps it current for the states
+// it produces; this fills it for primitives that arrive from elsewhere).
+static __global__ void k_outflow_column(FieldGeom g, double *rho, double *u, double *v, double *p) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= g.nj) return;
+  size_t k = (size_t)j * g.pitch + (size_t)(g.ni - 1);
+  Prim<double> q = outflow_ghost(Prim<double>{rho[k], u[k], v[k], p[k]}, g.gamma, g.bc_pressure);
+  rho[k + 1] = q.rho;
+  u[k + 1] = q.u;
+  v[k + 1] = q.v;
+  p[k + 1] = q.p;
+}
 
 // --------------------------- per-operator kernels --------------------------
 // Halo'd column-major arrays: element (i, j), i,j in [1-nh, n+nh], at
@@ -261,10 +302,12 @@ static __global__ void k_apply_bc(FieldGeom g, int ld, double *rho, double *u, d
   int sj = map_axis(gj, g.nj, g.bc[2], g.bc[3], fv);
   size_t d = (size_t)ii + (size_t)jj * ld;
   size_t s = (size_t)(si + g.nh - 1) + (size_t)(sj + g.nh - 1) * ld;
-  rho[d] = rho[s];
-  u[d] = fu ? -u[s] : u[s];
-  v[d] = fv ? -v[s] : v[s];
-  p[d] = p[s];
+  Prim<double> q{rho[s], fu ? -u[s] : u[s], fv ? -v[s] : v[s], p[s]};
+  if (gi > g.ni && g.bc[0] == BC_OUTFLOW) q = outflow_ghost(q, g.gamma, g.bc_pressure);
+  rho[d] = q.rho;
+  u[d] = q.u;
+  v[d] = q.v;
+  p[d] = q.p;
 }
 
 // muscl_interpolation_t%interpolate_edge_values on a halo'd field q:

@@ -39,7 +39,7 @@ template <typename R, int TX, int TY> constexpr size_t batch_smem_bytes() {
 
 #define CATO_BATCH_CASE(F, L)                                                                                 \
   case (F) * LIM_COUNT + (L):                                                                                 \
-    stage_tile<F, L, R, TX, TY>(a, it.tile, dt, *reinterpret_cast<TileSmem<L, R, TX, TY> *>(smem), red);    \
+    stage_tile<F, L, R, TX, TY, true>(a, it.tile, dt, *reinterpret_cast<TileSmem<L, R, TX, TY> *>(smem), red); \
     break;
 #define CATO_BATCH_FLUX(F)   \
   CATO_BATCH_CASE(F, 0)      \

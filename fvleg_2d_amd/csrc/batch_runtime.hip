@@ -322,6 +322,7 @@ int cato_batch_set_primitive(cato_batch *b, int k, const double *rho, const doub
                      g, s->P[0][0], s->P[0][1], s->P[0][2], s->P[0][3], s->U[s->cur][0], s->U[s->cur][1],
                      s->U[s->cur][2], s->U[s->cur][3]);
   if ((rc = hip_check(hipGetLastError(), "prim_to_cons"))) return rc;
+  if ((rc = fill_outflow_column(b->stream, *s, s->P[0]))) return rc;
   s->prims_valid = true;
   return hip_check(hipStreamSynchronize(b->stream), "batch set_primitive");
 }

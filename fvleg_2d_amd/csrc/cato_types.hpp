@@ -27,7 +27,12 @@ enum LimId {
   LIM_COUNT = 5
 };
 
-enum BcId { BC_PERIODIC = CATO_BC_PERIODIC, BC_ZERO_GRADIENT = CATO_BC_ZERO_GRADIENT, BC_SYMMETRY = CATO_BC_SYMMETRY };
+enum BcId {
+  BC_PERIODIC = CATO_BC_PERIODIC,
+  BC_ZERO_GRADIENT = CATO_BC_ZERO_GRADIENT,
+  BC_SYMMETRY = CATO_BC_SYMMETRY,
+  BC_OUTFLOW = CATO_BC_OUTFLOW
+};
 
 // How a stage kernel combines the residual L(U_k) with the stored states.
 enum StageMode {
@@ -78,6 +83,7 @@ struct StageArgs {
   int bc[4];                 // +x, -x, +y, -y
   int tiles_x, tiles_y;      // tile grid
   double gamma, cfl;
+  double bc_pressure;        // outflow ambient pressure (BC_OUTFLOW at +x)
   double a, b, c;            // stage coefficients
   const double *uk[4];       // U_k at the cell (MODE_FIRST / MODE_COMBINE)
   const double *u0[4];       // U_0 (MODE_COMBINE / MODE_BASE)

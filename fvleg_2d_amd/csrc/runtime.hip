@@ -106,8 +106,13 @@ static int validate_config(const cato_config &c) {
     return fail(CATO_ERR_INVALID,
                 "The number of required ghost cell layers doesn't match the edge interpolation order");
   const int bcs[4] = {c.bc_plus_x, c.bc_minus_x, c.bc_plus_y, c.bc_minus_y};
-  for (int k = 0; k < 4; ++k)
-    if (bcs[k] < 0 || bcs[k] > 2) return fail(CATO_ERR_INVALID, "unsupported boundary condition type");
+  bool outflow = false;
+  for (int k = 0; k < 4; ++k) {
+    if (bcs[k] < 0 || bcs[k] > 3) return fail(CATO_ERR_INVALID, "unsupported boundary condition type");
+    outflow = outflow || bcs[k] == BC_OUTFLOW;
+  }
+  if (outflow && !(c.bc_pressure > 0.0))
+    return fail(CATO_ERR_INVALID, "Outlet ambient pressure is <= 0!");  // outlet_bc.f90:57-62
   if ((c.bc_plus_x == BC_PERIODIC) != (c.bc_minus_x == BC_PERIODIC) ||
       (c.bc_plus_y == BC_PERIODIC) != (c.bc_minus_y == BC_PERIODIC))
     return fail(CATO_ERR_UNSUPPORTED, "a periodic boundary must be periodic on both sides of its axis");
@@ -134,7 +139,8 @@ int Solve::init(const cato_config &c, const cato_geom &g) {
   ni = g.ni;
   nj = g.nj;
   nh = c.n_ghost_layers;
-  pitch = (ni + 15) & ~15;
+  // a +x outflow side keeps its ghost column (ni+1) in the primitive arrays
+  pitch = (ni + (c.bc_plus_x == BC_OUTFLOW ? 1 : 0) + 15) & ~15;
   rc = build_geometry(ni, nj, nh, g.node_x, g.node_y, geo);
   if (rc) return rc;
   tile = default_tile(c, ni, nj);
@@ -205,6 +211,7 @@ FieldGeom Solve::fgeom() const {
   g.bc[3] = cfg.bc_minus_y;
   g.gamma = cfg.gamma;
   g.cfl = cfg.cfl;
+  g.bc_pressure = cfg.bc_pressure;
   g.dxi = dxi;
   g.dyj = dyj;
   return g;
@@ -229,6 +236,7 @@ StageArgs Solve::stage_args(int mode, double a, double b, double c, int in_buf, 
   s.tiles_y = tiles_y;
   s.gamma = cfg.gamma;
   s.cfl = cfg.cfl;
+  s.bc_pressure = cfg.bc_pressure;
   s.a = a;
   s.b = b;
   s.c = c;
@@ -380,8 +388,17 @@ int derive_primitives(hipStream_t st, const Solve &s) {
                      U[2], U[3], P[1], P[2], P[3], cs);
   int rc = hip_check(hipGetLastError(), "derive primitives");
   if (rc) return rc;
-  return hip_check(hipMemcpyAsync(P[0], U[0], s.field_elems() * sizeof(double), hipMemcpyDeviceToDevice, st),
-                   "derive primitives");
+  rc = hip_check(hipMemcpyAsync(P[0], U[0], s.field_elems() * sizeof(double), hipMemcpyDeviceToDevice, st),
+                 "derive primitives");
+  if (rc) return rc;
+  return fill_outflow_column(st, s, P);
+}
+
+int fill_outflow_column(hipStream_t st, const Solve &s, double *const *P) {
+  if (s.cfg.bc_plus_x != BC_OUTFLOW) return CATO_OK;
+  hipLaunchKernelGGL(k_outflow_column, dim3((unsigned)((s.nj + 255) / 256)), dim3(256), 0, st, s.fgeom(), P[0], P[1],
+                     P[2], P[3]);
+  return hip_check(hipGetLastError(), "outflow ghost column");
 }
 
 // Every pointer a stage launch will dereference must be set (checked before launch).
@@ -487,6 +504,7 @@ int launch_stages(cato_ctx *c, const std::vector<StageArgs> &st) {
     if (rc) return rc;
     rc = timed(c, true, [&] { s.launcher(grid, c->stream, a); });
     if (rc) return rc;
+    if (a.mode != MODE_RESIDUAL && (rc = fill_outflow_column(c->stream, s, a.prim_out))) return rc;
   }
   return CATO_OK;
 }
@@ -630,6 +648,7 @@ int cato_set_primitive(cato_ctx *c, const double *rho, const double *u, const do
   hipLaunchKernelGGL(k_prim_to_cons, field_grid(s), dim3(256), 0, c->stream, g, P[0], P[1], P[2], P[3],
                      s.U[s.cur][0], s.U[s.cur][1], s.U[s.cur][2], s.U[s.cur][3]);
   HIPCK(hipGetLastError());
+  if ((rc = fill_outflow_column(c->stream, s, P))) return rc;
   s.prims_valid = true;
   c->diag_iter = 0;
   c->diag_time = 0.0;
@@ -1006,6 +1025,7 @@ int cato_residual(cato_ctx *c, double *rho, double *u, double *v, double *p, dou
   const double *P[4] = {rho, u, v, p};
   for (int k = 0; k < 4; ++k)
     if ((rc = upload_field(c, Pbuf[k], P[k], ld, s.nh, nullptr))) return rc;
+  if ((rc = fill_outflow_column(c->stream, s, Pbuf))) return rc;
   StageArgs a = s.stage_args(MODE_RESIDUAL, 0, 0, 0, -1, -1, (s.cur + 2) % 3, 0, -1, false, nullptr, 1.0, false);
   for (int k = 0; k < 4; ++k) a.prim[k] = Pbuf[k];
   a.block_min = nullptr;

@@ -63,6 +63,8 @@ struct Solve {
 // copy, repacked on the device through `staging` (a free device field of the
 // same solve) when the device pitch differs; other layouts use a 2D copy.
 int derive_primitives(hipStream_t st, const Solve &s);
+// Column ni+1 of a primitive set for a +x outflow side (no-op otherwise).
+int fill_outflow_column(hipStream_t st, const Solve &s, double *const *P);
 int field_up(hipStream_t st, const Solve &s, double *dev, const double *host, int ld, int halo, double *staging);
 int field_down(hipStream_t st, const Solve &s, double *host, const double *dev, int ld, int halo, double *staging);
 

@@ -259,6 +259,7 @@ int admit_into(cato_server *S, Request &R, Solve &solve, int k, size_t off) {
   hipLaunchKernelGGL(k_prim_to_cons, dim3((unsigned)((R.ni + 255) / 256), (unsigned)R.nj), dim3(256), 0, st, fg,
                      sv->P[0][0], sv->P[0][1], sv->P[0][2], sv->P[0][3], sv->U[0][0], sv->U[0][1], sv->U[0][2],
                      sv->U[0][3]);
+  if ((rc = fill_outflow_column(st, *sv, sv->P[0]))) return rc;
   sv->prims_valid = true;
   LoopState L{};
   double cd = R.contour_dt > 0 ? R.contour_dt : R.max_time;

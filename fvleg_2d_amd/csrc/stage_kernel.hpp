@@ -36,21 +36,45 @@ __device__ inline int xcd_tile(int b, int nb) {
 }
 
 // Ghost-cell resolution: the per-axis composition of the BC fills
-// (flux_solver.f90:138-185 with periodic/zero-gradient/symmetry on one image).
+// (flux_solver.f90:138-185 with periodic/zero-gradient/symmetry/outflow on one
+// image).  Every fill but the +x outflow copies a (possibly mirrored) cell, so
+// the fills compose per axis; the +x outflow has the lowest priority and is
+// applied first among equals, so its transform acts last on the mapped cell.
 __device__ inline int map_axis(int g, int n, int bc_hi, int bc_lo, bool &flip) {
   flip = false;
   if (g < 1) {
     if (bc_lo == BC_PERIODIC) return g + n;
-    if (bc_lo == BC_ZERO_GRADIENT) return 1;
+    if (bc_lo == BC_ZERO_GRADIENT || bc_lo == BC_OUTFLOW) return 1;
     flip = true;
     return 1 - g;
   }
   if (g > n) {
     if (bc_hi == BC_PERIODIC) return g - n;
-    if (bc_hi == BC_ZERO_GRADIENT) return n;
+    if (bc_hi == BC_ZERO_GRADIENT || bc_hi == BC_OUTFLOW) return n;
     flip = true;
     return 2 * n + 1 - g;
   }
+  return g;
+}
+
+// outlet_bc_t%apply at +x (outlet_bc.f90:96-123): every ghost layer takes the
+// last interior cell of its row, extrapolated along the outgoing
+// characteristic to the ambient pressure p_b when the flow there is subsonic.
+// The reference leaves the ghost untouched when rho or p_b is not > tiny; here
+// it is then a plain copy (such a state fails the step's density check anyway).
+__device__ inline Prim<double> outflow_ghost(Prim<double> d, double gamma, double p_b) {
+  const double TINY = 2.2250738585072014e-308;
+  if (!(p_b > TINY && d.rho > TINY)) return d;
+  double cs = sqrt(gamma * absd(d.p / d.rho));
+  double vel = sqrt(d.u * d.u + d.v * d.v);
+  double mach = vel / cs;
+  double nx = d.u / (vel + 1e-30), ny = d.v / (vel + 1e-30);
+  if (mach > 1.0) return d;
+  Prim<double> g;
+  g.rho = d.rho + (p_b - d.p) / (cs * cs);
+  g.u = d.u + (nx * (d.p - p_b)) / (d.rho * cs);
+  g.v = d.v + (ny * (d.p - p_b)) / (d.rho * cs);
+  g.p = p_b;
   return g;
 }
 
@@ -80,6 +104,11 @@ __device__ inline int wave_or(int v) {
 __device__ inline Prim<double> load_prim(const StageArgs &a, int gi, int gj) {
   bool fu, fv;
   int si = map_axis(gi, a.ni, a.bc[0], a.bc[1], fu);
+  // +x outflow: every ghost layer reads column ni+1 of the primitive arrays,
+  // which the producer of the state filled with outflow_ghost of column ni.
+  // outflow_ghost commutes exactly with the v negation of a symmetry fill in
+  // y, so the y-mapped corners equal the reference's fill-after-mirror.
+  if (gi > a.ni && a.bc[0] == BC_OUTFLOW) si = a.ni + 1;
   int sj = map_axis(gj, a.nj, a.bc[2], a.bc[3], fv);
   size_t off = (size_t)(sj - 1) * a.pitch + (size_t)(si - 1);
   Prim<double> pr;
@@ -127,8 +156,11 @@ struct RedSmem {
 };
 
 // One tile of one stage.  Writes the block's CFL partial and sanity flags to
-// a.block_min[tile] / a.block_err[tile] when requested.
-template <int FLUX, int LIMC, typename R, int TX, int TY>
+// a.block_min[tile] / a.block_err[tile] when requested.  OUTFLOW_COL: the tile
+// also writes the +x outflow ghost column of the new state (the batched
+// kernel); the single-solve launch leaves that to k_outflow_column, since the
+// extra epilogue slows every tile of the fused kernel by ~5%.
+template <int FLUX, int LIMC, typename R, int TX, int TY, bool OUTFLOW_COL = false>
 __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileSmem<LIMC, R, TX, TY> &sm,
                                   RedSmem &red) {
   constexpr int H = halo_of<LIMC>();
@@ -246,6 +278,11 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
     bool pow2 = (vb & 0x000FFFFFFFFFFFFFull) == 0 && ((vb >> 52) & 0x7FF) > 1 && ((vb >> 52) & 0x7FF) < 2046;
     double inv_vol = kExact ? (pow2 ? 1.0 / vol : 0.0) : rcp64(vol);
     size_t off = (size_t)(gj - 1) * a.pitch + (size_t)(gi - 1);
+    double fst[4] = {0.0, 0.0, 0.0, 0.0};  // issued before the flux balance to hide their latency
+    if (conv) {
+#pragma unroll
+      for (int v = 0; v < 4; ++v) fst[v] = a.conv_first[v][off];
+    }
     double nv[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -274,9 +311,9 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
       }
       nv[v] = r;
       if (conv) {
-        double f = a.conv_first[v][off], d = r - f;
+        double d = r - fst[v];
         csq[v] = csq[v] + d * d;
-        csq[4 + v] = csq[4 + v] + f * f;
+        csq[4 + v] = csq[4 + v] + fst[v] * fst[v];
       }
       a.out[v][off] = r;
     }
@@ -290,6 +327,13 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
     a.prim_out[1][off] = q.u;
     a.prim_out[2][off] = q.v;
     a.prim_out[3][off] = q.p;
+    if (OUTFLOW_COL && gi == ni && a.bc[0] == BC_OUTFLOW) {
+      Prim<double> g = outflow_ghost(q, a.gamma, a.bc_pressure);
+      a.prim_out[0][off + 1] = g.rho;
+      a.prim_out[1][off + 1] = g.u;
+      a.prim_out[2][off + 1] = g.v;
+      a.prim_out[3][off + 1] = g.p;
+    }
     if (q.rho != q.rho || q.u != q.u || q.v != q.v || q.p != q.p) err |= 1;
     if (a.last_stage) {
       if (q.rho < 0.0) err |= 2;

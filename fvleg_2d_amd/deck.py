@@ -35,7 +35,8 @@ def write_deck(pb, directory: str, name: str = "cato", max_iterations: int | Non
         f"apply_low_mach_fix = {'true' if cfg.apply_low_mach_fix else 'false'}", "",
         "[initial_conditions]", "read_from_file = true", f"initial_condition_file = '{h5}'", "",
         "[boundary_conditions]", f"plus_x = '{cfg.bcs[0]}'", f"minus_x = '{cfg.bcs[1]}'",
-        f"plus_y = '{cfg.bcs[2]}'", f"minus_y = '{cfg.bcs[3]}'", "",
+        f"plus_y = '{cfg.bcs[2]}'", f"minus_y = '{cfg.bcs[3]}'",
+        *([f"outflow_ambient_pressure = {cfg.bc_pressure!r}"] if "outflow" in cfg.bcs else []), "",
         "[io]", f"contour_interval_dt = {pb.contour_interval_dt or pb.max_time!r}", "",
     ]
     ini = os.path.join(directory, name + ".ini")

@@ -44,6 +44,7 @@ module cato_deck
     character(len=256) :: initial_condition_file = ''
     ! [boundary_conditions]
     character(len=32) :: plus_x = 'periodic', minus_x = 'periodic', plus_y = 'periodic', minus_y = 'periodic'
+    real(rk) :: outflow_ambient_pressure = 0.0_rk
     ! [io]
     real(rk) :: contour_interval_dt = huge(1.0_rk)
     ! [grid]
@@ -253,6 +254,7 @@ contains
       case ('boundary_conditions.minus_x'); inp%minus_x = val
       case ('boundary_conditions.plus_y'); inp%plus_y = val
       case ('boundary_conditions.minus_y'); inp%minus_y = val
+      case ('boundary_conditions.outflow_ambient_pressure'); inp%outflow_ambient_pressure = to_real(val, key)
       case ('io.contour_interval_dt'); inp%contour_interval_dt = to_real(val, key)
       case ('grid.n_ghost_layers'); inp%n_ghost_layers = int(to_real(val, key))
       end select

@@ -28,6 +28,7 @@ module cato_hip_api
     real(c_double) :: gamma = 1.4_c_double
     real(c_double) :: cfl = 0.1_c_double
     integer(c_int) :: tile = -1
+    real(c_double) :: bc_pressure = 0.0_c_double
   end type
 
   !> cato_geom: interior cells and the node coordinates incl. ghost nodes

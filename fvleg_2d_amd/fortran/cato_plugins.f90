@@ -183,9 +183,9 @@ contains
     character(len=16), parameter :: limiters(6) = [character(len=16) :: 'minmod', 'superbee', 'van_leer', &
                                                     'vanleer', 'MLP3', 'MLP5']
     integer, parameter :: limiter_ids(6) = [0, 1, 2, 2, 3, 4]
-    character(len=16), parameter :: bcs(5) = [character(len=16) :: 'periodic', 'zero_gradient', 'symmetry', &
-                                               'mirror', 'reflection']
-    integer, parameter :: bc_ids(5) = [0, 1, 2, 2, 2]
+    character(len=16), parameter :: bcs(7) = [character(len=16) :: 'periodic', 'zero_gradient', 'symmetry', &
+                                               'mirror', 'reflection', 'outflow', 'outlet']
+    integer, parameter :: bc_ids(7) = [0, 1, 2, 2, 2, 3, 3]
     character(len=16), parameter :: rks(9) = [character(len=16) :: 'ssp_rk2', 'ssp_rk3', 'ssp_rk33', 'rk(1,3)', &
                                                'rk(1,4)', 'rk(1,5)', 'rk(2,3)', 'rk(2,4)', 'rk(2,5)']
     integer, parameter :: rk_ids(9) = [0, 1, 1, 2, 3, 4, 5, 6, 7]
@@ -206,6 +206,7 @@ contains
     cfg%gamma = input%polytropic_index
     cfg%cfl = input%cfl
     cfg%tile = -1
+    cfg%bc_pressure = input%outflow_ambient_pressure
   end function
 
   ! ------------------------------------------------------------ device solve

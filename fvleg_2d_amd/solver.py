@@ -27,8 +27,9 @@ class SchemeConfig:
 
     def __init__(self, flux="AUSMPW+", limiter="minmod", integrator="ssp_rk2",
                  bcs=("periodic", "periodic", "periodic", "periodic"), gamma=1.4, cfl=0.1,
-                 apply_low_mach_fix=True, precision="fp64", tile=-1):
+                 apply_low_mach_fix=True, precision="fp64", tile=-1, bc_pressure=0.0):
         self.flux, self.limiter, self.integrator = flux, limiter, integrator
+        self.bc_pressure = float(bc_pressure)  # outflow_ambient_pressure
         self.bcs = tuple(bcs)  # (+x, -x, +y, -y)
         self.gamma, self.cfl = float(gamma), float(cfl)
         self.apply_low_mach_fix = bool(apply_low_mach_fix)
@@ -44,7 +45,7 @@ class SchemeConfig:
             return N.CatoConfig(N.FLUX[self.flux], N.LIMITER[self.limiter], N.INTEGRATOR[self.integrator],
                                 N.PRECISION[self.precision], N.BC[self.bcs[0]], N.BC[self.bcs[1]],
                                 N.BC[self.bcs[2]], N.BC[self.bcs[3]], self.n_ghost, int(self.apply_low_mach_fix),
-                                self.gamma, self.cfl, self.tile)
+                                self.gamma, self.cfl, self.tile, self.bc_pressure)
         except KeyError as e:
             raise ValueError(f"unknown scheme option {e}") from None
 

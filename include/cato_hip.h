@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define CATO_HIP_ABI_VERSION 1
+#define CATO_HIP_ABI_VERSION 2
 
 /* flux_solver names of the input deck (mod_fluid.f90:223-238) */
 enum {
@@ -59,7 +59,13 @@ enum {
   CATO_LIMITER_MLP5 = 4
 };
 /* boundary condition names (bc_factory.f90:48-64) */
-enum { CATO_BC_PERIODIC = 0, CATO_BC_ZERO_GRADIENT = 1, CATO_BC_SYMMETRY = 2 };
+enum {
+  CATO_BC_PERIODIC = 0,      /* periodic_bc.fypp (priority 2)                                  */
+  CATO_BC_ZERO_GRADIENT = 1, /* zero_gradient_bc.f90 (priority 1)                              */
+  CATO_BC_SYMMETRY = 2,      /* symmetry_bc.f90, also 'mirror' / 'reflection' (priority 1)     */
+  CATO_BC_OUTFLOW = 3        /* outlet_bc.f90, 'outflow' / 'outlet' (priority 0): at +x the    */
+                             /* characteristic extrapolation to bc_pressure, elsewhere a copy  */
+};
 /* integration_strategy names (mod_fluid.f90:424-446) */
 enum {
   CATO_RK_SSP2 = 0, /* 'ssp_rk2'             */
@@ -93,6 +99,8 @@ typedef struct cato_config {
   double gamma;           /* [physics] polytropic_index */
   double cfl;             /* [time] cfl */
   int tile;               /* -1: library choice; otherwise a tile id from cato_tile_shape */
+  double bc_pressure;     /* outflow ambient pressure ([boundary_conditions] outflow_ambient_pressure,
+                             mod_input.f90:384-390); must be > 0 when a side is CATO_BC_OUTFLOW */
 } cato_config;
 
 typedef struct cato_geom {

@@ -181,7 +181,8 @@ void or_derived_quantities(const or_config *c, const double *rho, const double *
 /* ------------------------------------------------------------------------- */
 /* Boundary conditions on one image                                           */
 /* ------------------------------------------------------------------------- */
-static int bc_priority(int t) { return t == OR_BC_PERIODIC ? 2 : 1; }
+/* periodic_bc.fypp:65, zero_gradient_bc.f90:50, symmetry_bc.f90:52, outlet_bc.f90:54 */
+static int bc_priority(int t) { return t == OR_BC_PERIODIC ? 2 : t == OR_BC_OUTFLOW ? 0 : 1; }
 
 /* periodic_bc.fypp:78-283: either x side fills both x ghost strips (rows jlo:jhi);
  * corners only when all four sides are periodic. */
@@ -246,8 +247,50 @@ static void bc_copy(const or_config *c, int loc, int mirror, int negate, double 
   }
 }
 
+/* outlet_bc.f90:96-123: at +x every ghost layer of row j (j over the whole halo'd extent) takes the
+ * last interior cell of the row, extrapolated along the outgoing characteristic to p_b when that cell
+ * is subsonic.  The reference skips a row whose rho or p_b is not > tiny (its ghost keeps an older
+ * value); this restatement copies the cell there instead, as the HIP path does. */
+static void bc_outflow_plus_x(const or_config *c, double *rho, double *u, double *v, double *p) {
+  const int ni = c->ni, nj = c->nj, nh = c->nh;
+  const double gamma = c->gamma, p_b = c->bc_pressure; /* TINY = tiny(1.0_rk) */
+  for (int i = ni + 1; i <= ni + nh; ++i)
+    for (int j = 1 - nh; j <= nj + nh; ++j) {
+      const size_t s = CI(ni, j), d = CI(i, j);
+      const double p_d = p[s], rho_0 = rho[s], u_d = u[s], v_d = v[s];
+      double gr = rho_0, gu = u_d, gv = v_d, gp = p_d;
+      if (p_b > TINY && rho_0 > TINY) {
+        double cs = sqrt(gamma * fabs(p_d / rho_0));
+        double vel = sqrt(u_d * u_d + v_d * v_d);
+        double mach = vel / cs;
+        double nx = u_d / (vel + 1e-30), ny = v_d / (vel + 1e-30);
+        if (!(mach > 1.0)) {
+          gr = rho_0 + (p_b - p_d) / (cs * cs);
+          gu = u_d + nx * (p_d - p_b) / (rho_0 * cs);
+          gv = v_d + ny * (p_d - p_b) / (rho_0 * cs);
+          gp = p_b;
+        }
+      }
+      rho[d] = gr;
+      u[d] = gu;
+      v[d] = gv;
+      p[d] = gp;
+    }
+}
+
 static void bc_apply_one(const or_config *c, int type, int loc, double *rho, double *u, double *v, double *p) {
   int xdir = (loc == 0 || loc == 1);
+  if (type == OR_BC_OUTFLOW) {
+    if (loc == 0) {
+      bc_outflow_plus_x(c, rho, u, v, p);
+    } else { /* outlet_bc.f90:124-162: the other sides copy the last interior line over the full extent */
+      bc_copy(c, loc, 0, 0, rho);
+      bc_copy(c, loc, 0, 0, u);
+      bc_copy(c, loc, 0, 0, v);
+      bc_copy(c, loc, 0, 0, p);
+    }
+    return;
+  }
   if (type == OR_BC_PERIODIC) {
     bc_periodic(c, xdir, rho);
     bc_periodic(c, xdir, u);

@@ -35,7 +35,7 @@ enum {
 enum { OR_LIM_MINMOD = 0, OR_LIM_SUPERBEE = 1, OR_LIM_VAN_LEER = 2, OR_LIM_MLP3 = 3, OR_LIM_MLP5 = 4 };
 
 /* Boundary condition ids (bc_factory.f90:48-64). */
-enum { OR_BC_PERIODIC = 0, OR_BC_ZERO_GRADIENT = 1, OR_BC_SYMMETRY = 2 };
+enum { OR_BC_PERIODIC = 0, OR_BC_ZERO_GRADIENT = 1, OR_BC_SYMMETRY = 2, OR_BC_OUTFLOW = 3 };
 
 /* Time integrators (mod_fluid.f90:424-446). */
 enum {
@@ -50,6 +50,7 @@ typedef struct {
   int apply_low_mach_fix;
   double gamma;
   double cfl;
+  double bc_pressure; /* outflow ambient pressure (outlet_bc.f90:55) */
 } or_config;
 
 /* Per-cell geometry of the halo'd block (grid_block_2d.f90:352-391). */

@@ -27,7 +27,7 @@ FLUX_IDS = {
     "Roe": 5, "S-Roe": 5, "SP-Roe": 5, "AUSM+": 6, "AUSM+-u": 6, "HLLC": 7, "M-AUSMPW+": 8,
 }
 LIMITER_IDS = {"minmod": 0, "superbee": 1, "van_leer": 2, "vanleer": 2, "MLP3": 3, "MLP5": 4}
-BC_IDS = {"periodic": 0, "zero_gradient": 1, "symmetry": 2, "mirror": 2, "reflection": 2}
+BC_IDS = {"periodic": 0, "zero_gradient": 1, "symmetry": 2, "mirror": 2, "reflection": 2, "outflow": 3, "outlet": 3}
 RK_IDS = {
     "ssp_rk2": 0, "ssp_rk3": 1, "ssp_rk33": 1,
     "rk(1,3)": 2, "rk(1,4)": 3, "rk(1,5)": 4, "rk(2,3)": 5, "rk(2,4)": 6, "rk(2,5)": 7,
@@ -40,7 +40,7 @@ class _Config(C.Structure):
         ("flux", C.c_int), ("limiter", C.c_int), ("integrator", C.c_int),
         ("bc_plus_x", C.c_int), ("bc_minus_x", C.c_int), ("bc_plus_y", C.c_int), ("bc_minus_y", C.c_int),
         ("apply_low_mach_fix", C.c_int),
-        ("gamma", C.c_double), ("cfl", C.c_double),
+        ("gamma", C.c_double), ("cfl", C.c_double), ("bc_pressure", C.c_double),
     ]
 
 
@@ -107,8 +107,9 @@ class Config:
 
     def __init__(self, ni, nj, flux="AUSMPW+", limiter="minmod", integrator="ssp_rk2",
                  bcs=("periodic", "periodic", "periodic", "periodic"), gamma=1.4, cfl=0.1,
-                 apply_low_mach_fix=True, n_ghost=None):
+                 apply_low_mach_fix=True, n_ghost=None, bc_pressure=0.0):
         self.ni, self.nj = int(ni), int(nj)
+        self.bc_pressure = float(bc_pressure)
         self.flux, self.limiter, self.integrator = flux, limiter, integrator
         self.bcs = tuple(bcs)  # (+x, -x, +y, -y)
         self.gamma, self.cfl = float(gamma), float(cfl)
@@ -122,7 +123,7 @@ class Config:
     def c(self) -> _Config:
         return _Config(self.ni, self.nj, self.nh, FLUX_IDS[self.flux], LIMITER_IDS[self.limiter],
                        RK_IDS[self.integrator], BC_IDS[self.bcs[0]], BC_IDS[self.bcs[1]], BC_IDS[self.bcs[2]],
-                       BC_IDS[self.bcs[3]], int(self.apply_low_mach_fix), self.gamma, self.cfl)
+                       BC_IDS[self.bcs[3]], int(self.apply_low_mach_fix), self.gamma, self.cfl, self.bc_pressure)
 
     def field(self):
         return np.zeros(self.shape, dtype=np.float64, order="F")

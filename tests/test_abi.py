@@ -40,7 +40,7 @@ def test_abi_version_and_tiles_without_device():
     from fvleg_2d_amd import _native
 
     lib = _native.load()
-    assert lib.cato_abi_version() == 1
+    assert lib.cato_abi_version() == 2
     n = lib.cato_num_tiles()
     assert n >= 1
     tx, ty = ctypes.c_int(), ctypes.c_int()
@@ -56,7 +56,7 @@ def test_struct_layout_matches_header():
 
     text = open(HEADER).read()
     body = re.search(r"typedef struct cato_config \{(.*?)\} cato_config;", text, re.S).group(1)
-    body = re.sub(r"/\*.*?\*/", "", body)
+    body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
     names = []
     for decl in body.split(";"):
         decl = decl.strip()

@@ -63,6 +63,22 @@ def test_host_rejects_unsupported_scheme(tmp_path):
     assert "unsupported flux_solver 'FVLEG'" in r.stdout
 
 
+def test_host_reads_outflow_deck(tmp_path):
+    """'outflow' at +x with its outflow_ambient_pressure (mod_input.f90:384-390, bc_factory.f90:53)."""
+    from fvleg_2d_amd.deck import write_deck
+
+    pb = sod_problem(ni=40, nj=4)
+    pb.cfg.bcs = ("outflow", "zero_gradient", "periodic", "periodic")
+    pb.cfg.bc_pressure = 0.25
+    ini = write_deck(pb, str(tmp_path), "sod")
+    r = subprocess.run([host_or_fail(), ini, "--dry-run"], capture_output=True, text=True, timeout=60,
+                       cwd=str(tmp_path))
+    assert r.returncode == 0, r.stdout + r.stderr
+    cfg_line = [l for l in r.stdout.splitlines() if l.startswith("config:")][0]
+    assert cfg_line.split("=")[1].split()[3:7] == ["3", "1", "0", "0"]
+    assert float([l for l in r.stdout.splitlines() if l.startswith("bc_pressure")][0].split("=")[1]) == 0.25
+
+
 @pytest.mark.parametrize("extra, message", [
     ("[time]\nallow_subcycle = true\n", "allow_subcycle = true is not supported"),
     ("[source_terms]\nenable_source_terms = true\n", "[source_terms] from the deck are not supported"),
@@ -146,6 +162,60 @@ def test_host_writes_contours_on_the_reference_cadence(tmp_path):
     assert all(b > a for a, b in zip(times, times[1:]))
 
 
+def oracle_main_loop(f, max_time, contour_dt, startup=None):
+    """main.f90:100-198 on an oracle Fluid: dt from the startup period (main.f90:141-145) or the CFL, the
+    contour-interval clip of the driver time; returns (iterations, driver time, residual rows)."""
+    eps = np.finfo(float).eps
+    rows, master, time, it, next_out = [], 0.0, 0.0, 0, 0.0 + contour_dt
+    while time < max_time:
+        if startup is not None and time <= startup[0]:
+            dt = startup[1]
+        else:
+            dt = f.get_timestep()
+        assert f.integrate(dt) == 0
+        master = master + dt
+        it += 1
+        rows.append([it, master, dt] + f.convergence())
+        new_time = master
+        if abs(time - next_out) < eps:
+            next_out = next_out + contour_dt
+        if new_time > next_out:
+            new_time = time + abs(next_out - time)
+        time = new_time
+    return it, time, rows
+
+
+@pytest.mark.gpu
+def test_host_startup_period_uses_constant_dt(tmp_path, oracle):
+    """[time] enable_startup_time (main.f90:141-145): startup_delta_t while time <= startup_period, CFL after."""
+    from fvleg_2d_amd.deck import read_state, write_deck
+
+    pb = sod_problem(ni=100, nj=4, flux="AUSMPW+", limiter="van_leer", integrator="ssp_rk2", cfl=0.4, max_time=0.06)
+    ini = write_deck(pb, str(tmp_path), "sod")
+    period, dt0 = 0.01, 3e-4
+    with open(ini, "a") as fh:
+        fh.write(f"\n[time]\nenable_startup_time = true\nstartup_period = {period!r}\nstartup_delta_t = {dt0!r}\n")
+    out = os.path.join(str(tmp_path), "final.h5")
+    r = subprocess.run([host_or_fail(), ini, out], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    got = read_state(out)
+    hist = read_residual_hist(os.path.join(str(tmp_path), "residual_hist.csv"))
+
+    cfg = pb.cfg
+    nh = cfg.n_ghost
+    oc = oracle.Config(pb.ni, pb.nj, flux=cfg.flux, limiter=cfg.limiter, integrator=cfg.integrator, bcs=cfg.bcs,
+                       gamma=cfg.gamma, cfl=cfg.cfl)
+    og = oracle.Geometry(oc, pb.node_x, pb.node_y)
+    f = oracle.Fluid(oc, og, *[np.pad(a, nh) for a in (pb.rho, pb.u, pb.v, pb.p)])
+    n, t, rows = oracle_main_loop(f, pb.max_time, pb.max_time, startup=(period, dt0))
+    dts = np.array(rows)[:, 2]
+    assert np.sum(dts == dt0) >= 30 and np.any(dts != dt0)  # both regimes ran
+    assert got["iteration"] == n and got["time"] == t
+    assert np.array_equal(hist[:, 2], [float(f"{x:.6e}") for x in dts])
+    for key, name in (("density", "rho"), ("rho_u", "rho_u"), ("rho_v", "rho_v"), ("rho_E", "rho_E")):
+        assert np.array_equal(got[key], f.interior(name)), key
+
+
 def read_residual_hist(path):
     lines = open(path).read().splitlines()
     assert lines[0] == "iteration,time,dt,rho,rho_u,rho_v,rho_E"
@@ -184,22 +254,7 @@ def test_host_residual_history_matches_oracle(tmp_path, oracle):
                        gamma=cfg.gamma, cfl=cfg.cfl)
     og = oracle.Geometry(oc, pb.node_x, pb.node_y)
     f = oracle.Fluid(oc, og, *[np.pad(a, nh) for a in (pb.rho, pb.u, pb.v, pb.p)])
-    # main.f90:100-198 with contour_interval_dt = max_time (the deck's), including the clip of the driver time
-    eps = np.finfo(float).eps
-    want, master, time, it, next_out = [], 0.0, 0.0, 0, pb.max_time
-    while time < pb.max_time:
-        dt = f.get_timestep()
-        assert f.integrate(dt) == 0
-        master = master + dt
-        it += 1
-        want.append([it, master, dt] + f.convergence())
-        new_time = master
-        if abs(time - next_out) < eps:
-            next_out = next_out + pb.max_time
-        if new_time > next_out:
-            new_time = time + abs(next_out - time)
-        time = new_time
-    want = np.array(want)
+    want = np.array(oracle_main_loop(f, pb.max_time, pb.max_time)[2])
     assert got.shape == want.shape
     assert np.array_equal(got[:, 0], want[:, 0])
     assert np.allclose(got[:, 1:], want[:, 1:], rtol=6e-7, atol=0)

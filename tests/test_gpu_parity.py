@@ -30,7 +30,8 @@ def rel_linf(a, b):
 
 def oracle_cfg(O, cfg: SchemeConfig, ni, nj):
     return O.Config(ni, nj, flux=cfg.flux, limiter=cfg.limiter, integrator=cfg.integrator, bcs=cfg.bcs,
-                    gamma=cfg.gamma, cfl=cfg.cfl, apply_low_mach_fix=cfg.apply_low_mach_fix)
+                    gamma=cfg.gamma, cfl=cfg.cfl, apply_low_mach_fix=cfg.apply_low_mach_fix,
+                    bc_pressure=cfg.bc_pressure)
 
 
 def pad(a, nh):
@@ -59,6 +60,10 @@ BC_SETS = [
     ("symmetry",) * 4,
     ("symmetry", "zero_gradient", "zero_gradient", "symmetry"),
     ("zero_gradient", "symmetry", "periodic", "periodic"),
+    ("outflow", "zero_gradient", "symmetry", "symmetry"),
+    ("outflow", "symmetry", "periodic", "periodic"),
+    ("outflow",) * 4,
+    ("symmetry", "outflow", "outflow", "zero_gradient"),
 ]
 
 
@@ -67,11 +72,13 @@ BC_SETS = [
 def test_apply_bc_parity(oracle, bcs, lim):
     say("apply_bc", bcs, lim)
     ni, nj = 13, 9
-    cfg = SchemeConfig(limiter=lim, bcs=bcs)
+    cfg = SchemeConfig(limiter=lim, bcs=bcs, bc_pressure=0.9)
     nh = cfg.n_ghost
     nx, ny = uniform_nodes(ni, nj, nh)
     rng = np.random.default_rng(1)
-    fields = [np.asfortranarray(rng.uniform(-1, 1, (ni + 2 * nh, nj + 2 * nh))) for _ in range(4)]
+    # rho, p > 0 and |u|, |v| up to 2: both branches of the +x outflow fill (subsonic and supersonic rows)
+    lo_hi = [(0.5, 2.0), (-2.0, 2.0), (-2.0, 2.0), (0.2, 2.0)]
+    fields = [np.asfortranarray(rng.uniform(a, b, (ni + 2 * nh, nj + 2 * nh))) for a, b in lo_hi]
     with Solver(cfg, nx, ny) as s:
         g = [f.copy(order="F") for f in fields]
         s.apply_bc(*g)
@@ -178,6 +185,23 @@ def test_step_parity_ssp_rk3(oracle, flux, lim):
     g, o, gp, op = run_steps(oracle, cfg, nx, ny, shock_state(ni, nj, 5), 6)
     for k in range(4):
         assert rel_linf(g[k], o[k]) <= TOL
+        assert np.array_equal(g[k], o[k]), (k, rel_linf(g[k], o[k]))
+    for k in range(5):
+        assert np.array_equal(gp[k], op[k]), ("prim", k)
+
+
+@pytest.mark.parametrize("flux, lim", [("AUSMPW+", "minmod"), ("SLAU", "MLP5"), ("M-AUSMPW+", "van_leer")])
+def test_step_parity_outflow(oracle, flux, lim):
+    """outlet_bc.f90 at +x (subsonic extrapolation to the ambient pressure) with symmetry in y, 20 steps."""
+    say("step outflow", flux, lim)
+    ni, nj = 47, 21
+    cfg = SchemeConfig(flux=flux, limiter=lim, integrator="ssp_rk3",
+                       bcs=("outflow", "zero_gradient", "symmetry", "symmetry"), cfl=0.4, bc_pressure=0.6)
+    nx, ny = uniform_nodes(ni, nj, cfg.n_ghost, (0.0, 1.0), (0.0, 0.5))
+    rho, u, v, p = shock_state(ni, nj, 8)
+    u = np.asfortranarray(u + 0.4)  # flow out through +x
+    g, o, gp, op = run_steps(oracle, cfg, nx, ny, [rho, u, v, p], 20)
+    for k in range(4):
         assert np.array_equal(g[k], o[k]), (k, rel_linf(g[k], o[k]))
     for k in range(5):
         assert np.array_equal(gp[k], op[k]), ("prim", k)

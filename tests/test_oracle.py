@@ -450,3 +450,72 @@ def test_convergence_norms_against_numpy(oracle, integrator):
     assert np.linalg.norm(first[2]) < 1.0 < np.linalg.norm(first[1])
     assert min(want) > 1e-6
     assert np.allclose(got, want, rtol=1e-12, atol=0), (got, want)
+
+
+def outflow_setup(oracle, bcs, seed=5, ni=9, nj=7):
+    cfg = oracle.Config(ni, nj, flux="SLAU", limiter="minmod", bcs=bcs, bc_pressure=0.8)
+    rng = np.random.default_rng(seed)
+    nh = cfg.nh
+    rho, u, v, p = (cfg.field() for _ in range(4))
+    sl = (slice(nh, nh + ni), slice(nh, nh + nj))
+    rho[sl] = rng.uniform(0.5, 2.0, (ni, nj))
+    u[sl] = rng.uniform(-2.0, 2.0, (ni, nj))  # some rows supersonic at +x
+    v[sl] = rng.uniform(-1.0, 1.0, (ni, nj))
+    p[sl] = rng.uniform(0.3, 1.5, (ni, nj))
+    p[nh + ni - 1, nh + 1] = 0.8  # one row exactly at the ambient pressure
+    return cfg, [rho, u, v, p]
+
+
+def test_outflow_bc_characteristics(oracle):
+    """outlet_bc.f90:96-123 at +x, checked against what the characteristic theory fixes (Blazek 2015, 8.4):
+    a supersonic row is copied; a subsonic row gets p = p_b and keeps the outgoing acoustic invariant
+    p + rho c u_n, the entropy invariant p - c^2 rho and the tangential velocity of the last interior cell
+    (c, n from that cell), the same in every ghost layer; a row at p_b is copied exactly."""
+    cfg, f = outflow_setup(oracle, ("outflow", "zero_gradient", "zero_gradient", "zero_gradient"))
+    rho, u, v, p = f
+    oracle.apply_bcs(cfg, rho, u, v, p)
+    nh, ni, nj, pb, gam = cfg.nh, cfg.ni, cfg.nj, cfg.bc_pressure, cfg.gamma
+    i_in = nh + ni - 1
+    n_sub = n_sup = 0
+    for j in range(nj + 2 * nh):
+        d = [a[i_in, j] for a in (rho, u, v, p)]
+        c = np.sqrt(gam * d[3] / d[0])
+        vel = np.hypot(d[1], d[2])
+        for g in range(1, nh + 1):
+            gh = [a[i_in + g, j] for a in (rho, u, v, p)]
+            if vel / c > 1.0 or d[3] == pb:
+                assert gh == d
+                n_sup += vel / c > 1.0
+                continue
+            n_sub += 1
+            nx, ny = d[1] / vel, d[2] / vel
+            assert gh[3] == pb
+            jp = lambda q: q[3] + d[0] * c * (q[1] * nx + q[2] * ny)
+            assert abs(jp(gh) - jp(d)) <= 1e-13 * abs(jp(d))
+            assert abs((gh[3] - c * c * gh[0]) - (d[3] - c * c * d[0])) <= 1e-13 * c * c * d[0]
+            assert abs((gh[1] * ny - gh[2] * nx) - (d[1] * ny - d[2] * nx)) <= 1e-13 * vel
+    assert n_sub > 0 and n_sup > 0
+
+
+def test_outflow_bc_priority_and_other_sides(oracle):
+    """Priority 0 (outlet_bc.f90:54): the y fills of higher priority run first, so the +x corners take the
+    extrapolation of the y-filled last column (a mirrored row under symmetry); -x, +y, -y outflow copy the
+    last interior line over the whole extent, after the others."""
+    cfg, f = outflow_setup(oracle, ("outflow", "outflow", "symmetry", "zero_gradient"))
+    ref = [a.copy(order="F") for a in f]
+    rho, u, v, p = f
+    oracle.apply_bcs(cfg, rho, u, v, p)
+    nh, ni, nj = cfg.nh, cfg.ni, cfg.nj
+    # -x: plain copy of column 1 over the whole j extent (incl. the y ghosts filled first)
+    for a in f:
+        assert np.array_equal(a[:nh, :], np.repeat(a[nh:nh + 1, :], nh, axis=0))
+    # +x corners at +y (symmetry): the extrapolation of the mirrored cell, v negated before the fill
+    one = oracle.Config(ni, nj, flux="SLAU", limiter="minmod", bcs=("outflow", "outflow", "zero_gradient",
+                        "zero_gradient"), bc_pressure=0.8)
+    m = [a.copy(order="F") for a in ref]
+    for g in range(1, nh + 1):  # symmetry at +y: row nj+g mirrors row nj+1-g, v negated
+        for k, a in enumerate(m):
+            a[:, nh + nj - 1 + g] = (-1 if k == 2 else 1) * a[:, nh + nj - g]
+    oracle.apply_bc_single(one, "outflow", "+x", *m)
+    for k in range(4):
+        assert np.array_equal(f[k][nh + ni:, nh + nj:], m[k][nh + ni:, nh + nj:]), k
