@@ -12,7 +12,7 @@ module cato_deck
   private
 
   integer, parameter :: rk = c_double
-  public :: cato_input_t, read_input_ini, read_initial_conditions, write_state, write_contour
+  public :: cato_input_t, read_input_ini, read_initial_conditions, read_restart, write_state, write_contour
 
   !> Deck entries of the hot path (names as in input.ini)
   type :: cato_input_t
@@ -29,6 +29,9 @@ module cato_deck
     logical :: allow_subcycle = .false.
     ! [source_terms]
     logical :: enable_source_terms = .false.
+    ! [restart]
+    logical :: restart_from_file = .false.
+    character(len=256) :: restart_file = ''
     ! [reference_state]
     logical :: non_dimensionalize = .true.
     real(rk) :: reference_length = 1.0_rk, reference_density = 1.0_rk, reference_velocity = 1.0_rk
@@ -161,6 +164,50 @@ module cato_deck
       type(c_ptr), value :: buf
       integer(c_int) :: r
     end function
+    function H5Lexists(loc, name, lapl) bind(C, name="H5Lexists") result(r)
+      import :: c_int, c_int64_t, c_char
+      integer(c_int64_t), value :: loc, lapl
+      character(kind=c_char), intent(in) :: name(*)
+      integer(c_int) :: r
+    end function
+    function H5Aexists_by_name(loc, obj, attr, lapl) bind(C, name="H5Aexists_by_name") result(r)
+      import :: c_int, c_int64_t, c_char
+      integer(c_int64_t), value :: loc, lapl
+      character(kind=c_char), intent(in) :: obj(*), attr(*)
+      integer(c_int) :: r
+    end function
+    function H5Aopen_by_name(loc, obj, attr, aapl, lapl) bind(C, name="H5Aopen_by_name") result(h)
+      import :: c_int64_t, c_char
+      integer(c_int64_t), value :: loc, aapl, lapl
+      character(kind=c_char), intent(in) :: obj(*), attr(*)
+      integer(c_int64_t) :: h
+    end function
+    function H5Aget_type(a) bind(C, name="H5Aget_type") result(h)
+      import :: c_int64_t
+      integer(c_int64_t), value :: a
+      integer(c_int64_t) :: h
+    end function
+    function H5Tget_size(t) bind(C, name="H5Tget_size") result(n)
+      import :: c_int64_t, c_size_t
+      integer(c_int64_t), value :: t
+      integer(c_size_t) :: n
+    end function
+    function H5Tis_variable_str(t) bind(C, name="H5Tis_variable_str") result(r)
+      import :: c_int, c_int64_t
+      integer(c_int64_t), value :: t
+      integer(c_int) :: r
+    end function
+    function H5Aread(a, mtype, buf) bind(C, name="H5Aread") result(r)
+      import :: c_int, c_int64_t, c_ptr
+      integer(c_int64_t), value :: a, mtype
+      type(c_ptr), value :: buf
+      integer(c_int) :: r
+    end function
+    function H5Aclose(a) bind(C, name="H5Aclose") result(r)
+      import :: c_int, c_int64_t
+      integer(c_int64_t), value :: a
+      integer(c_int) :: r
+    end function
   end interface
 
 contains
@@ -239,6 +286,8 @@ contains
       case ('time.startup_delta_t'); inp%startup_delta_t = to_real(val, key)
       case ('time.allow_subcycle'); inp%allow_subcycle = to_logical(val)
       case ('source_terms.enable_source_terms'); inp%enable_source_terms = to_logical(val)
+      case ('restart.restart_from_file'); inp%restart_from_file = to_logical(val)
+      case ('restart.restart_file'); inp%restart_file = val
       case ('reference_state.non_dimensionalize'); inp%non_dimensionalize = to_logical(val)
       case ('reference_state.reference_length'); inp%reference_length = to_real(val, key)
       case ('reference_state.reference_density'); inp%reference_density = to_real(val, key)
@@ -285,6 +334,104 @@ contains
     if (r < 0) error stop 'initial conditions: cannot read '//trim(name)
     r = H5Sclose(s)
     r = H5Dclose(d)
+  end subroutine
+
+  !> The 'units' attribute of a dataset ('' when absent; fixed-length strings as h5fortran writes them).
+  function units_of(f, name) result(u)
+    integer(c_int64_t), intent(in) :: f
+    character(len=*), intent(in) :: name
+    character(len=32) :: u
+    character(kind=c_char, len=256), target :: buf
+    integer(c_int64_t) :: a, ft, mt
+    integer(c_size_t) :: n
+    integer(c_int) :: r
+    u = ''
+    if (H5Aexists_by_name(f, cstr(name), cstr('units'), H5P_DEFAULT) <= 0) return
+    a = H5Aopen_by_name(f, cstr(name), cstr('units'), H5P_DEFAULT, H5P_DEFAULT)
+    ft = H5Aget_type(a)
+    if (H5Tis_variable_str(ft) > 0) error stop 'restart: variable-length units attributes are not supported'
+    n = min(H5Tget_size(ft), int(len(buf), c_size_t))
+    mt = H5Tcopy(h5t_c_s1)
+    r = H5Tset_size(mt, n)
+    buf = ' '
+    r = H5Aread(a, mt, c_loc(buf))
+    r = H5Tclose(mt)
+    r = H5Tclose(ft)
+    r = H5Aclose(a)
+    u = buf(1:min(int(n), 32))
+    if (index(u, c_null_char) > 0) u = u(1:index(u, c_null_char) - 1)
+  end function
+
+  !> One scalar (any numeric type, any shape with one element) read as a double.
+  real(rk) function read_scalar(f, name) result(x)
+    integer(c_int64_t), intent(in) :: f
+    character(len=*), intent(in) :: name
+    real(rk), target :: buf(1)
+    integer(c_int64_t) :: d
+    integer(c_int) :: r
+    d = H5Dopen2(f, cstr(name), H5P_DEFAULT)
+    if (d < 0) error stop 'restart: missing dataset '//trim(name)
+    r = H5Dread(d, h5t_native_double, H5S_ALL, H5S_ALL, H5P_DEFAULT, c_loc(buf))
+    if (r < 0) error stop 'restart: cannot read '//trim(name)
+    r = H5Dclose(d)
+    x = buf(1)
+  end function
+
+  !> [restart] restart_file: master_puppeteer.f90:106-121 (/time in 's' or 'ns', /iteration) and
+  !> fluid_t%initialize_from_hdf5 (mod_fluid.f90:273-345: density 'g/cc', velocity 'km/s' or 'cm/s',
+  !> pressure 'barye' or 'Mbar'; halo'd fields).  A file written by write_state (it holds /rho_E) is a
+  !> checkpoint of the conserved state instead: conserved = .true. and rho/u/v/p hold rho, rho u, rho v,
+  !> rho E on the interior.  Datasets without a units attribute are taken in code units.
+  subroutine read_restart(filename, time, iteration, conserved, rho, u, v, p)
+    character(len=*), intent(in) :: filename
+    real(rk), intent(out) :: time
+    integer, intent(out) :: iteration
+    logical, intent(out) :: conserved
+    real(rk), allocatable, intent(out) :: rho(:, :), u(:, :), v(:, :), p(:, :)
+    integer(c_int64_t) :: f
+    integer(c_int) :: r
+    character(len=32) :: un
+    r = H5open()
+    f = H5Fopen(cstr(filename), H5F_ACC_RDONLY, H5P_DEFAULT)
+    if (f < 0) error stop 'restart: cannot open '//trim(filename)
+    time = read_scalar(f, '/time')
+    iteration = nint(read_scalar(f, '/iteration'))
+    un = units_of(f, '/time')
+    select case (trim(un))
+    case ('ns'); time = time * 1.0e-9_rk
+    case ('s', '')
+    case default; error stop "Unknown time units in .h5 file. Acceptable units are 'ns' or 's'."
+    end select
+    conserved = H5Lexists(f, cstr('/rho_E'), H5P_DEFAULT) > 0
+    if (conserved) then
+      call read_2d(f, '/density', rho)
+      call read_2d(f, '/rho_u', u)
+      call read_2d(f, '/rho_v', v)
+      call read_2d(f, '/rho_E', p)
+    else
+      call read_2d(f, '/density', rho)
+      un = units_of(f, '/density')
+      if (len_trim(un) > 0 .and. trim(un) /= 'g/cc') &
+        error stop "Unknown density units in .h5 file. Acceptable units are 'g/cc'"
+      call read_2d(f, '/x_velocity', u)
+      call read_2d(f, '/y_velocity', v)
+      un = units_of(f, '/x_velocity')
+      select case (trim(un))
+      case ('km/s')
+        u = u * 1.0e5_rk
+        v = v * 1.0e5_rk
+      case ('cm/s', '')
+      case default; error stop "Unknown velocity units in .h5 file. Acceptable units are 'km/s' and 'cm/s'"
+      end select
+      call read_2d(f, '/pressure', p)
+      un = units_of(f, '/pressure')
+      select case (trim(un))
+      case ('Mbar'); p = p * 1.0e12_rk
+      case ('barye', '')
+      case default; error stop "Unknown pressure units in .h5 file. Acceptable units are 'barye', 'Mbar'"
+      end select
+    end if
+    r = H5Fclose(f)
   end subroutine
 
   !> The deck: node coordinates and cell primitives incl. ghost cells (grid_block_2d.f90:194-265).

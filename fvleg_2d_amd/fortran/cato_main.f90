@@ -27,7 +27,9 @@ program cato_hip_host
   real(rk), allocatable :: x(:, :), y(:, :), rho(:, :), u(:, :), v(:, :), p(:, :)
   real(rk), allocatable :: r(:, :), ru(:, :), rv(:, :), rE(:, :), cs(:, :)
   character(len=512) :: ini_file, out_file, arg
-  logical :: device_loop, dry_run, residual_hist, constant_dt
+  logical :: device_loop, dry_run, residual_hist, constant_dt, from_checkpoint
+  real(rk) :: restart_time
+  integer :: restart_iteration
   integer :: hist_unit
   character(len=512) :: contour_dir
   type(hip_boundary_conditions_t) :: bcs
@@ -114,7 +116,31 @@ program cato_hip_host
     stop
   end if
   call dev%create(cfg, grid)
-  call fluid%initialize(dev, rho, u, v, p)
+  restart_time = 0.0_rk
+  restart_iteration = 0
+  if (input%restart_from_file) then
+    ! main.f90:100-104 + master_puppeteer.f90:106-121: state, time and iteration from the restart file
+    if (device_loop) then
+      write(*, '(a)') 'Error: --device-loop starts at time 0; a restart needs the host-side loop'
+      error stop 1
+    end if
+    call read_restart(trim(input%restart_file), restart_time, restart_iteration, from_checkpoint, rho, u, v, p)
+    if (from_checkpoint) then
+      call fluid%initialize_conserved(dev, rho, u, v, p, restart_time, restart_iteration)
+    else
+      if (size(rho, 1) /= grid%ni + 2 * nh .or. size(rho, 2) /= grid%nj + 2 * nh) then
+        write(*, '(a)') 'Error: the restart file does not match the grid of the deck'
+        error stop 1
+      end if
+      call fluid%initialize(dev, rho, u, v, p)
+      fluid%time = restart_time
+      fluid%iteration = restart_iteration
+    end if
+    write(*, '(a, es22.15, a, i0)') 'Restart from '//trim(input%restart_file)//': time = ', restart_time, &
+      ', iteration ', restart_iteration
+  else
+    call fluid%initialize(dev, rho, u, v, p)
+  end if
   bcs%dev => dev
   bcs%name = 'deck boundary conditions'
   if (len_trim(contour_dir) > 0) then
@@ -150,12 +176,15 @@ program cato_hip_host
     if (residual_hist) call drain_history()
   else
     ! main.f90:100-198: contour at iteration 0, then the loop with the contour-interval clip of the driver time
-    next_output_time = 0.0_rk + contour_interval_dt
-    master_time = 0.0_rk
+    ! main.f90:100-107: a restart resumes the driver at the file's time and iteration, with no contour
+    time = restart_time
+    iteration = restart_iteration
+    next_output_time = time + contour_interval_dt
+    master_time = restart_time
     error_code = ALL_OK
     last_dt = 0.0_rk
-    last_io_iteration = 0
-    if (len_trim(contour_dir) > 0) call contour(0, master_time, last_dt)
+    last_io_iteration = iteration
+    if (len_trim(contour_dir) > 0 .and. .not. input%restart_from_file) call contour(0, master_time, last_dt)
     do while (time < input%max_time .and. iteration < input%max_iterations)
       ! main.f90:140-158: a startup period of constant dt, else a constant or given first dt, else CFL
       constant_dt = .false.

@@ -137,6 +137,7 @@ module cato_hip_plugins
     integer :: iteration = 0
   contains
     procedure :: initialize => fluid_initialize
+    procedure :: initialize_conserved => fluid_initialize_conserved
     procedure :: get_timestep => fluid_get_timestep
     procedure :: integrate => fluid_integrate
     procedure :: set_energy_source => fluid_set_energy_source
@@ -316,6 +317,25 @@ contains
                'fluid%initialize')
     self%time = 0.0_rk
     self%iteration = 0
+  end subroutine
+
+  !> A checkpoint of the conserved state (interior arrays): the continuation of the run that wrote it,
+  !> bit for bit, since the primitives are recovered exactly as every stage recovers them
+  subroutine fluid_initialize_conserved(self, dev, rho, rho_u, rho_v, rho_E, time, iteration)
+    class(hip_fluid_t), intent(inout) :: self
+    type(hip_solve_t), target, intent(inout) :: dev
+    real(rk), intent(in) :: rho(:, :), rho_u(:, :), rho_v(:, :), rho_E(:, :)
+    real(rk), intent(in) :: time
+    integer, intent(in) :: iteration
+    self%dev => dev
+    if (size(rho, 1) /= dev%ni .or. size(rho, 2) /= dev%nj) then
+      write(*, '(a)') 'Error: the checkpoint does not match the grid of the deck'
+      error stop 1
+    end if
+    call check(cato_set_conserved(dev%ctx, rho, rho_u, rho_v, rho_E, int(size(rho, 1), c_int), 0_c_int), &
+               'fluid%initialize_conserved')
+    self%time = time
+    self%iteration = iteration
   end subroutine
 
   !> fluid_t%get_timestep (mod_fluid.f90:518-562)

@@ -258,3 +258,82 @@ def test_host_residual_history_matches_oracle(tmp_path, oracle):
     assert got.shape == want.shape
     assert np.array_equal(got[:, 0], want[:, 0])
     assert np.allclose(got[:, 1:], want[:, 1:], rtol=6e-7, atol=0)
+
+
+def run_host(args, cwd):
+    r = subprocess.run([host_or_fail()] + args, capture_output=True, text=True, timeout=120, cwd=cwd)
+    print(r.stdout[-1500:], flush=True)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    return r.stdout
+
+
+@pytest.mark.gpu
+def test_host_restart_from_checkpoint_is_bit_exact(tmp_path):
+    """[restart] (main.f90:100-107, master_puppeteer.f90:106-121) from the conserved checkpoint the host writes:
+    40 steps, then a restart to t_end, lands on exactly the state, time and iteration of the uninterrupted run."""
+    from fvleg_2d_amd.deck import read_state, write_deck
+
+    pb = sod_problem(ni=200, nj=4, flux="SLAU", limiter="minmod", integrator="ssp_rk3", cfl=0.4, max_time=0.1)
+    d = str(tmp_path)
+    full = write_deck(pb, d, "full")
+    run_host([full, os.path.join(d, "a.h5")], d)
+    part = write_deck(pb, d, "part", max_iterations=40)
+    run_host([part, os.path.join(d, "b1.h5")], d)
+    rest = write_deck(pb, d, "rest")
+    with open(rest, "a") as fh:
+        fh.write(f"\n[restart]\nrestart_from_file = true\nrestart_file = '{os.path.join(d, 'b1.h5')}'\n")
+    out = run_host([rest, os.path.join(d, "b2.h5")], d)
+    assert "Restart from" in out and "iteration 40" in out
+    a, b1, b2 = (read_state(os.path.join(d, n)) for n in ("a.h5", "b1.h5", "b2.h5"))
+    assert b1["iteration"] == 40 and a["iteration"] > 40
+    # same steps; the driver time differs by the reference's own rule: a restart sets next_output_time to
+    # time + contour_interval_dt (main.f90:101), so the last step's clip to t_end (main.f90:188-194) is gone
+    assert b2["iteration"] == a["iteration"]
+    assert a["time"] == pb.max_time and b2["time"] > pb.max_time
+    for key in ("density", "rho_u", "rho_v", "rho_E", "x_velocity", "y_velocity", "pressure"):
+        assert np.array_equal(b2[key], a[key]), key
+
+
+@pytest.mark.gpu
+def test_host_restart_from_contour_matches_oracle(tmp_path, oracle):
+    """A restart from a contour file (halo'd primitives, mod_fluid.f90:273-345): the run continues from that
+    state, time and iteration like the oracle initialised from the same primitives."""
+    import glob
+
+    from fvleg_2d_amd.deck import read_contour, read_state, write_deck
+
+    pb = sod_problem(ni=100, nj=4, flux="AUSM+", limiter="minmod", integrator="ssp_rk2", cfl=0.4, max_time=0.2)
+    pb.contour_interval_dt = 0.05
+    d = str(tmp_path)
+    ini = write_deck(pb, d, "sod")
+    run_host([ini, os.path.join(d, "a.h5"), "--contours", os.path.join(d, "results"), "--no-residual-hist"], d)
+    files = sorted(glob.glob(os.path.join(d, "results", "step_*.h5")))
+    mid = files[2]
+    c = read_contour(mid)
+    with open(ini, "a") as fh:
+        fh.write(f"\n[restart]\nrestart_from_file = true\nrestart_file = '{mid}'\n")
+    run_host([ini, os.path.join(d, "b.h5")], d)
+    got = read_state(os.path.join(d, "b.h5"))
+
+    cfg = pb.cfg
+    oc = oracle.Config(pb.ni, pb.nj, flux=cfg.flux, limiter=cfg.limiter, integrator=cfg.integrator, bcs=cfg.bcs,
+                       gamma=cfg.gamma, cfl=cfg.cfl)
+    og = oracle.Geometry(oc, pb.node_x, pb.node_y)
+    f = oracle.Fluid(oc, og, *[np.asfortranarray(c[k]) for k in ("density", "x_velocity", "y_velocity", "pressure")])
+    eps = np.finfo(float).eps
+    time = master = c["time"]
+    it, next_out = c["iteration"], c["time"] + pb.contour_interval_dt
+    while time < pb.max_time:  # main.f90:138-198 resumed at the contour's time (main.f90:100-104)
+        dt = f.get_timestep()
+        assert f.integrate(dt) == 0
+        master = master + dt
+        new_time = master
+        if abs(time - next_out) < eps:
+            next_out = next_out + pb.contour_interval_dt
+        if new_time > next_out:
+            new_time = time + abs(next_out - time)
+        time = new_time
+        it += 1
+    assert got["iteration"] == it and got["time"] == time
+    for key, name in (("density", "rho"), ("rho_u", "rho_u"), ("rho_v", "rho_v"), ("rho_E", "rho_E")):
+        assert np.array_equal(got[key], f.interior(name)), key
