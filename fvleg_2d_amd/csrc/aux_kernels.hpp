@@ -271,6 +271,24 @@ static __global__ __launch_bounds__(1024) void k_reduce_finalize(int nb, double 
 
 static __global__ void k_set_scalar(double *dst, double value) { *dst = value; }
 
+// Primitives of a stored state as the precision form's stages recover them
+// (cons_to_prim_fast), for the host-facing reads of P[0] in that form.
+static __global__ void k_prims_fast(FieldGeom g, const double *__restrict__ U0, const double *__restrict__ U1,
+                                    const double *__restrict__ U2, const double *__restrict__ U3,
+                                    double *__restrict__ P0, double *__restrict__ P1, double *__restrict__ P2,
+                                    double *__restrict__ P3) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int j = blockIdx.y;
+  if (i >= g.ni) return;
+  size_t k = (size_t)j * g.pitch + i;
+  double cs;
+  Prim<double> q = cons_to_prim_fast(U0[k], U1[k], U2[k], U3[k], g.gamma, g.gamma - 1.0, 0, &cs);
+  P0[k] = q.rho;
+  P1[k] = q.u;
+  P2[k] = q.v;
+  P3[k] = q.p;
+}
+
 // +x outflow ghost column of a primitive set: column ni+1 of every row takes
 // outflow_ghost of column ni (the stage kernel keeps it current for the states
 // it produces; this fills it for primitives that arrive from elsewhere).

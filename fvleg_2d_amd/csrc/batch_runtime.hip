@@ -211,6 +211,7 @@ int BatchEngine::run(const double *max_time, const double *contour_dt, long long
   // first dt of every solve
   for (int k = 0; k < n; ++k) {
     Solve &sv = solves[k];
+    if ((rc = refresh_prims(stream, sv))) return rc;  // a precision-form batch that already ran
     FieldGeom g = sv.fgeom();
     int nb = sv.tiles_x * sv.tiles_y;
     const double *a[4];
@@ -223,7 +224,10 @@ int BatchEngine::run(const double *max_time, const double *contour_dt, long long
   }
   // first step eagerly (its first stage reads the caller's primitives)
   if ((rc = launch_step(true))) return rc;
-  for (auto &sv : solves) sv.prims_valid = false;
+  for (auto &sv : solves) {
+    sv.prims_valid = false;
+    sv.prims_stale = sv.cfg.precision != CATO_PRECISION_FP64;  // no primitives stored by those stages
+  }
   if (use_graph && !exec) {
     BCK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     rc = launch_step(false);
@@ -324,6 +328,7 @@ int cato_batch_set_primitive(cato_batch *b, int k, const double *rho, const doub
   if ((rc = hip_check(hipGetLastError(), "prim_to_cons"))) return rc;
   if ((rc = fill_outflow_column(b->stream, *s, s->P[0]))) return rc;
   s->prims_valid = true;
+  s->prims_stale = false;
   return hip_check(hipStreamSynchronize(b->stream), "batch set_primitive");
 }
 

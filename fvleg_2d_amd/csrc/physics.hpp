@@ -38,9 +38,11 @@ template <> struct Lim<float> {
 // Arithmetic type of the FP32-flux precision form: IEEE binary32 except that
 // division and square root use the hardware reciprocal and square root
 // (v_rcp_f32 plus one Newton correction, v_sqrt_f32: about 1 ulp) instead of
-// the correctly rounded sequences, whose scale/fixup steps dominate the cost
-// of the reconstruction and flux code.  Only explicit conversions exist, so
-// an unintended double or int operand fails to compile.
+// the correctly rounded sequences, whose scale steps dominate the cost of the
+// reconstruction and flux code, and that a product feeding a sum may be fused
+// into one FMA (the form's results are judged by their error against the
+// FP64 path, not bit for bit).  Only explicit conversions exist, so an
+// unintended double or int operand fails to compile.
 struct f32x {
   float v;
   f32x() = default;
@@ -50,16 +52,26 @@ struct f32x {
   __host__ __device__ constexpr explicit operator double() const { return (double)v; }
   __host__ __device__ constexpr explicit operator float() const { return v; }
 };
-__device__ inline f32x operator+(f32x a, f32x b) { return f32x(a.v + b.v); }
-__device__ inline f32x operator-(f32x a, f32x b) { return f32x(a.v - b.v); }
-__device__ inline f32x operator*(f32x a, f32x b) { return f32x(a.v * b.v); }
+__device__ inline f32x operator+(f32x a, f32x b) {
+#pragma clang fp contract(fast)
+  return f32x(a.v + b.v);
+}
+__device__ inline f32x operator-(f32x a, f32x b) {
+#pragma clang fp contract(fast)
+  return f32x(a.v - b.v);
+}
+__device__ inline f32x operator*(f32x a, f32x b) {
+#pragma clang fp contract(fast)
+  return f32x(a.v * b.v);
+}
 __device__ inline f32x operator-(f32x a) { return f32x(-a.v); }
 __device__ inline f32x operator/(f32x a, f32x b) {
   const float r = __builtin_amdgcn_rcpf(b.v);
   const float q = a.v * r;
   const float e = __builtin_fmaf(-b.v, q, a.v);
-  // e is NaN only when b is 0 or inf (0 * inf): then q is already the IEEE quotient
-  return f32x((e == e) ? __builtin_fmaf(e, r, q) : q);
+  // v_div_fixup_f32 passes the corrected quotient through (with the sign of
+  // a/b) and substitutes the IEEE result when a or b is 0, inf or NaN
+  return f32x(__builtin_amdgcn_div_fixupf(__builtin_fmaf(e, r, q), b.v, a.v));
 }
 __device__ inline bool operator<(f32x a, f32x b) { return a.v < b.v; }
 __device__ inline bool operator>(f32x a, f32x b) { return a.v > b.v; }

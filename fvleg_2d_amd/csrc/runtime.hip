@@ -394,6 +394,18 @@ int derive_primitives(hipStream_t st, const Solve &s) {
   return fill_outflow_column(st, s, P);
 }
 
+int refresh_prims(hipStream_t st, Solve &s) {
+  if (!s.prims_stale) return CATO_OK;
+  double *const *U = s.U[s.cur];
+  double *const *P = s.P[0];
+  hipLaunchKernelGGL(k_prims_fast, dim3((unsigned)((s.ni + 255) / 256), (unsigned)s.nj), dim3(256), 0, st, s.fgeom(),
+                     U[0], U[1], U[2], U[3], P[0], P[1], P[2], P[3]);
+  int rc = hip_check(hipGetLastError(), "recover primitives");
+  if (rc) return rc;
+  s.prims_stale = false;
+  return fill_outflow_column(st, s, P);
+}
+
 int fill_outflow_column(hipStream_t st, const Solve &s, double *const *P) {
   if (s.cfg.bc_plus_x != BC_OUTFLOW) return CATO_OK;
   hipLaunchKernelGGL(k_outflow_column, dim3((unsigned)((s.nj + 255) / 256)), dim3(256), 0, st, s.fgeom(), P[0], P[1],
@@ -504,7 +516,12 @@ int launch_stages(cato_ctx *c, const std::vector<StageArgs> &st) {
     if (rc) return rc;
     rc = timed(c, true, [&] { s.launcher(grid, c->stream, a); });
     if (rc) return rc;
-    if (a.mode != MODE_RESIDUAL && (rc = fill_outflow_column(c->stream, s, a.prim_out))) return rc;
+    if (a.mode == MODE_RESIDUAL) continue;
+    if (s.cfg.precision == CATO_PRECISION_FP64) {
+      if ((rc = fill_outflow_column(c->stream, s, a.prim_out))) return rc;
+    } else {
+      s.prims_stale = true;  // the precision form's stages store no primitives
+    }
   }
   return CATO_OK;
 }
@@ -526,7 +543,9 @@ int launch_cfl(cato_ctx *c, int loop_mode) {
   FieldGeom g = s.fgeom();
   int nb = s.tiles_x * s.tiles_y;
   // P[0] holds the primitives of U^n (the caller's after cato_set_primitive)
-  int rc = timed(c, false, [&] {
+  int rc = refresh_prims(c->stream, s);
+  if (rc) return rc;
+  rc = timed(c, false, [&] {
     hipLaunchKernelGGL(k_cfl_partial, dim3((unsigned)nb), dim3(kThreads), 0, c->stream, g, 1, s.P[0][0], s.P[0][1],
                        s.P[0][2], s.P[0][3], s.block_min, s.block_err);
   });
@@ -650,6 +669,7 @@ int cato_set_primitive(cato_ctx *c, const double *rho, const double *u, const do
   HIPCK(hipGetLastError());
   if ((rc = fill_outflow_column(c->stream, s, P))) return rc;
   s.prims_valid = true;
+  s.prims_stale = false;
   c->diag_iter = 0;
   c->diag_time = 0.0;
   return hip_check(hipStreamSynchronize(c->stream), "set_primitive");
@@ -667,6 +687,7 @@ int cato_set_conserved(cato_ctx *c, const double *r, const double *ru, const dou
     return rc;
   if ((rc = derive_primitives(c->stream, s))) return rc;
   s.prims_valid = false;
+  s.prims_stale = false;
   c->diag_iter = 0;
   c->diag_time = 0.0;
   return hip_check(hipStreamSynchronize(c->stream), "set_conserved");
@@ -709,6 +730,7 @@ int cato_get_primitive(cato_ctx *c, double *rho, double *u, double *v, double *p
   if (rc) return rc;
   Solve &s = c->s;
   FieldGeom g = s.fgeom();
+  if ((rc = refresh_prims(c->stream, s))) return rc;
   // P[0]: primitives of U^n; scratch: the two non-current state buffers
   double **P = s.P[0];
   double **T = s.U[(s.cur + 1) % 3];

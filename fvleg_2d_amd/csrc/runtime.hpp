@@ -45,6 +45,7 @@ struct Solve {
   static constexpr int kHistCap = 64;
   int cur = 0;             // which U[] holds U^n
   bool prims_valid = false; // P[0] holds the caller's primitives (not yet checked by a stage)
+  bool prims_stale = false; // precision form: stages left P[0] behind U[cur] (they store no primitives)
 
   int init(const cato_config &c, const cato_geom &g);
   size_t field_elems() const;
@@ -65,6 +66,8 @@ struct Solve {
 int derive_primitives(hipStream_t st, const Solve &s);
 // Column ni+1 of a primitive set for a +x outflow side (no-op otherwise).
 int fill_outflow_column(hipStream_t st, const Solve &s, double *const *P);
+// Precision form: P[0] from U[cur] when stages have run since it was last set.
+int refresh_prims(hipStream_t st, Solve &s);
 int field_up(hipStream_t st, const Solve &s, double *dev, const double *host, int ld, int halo, double *staging);
 int field_down(hipStream_t st, const Solve &s, double *host, const double *dev, int ld, int halo, double *staging);
 
