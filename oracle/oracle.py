@@ -97,6 +97,19 @@ def lib():
     return _lib
 
 
+def use_openmp() -> bool:
+    """Switch this process to the OpenMP build (same results, rows in parallel) for large grids."""
+    global _lib, LIB_PATH
+    omp = os.path.join(HERE, "liboracle_omp.so")
+    if LIB_PATH == omp:
+        return True
+    if not os.path.exists(omp):
+        return False
+    LIB_PATH, _lib = omp, None
+    lib()
+    return True
+
+
 def _p(a: np.ndarray):
     assert a.dtype == np.float64 and (a.flags.f_contiguous or a.ndim == 1)
     return a.ctypes.data_as(C.POINTER(C.c_double))

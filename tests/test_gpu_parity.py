@@ -312,6 +312,21 @@ def test_convergence_history_parity(oracle, integrator):
         assert np.array_equal(g[k], f.interior(name)), name
 
 
+@pytest.mark.parametrize("case", ["kh2048", "riemann1024"])
+def test_full_size_workloads_match_oracle(oracle, case):
+    """The bench workloads at their full sizes: Kelvin-Helmholtz 2048^2 (SLAU, van Leer, SSP-RK3) for 3 steps
+    and the four-quadrant Riemann problem 1024^2 (HLLC, MLP5, SSP-RK3) for 5 steps, bit-exact (OpenMP oracle)."""
+    say("full size", case)
+    oracle.use_openmp()
+    pb = kelvin_helmholtz(n=2048) if case == "kh2048" else riemann2d(n=1024, seed=1, flux="HLLC", limiter="MLP5")
+    g, o, gp, op = run_steps(oracle, pb.cfg, pb.node_x, pb.node_y, [pb.rho, pb.u, pb.v, pb.p],
+                             3 if case == "kh2048" else 5)
+    for k in range(4):
+        assert np.array_equal(g[k], o[k]), (k, rel_linf(g[k], o[k]))
+    for k in range(5):
+        assert np.array_equal(gp[k], op[k]), ("prim", k)
+
+
 def run_both(oracle, pb, max_iterations=0):
     nh = pb.cfg.n_ghost
     oc = oracle_cfg(oracle, pb.cfg, pb.ni, pb.nj)
