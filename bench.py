@@ -14,6 +14,8 @@ The JSON line carries, beside the headline throughput:
   precision_form  the same workload with FP32 reconstruction/flux and FP64
                   state: throughput, roofline and its L1/Linf difference from
                   the FP64 state after the same steps
+  shard_512x1024  the FP64 stage at one GPU's share of the headline grid split
+                  4x2 over 8 GPUs (the shape a sharded solve would run per GPU)
   diagnostics     ms/step of the device loop with the per-step convergence
                   norms and history rows off and on
   riemann_1024    four-quadrant Riemann, 1024^2, HLLC + MLP5, SSP-RK3 to
@@ -102,12 +104,12 @@ def rel_errors(ref, got):
     return out
 
 
-def timed_kh(n, precision, steps, warmup, tile, profile_only=False):
+def timed_kh(n, precision, steps, warmup, tile, profile_only=False, nj=None):
     """Advance the KH workload; returns (wall seconds, stage-kernel mean ms, launches, final state)."""
     from fvleg_2d_amd import Solver
     from fvleg_2d_amd.problems import kelvin_helmholtz
 
-    pb = kelvin_helmholtz(n=n, precision=precision)
+    pb = kelvin_helmholtz(n=n, precision=precision, nj=nj)
     tuned = None
     if tile < 0 and not profile_only:
         from fvleg_2d_amd.autotune import tune
@@ -272,6 +274,19 @@ def main():
             "autotune": pb32.meta.get("autotune"),
             "roofline": roofline(pb32, ms32, l32, "fp32_flux", args.n),
             "difference_from_fp64_state": rel_errors(state, state32),
+        }
+        say("[bench] one GPU's share of the 2048^2 solve split over 8 GPUs: 512x1024")
+        pbs, walls, mss, ls, _ = timed_kh(512, "fp64", 1000, args.warmup, pb.cfg.tile, nj=1024)
+        out["shard_512x1024"] = {
+            "value": pbs.ni * pbs.nj * 1000 / walls,
+            "unit": "cell-updates/s",
+            "steps": 1000,
+            "ms_per_step": walls / 1000 * 1e3,
+            "tile": pbs.cfg.tile,
+            "stage_kernel_avg_ms": mss,
+            "roofline_frac": (pbs.ni * pbs.nj * STEP_BYTES_PER_CELL[pbs.cfg.integrator] / 3) / (mss * 1e-3) / 8e12,
+            "note": "the FP64 stage at the per-GPU shape of a 4x2 decomposition of the headline grid "
+                    "(periodic on its own; no halo exchange)",
         }
         say("[bench] convergence diagnostics on/off (device loop)")
         out["diagnostics"] = diagnostics_cost(pb)

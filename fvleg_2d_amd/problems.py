@@ -101,12 +101,14 @@ def riemann2d(n=1024, seed=1, flux="HLLC", limiter="MLP5", integrator="ssp_rk3",
 
 
 def kelvin_helmholtz(n=2048, seed=7, flux="SLAU", limiter="van_leer", integrator="ssp_rk3", cfl=0.5, steps=2000,
-                     precision="fp64") -> Problem:
+                     precision="fp64", nj=None) -> Problem:
     """Periodic shear layer: rho=2, u=0.5 for |y-0.5|<0.25, rho=1, u=-0.5 elsewhere, p=2.5;
-    v = 0.01 * sum of 8 random-phase sine modes in x plus 1e-3 white noise."""
+    v = 0.01 * sum of 8 random-phase sine modes in x plus 1e-3 white noise.  nj (default n) rows of
+    the same square cells: the domain is [0,1] x [0, nj/n]."""
+    nj = n if nj is None else nj
     cfg = SchemeConfig(flux=flux, limiter=limiter, integrator=integrator, bcs=("periodic",) * 4, gamma=1.4,
                        cfl=cfl, precision=precision)
-    nx, ny = uniform_nodes(n, n, cfg.n_ghost)
+    nx, ny = uniform_nodes(n, nj, cfg.n_ghost, (0.0, 1.0), (0.0, nj / n))
     xc, yc = cell_centers(nx, ny, cfg.n_ghost)
     rng = np.random.default_rng(seed)
     inner = np.abs(yc - 0.5) < 0.25
@@ -118,7 +120,7 @@ def kelvin_helmholtz(n=2048, seed=7, flux="SLAU", limiter="van_leer", integrator
         v += np.sin(2.0 * math.pi * (m + 1) * xc + phases[m])
     v = 0.01 * v + 1e-3 * rng.uniform(-1.0, 1.0, xc.shape)
     p = np.full_like(xc, 2.5)
-    return Problem(f"kelvin_helmholtz_{n}x{n}", cfg, nx, ny, _f(rho), _f(u), _f(v), _f(p), max_time=1e300,
+    return Problem(f"kelvin_helmholtz_{n}x{nj}", cfg, nx, ny, _f(rho), _f(u), _f(v), _f(p), max_time=1e300,
                    max_iterations=steps, contour_interval_dt=1e300, meta={"seed": seed})
 
 
