@@ -376,6 +376,42 @@ def test_kelvin_helmholtz_reduced(oracle):
         assert rel_linf(g[k], o[k]) <= TOL
 
 
+@pytest.mark.parametrize("lim", ["minmod", "MLP5"])
+@pytest.mark.parametrize("bcs", BC_SETS[:7] + BC_SETS[8:9], ids=lambda b: "-".join(x[:3] for x in b))
+def test_minimal_grid_parity(oracle, bcs, lim):
+    """The smallest grid the reference accepts, ni = nj = n_ghost_layers (mod_input.f90:320-338): every halo
+    cell maps across the whole interior (periodic wrap, symmetry mirror of the full width), 4 SSP-RK3 steps."""
+    say("minimal grid", bcs, lim)
+    cfg = SchemeConfig(flux="AUSMPW+", limiter=lim, integrator="ssp_rk3", bcs=bcs, cfl=0.3, bc_pressure=0.7)
+    n = cfg.n_ghost
+    nx, ny = uniform_nodes(n, n, n, (0.0, 0.1), (0.0, 0.1))
+    rng = np.random.default_rng(31)
+    prims = [np.asfortranarray(rng.uniform(lo, hi, (n, n))) for lo, hi in ((0.8, 1.2), (-0.2, 0.2),
+                                                                          (-0.2, 0.2), (0.8, 1.2))]
+    g, o, gp, op = run_steps(oracle, cfg, nx, ny, prims, 4)
+    for k in range(4):
+        assert np.array_equal(g[k], o[k]), (k, rel_linf(g[k], o[k]))
+
+
+def test_maximum_size_grid_conserves_totals():
+    """8192^2 cells (67 M, ~13 GB of device state) on a periodic grid: 3 SSP-RK3 steps keep the mass and
+    momentum totals to round-off and the energy total to round-off (no source), the state finite."""
+    say("maximum size")
+    pb = kelvin_helmholtz(n=8192, steps=3)
+    with Solver(pb.cfg, pb.node_x, pb.node_y) as s:
+        s.set_primitive(pb.rho, pb.u, pb.v, pb.p)
+        u0 = s.conserved()
+        it, _, err = s.run(1e300, 3, 1e300)
+        u1 = s.conserved()
+    assert (it, err) == (3, 0)
+    for k in range(4):
+        assert np.all(np.isfinite(u1[k]))
+        tot0, tot1 = u0[k].sum(dtype=np.float64), u1[k].sum(dtype=np.float64)
+        mag = np.abs(u0[k]).sum()
+        say("8192^2 total", k, tot0, tot1)
+        assert abs(tot1 - tot0) <= 1e-12 * mag
+
+
 def test_error_codes_nan_and_negative_density():
     say("error codes")
     ni, nj = 20, 10
