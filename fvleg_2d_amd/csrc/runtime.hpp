@@ -33,6 +33,7 @@ struct Solve {
   // device arrays (carved from one allocation by bind())
   double *U[3][4] = {};
   double *P[3][4] = {};      // primitive buffer sets; P[0] = primitives of U^n between steps
+                             // (precision form: P[0] only, refreshed on demand, see prims_stale)
   double *dxi = nullptr, *lxi = nullptr, *dyj = nullptr, *lyj = nullptr;
   double *block_min = nullptr;
   double *result = nullptr;

@@ -12,10 +12,12 @@
 //      the next stage (fluid_t%calculate_derived_quantities), and on the last
 //      stage reduces the CFL time step and the sanity flags per workgroup
 //      (get_timestep, sanity_check).
-// Each cell's primitives are computed once per stage instead of once per
-// tile that needs them as halo; the price is 32 B/cell of extra writes on a
-// kernel whose time is spent in FP64 division, not in HBM traffic.  The
-// geometry of rectilinear grids is two 1D arrays.
+// FP64 form: each cell's primitives are computed once per stage instead of
+// once per tile that needs them as halo; the price is 32 B/cell of extra
+// writes on a kernel whose time is spent in FP64 division, not in HBM
+// traffic.  Precision form (R = f32x): the balance is the other way round, so
+// step 1 reads U_k and recovers the primitives of tile and halo itself, and
+// step 4 stores none.  The geometry of rectilinear grids is two 1D arrays.
 #pragma once
 
 #include "physics.hpp"
