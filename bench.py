@@ -158,6 +158,38 @@ def diagnostics_cost(pb, steps=500):
     return out
 
 
+def device_graph_cost(pb_kh, kh_steps=200):
+    """Device-side loop with the 32-step hipGraph replay against one launch per kernel: the Sod deck
+    (400x4, AUSM+, minmod, SSP-RK2, to t = 0.2; launch-bound) through cato_run, and the KH workload through
+    cato_advance.  Timing is off in both (per-launch events would force individual launches)."""
+    from fvleg_2d_amd import Solver
+    from fvleg_2d_amd.problems import sod
+
+    out = {}
+    pbs = sod()
+    for label, pb in (("sod_400x4_run", pbs), (f"kh_{pb_kh.ni}x{pb_kh.nj}_advance", pb_kh)):
+        res = {}
+        with Solver(pb.cfg, pb.node_x, pb.node_y) as s:
+            for graph in (True, False, True, False):  # second pair after warm clocks; the last of each is kept
+                s.set_device_graph(graph)
+                s.set_primitive(pb.rho, pb.u, pb.v, pb.p)
+                s.synchronize()
+                t0 = time.perf_counter()
+                if pb is pbs:
+                    n, _, err = s.run(pb.max_time, 0, pb.contour_interval_dt)
+                    assert err == 0
+                else:
+                    n = kh_steps
+                    s.advance(n)
+                    s.synchronize()
+                wall = time.perf_counter() - t0
+                res["steps"] = n
+                res["us_per_step_graph" if graph else "us_per_step_eager"] = wall / n * 1e6
+        res["speedup"] = res["us_per_step_eager"] / res["us_per_step_graph"]
+        out[label] = res
+    return out
+
+
 def roofline(pb, stage_avg_ms, launches, precision, n):
     integ = pb.cfg.integrator
     cells = pb.ni * pb.nj
@@ -290,6 +322,8 @@ def main():
         }
         say("[bench] convergence diagnostics on/off (device loop)")
         out["diagnostics"] = diagnostics_cost(pb)
+        say("[bench] device loop: 32-step hipGraph replay vs one launch per kernel")
+        out["device_graph"] = device_graph_cost(pb)
         say("[bench] riemann 1024^2 HLLC/MLP5, FP64 and FP32-flux")
         out["riemann_1024"] = riemann_forms()
         from fvleg_2d_amd.benchforms import measure_batched, measure_served

@@ -63,6 +63,7 @@ SIGNATURES = {
     "cato_get_history": (C.c_int, [_CTX, _DP, C.c_longlong, C.POINTER(C.c_longlong)]),
     "cato_synchronize": (C.c_int, [_CTX]),
     "cato_set_timing": (C.c_int, [_CTX, C.c_int]),
+    "cato_set_device_graph": (C.c_int, [_CTX, C.c_int]),
     "cato_get_timing": (C.c_int, [_CTX, _DP, C.POINTER(C.c_longlong), _DP, C.POINTER(C.c_longlong)]),
     "cato_apply_bc": (C.c_int, [_CTX, _DP, _DP, _DP, _DP, C.c_int]),
     "cato_interpolate_edges": (C.c_int, [_CTX, _DP, C.c_int, _DP, _DP]),

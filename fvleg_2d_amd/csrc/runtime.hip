@@ -455,7 +455,19 @@ struct cato_ctx {
   double diag_time = 0.0;
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<std::pair<hipEvent_t, hipEvent_t>, int>> pending;  // (start, stop), is_stage
+  // kGraphSteps steps of the device-side loop, captured once and replayed by
+  // cato_run / cato_advance while timing is off; dropped whenever the stage
+  // arguments change (energy source, diagnostics)
+  hipGraphExec_t steps_exec = nullptr;
+  bool use_graph = true;
 };
+
+constexpr long long kGraphSteps = 32;
+
+static void drop_steps_graph(cato_ctx *c) {
+  if (c->steps_exec) (void)hipGraphExecDestroy(c->steps_exec);
+  c->steps_exec = nullptr;
+}
 
 namespace {
 
@@ -645,6 +657,7 @@ int cato_ctx_destroy(cato_ctx *c) {
     (void)hipEventDestroy(p.first.second);
   }
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  drop_steps_graph(c);
   if (c->s.src_E) (void)hipFree(c->s.src_E);
   if (c->diag) (void)hipFree(c->diag);
   if (c->mem) (void)hipFree(c->mem);
@@ -697,6 +710,8 @@ int cato_set_energy_source(cato_ctx *c, const double *q, int ld, int halo) {
   int rc = check_ctx(c);
   if (rc) return rc;
   Solve &s = c->s;
+  HIPCK(hipStreamSynchronize(c->stream));
+  drop_steps_graph(c);  // the stage arguments carry the source pointer
   if (!q) {
     // the stages already enqueued keep reading the old field: free it after them
     if (s.src_E) {
@@ -783,6 +798,53 @@ int cato_step(cato_ctx *c, double dt, int *error_code) {
   return CATO_OK;
 }
 
+// One step of the device-side loop (stages with the device dt, then the
+// end-of-step reduction and loop update).
+static int enqueue_step(cato_ctx *c) {
+  Solve &s = c->s;
+  int rc = launch_stages(c, s.step_stages(s.loop, 0.0));
+  if (rc) return rc;
+  s.prims_valid = false;
+  return launch_reduce(c, 2, true);
+}
+
+// n steps of the device-side loop.  The arguments of every step are the same
+// (the primitive buffers cycle back each step), so with timing off whole runs
+// of kGraphSteps steps replay one captured graph: no host work per kernel and
+// shorter gaps between them.  The first step after cato_set_primitive (which
+// checks the caller's primitives) and the remainder run eagerly.
+static int enqueue_steps(cato_ctx *c, long long n) {
+  Solve &s = c->s;
+  long long k = 0;
+  int rc = CATO_OK;
+  if (n > 0 && s.prims_valid) {
+    if ((rc = enqueue_step(c))) return rc;
+    k = 1;
+  }
+  if (c->use_graph && !c->timing && n - k >= kGraphSteps) {
+    if (!c->steps_exec) {
+      hipGraph_t g = nullptr;
+      HIPCK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      for (long long q = 0; q < kGraphSteps && !rc; ++q) rc = enqueue_step(c);
+      hipError_t e = hipStreamEndCapture(c->stream, &g);
+      if (rc || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? rc : hip_check(e, "capture device-loop steps");
+      }
+      hipError_t ie = hipGraphInstantiate(&c->steps_exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) {
+        c->steps_exec = nullptr;
+        return hip_check(ie, "instantiate device-loop steps");
+      }
+    }
+    for (; n - k >= kGraphSteps; k += kGraphSteps) HIPCK(hipGraphLaunch(c->steps_exec, c->stream));
+  }
+  for (; k < n; ++k)
+    if ((rc = enqueue_step(c))) return rc;
+  return CATO_OK;
+}
+
 static int run_impl(cato_ctx *c, double max_time, long long max_iter, double contour_dt, double constant_dt,
                     bool poll, long long fixed_steps, long long *iters, double *time, int *err) {
   Solve &s = c->s;
@@ -799,16 +861,12 @@ static int run_impl(cato_ctx *c, double max_time, long long max_iter, double con
   int rc = launch_cfl(c, 1);
   if (rc) return rc;
   long long launched = 0;
-  const long long chunk = 32;
-  static_assert(32 < Solve::kHistCap, "a poll must find every row of its chunk in the ring");
+  const long long chunk = kGraphSteps;
+  static_assert(kGraphSteps < Solve::kHistCap, "a poll must find every row of its chunk in the ring");
   LoopState H{};
   while (true) {
     long long n = poll ? chunk : fixed_steps;
-    for (long long k = 0; k < n; ++k) {
-      if ((rc = launch_stages(c, s.step_stages(s.loop, 0.0)))) return rc;
-      s.prims_valid = false;
-      if ((rc = launch_reduce(c, 2, true))) return rc;
-    }
+    if ((rc = enqueue_steps(c, n))) return rc;
     launched += n;
     if (!poll) break;
     long long seen = H.iteration;
@@ -869,6 +927,7 @@ int cato_set_diagnostics(cato_ctx *c, int enable) {
   if (rc) return rc;
   Solve &s = c->s;
   HIPCK(hipStreamSynchronize(c->stream));
+  drop_steps_graph(c);  // the last stage's arguments carry the diagnostics buffers
   c->history.clear();
   c->diag_iter = 0;
   c->diag_time = 0.0;
@@ -921,6 +980,15 @@ int cato_set_timing(cato_ctx *c, int enable) {
   c->timing = enable != 0;
   c->stage_ms = c->other_ms = 0.0;
   c->stage_n = c->other_n = 0;
+  return CATO_OK;
+}
+
+int cato_set_device_graph(cato_ctx *c, int enable) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  HIPCK(hipStreamSynchronize(c->stream));
+  drop_steps_graph(c);
+  c->use_graph = enable != 0;
   return CATO_OK;
 }
 

@@ -169,6 +169,10 @@ class Solver:
             N.check(self.lib.cato_get_history(self.h, _p(rows), n.value, C.byref(n)), "cato_get_history")
         return rows[: n.value]
 
+    def set_device_graph(self, on: bool):
+        """run() / advance() replay a captured 32-step hipGraph (on, the default) or launch every kernel."""
+        N.check(self.lib.cato_set_device_graph(self.h, int(on)), "cato_set_device_graph")
+
     def set_timing(self, on: bool):
         N.check(self.lib.cato_set_timing(self.h, int(on)), "cato_set_timing")
 

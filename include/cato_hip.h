@@ -153,6 +153,11 @@ int cato_run(cato_ctx *ctx, double max_time, long long max_iterations, double co
 int cato_set_energy_source(cato_ctx *ctx, const double *q_dot, int ld, int halo);
 /* Enqueue exactly n steps with CFL dt and no host synchronisation (benchmarks). */
 int cato_advance(cato_ctx *ctx, long long n);
+/* Device-side loop of cato_run / cato_advance as a captured hipGraph of 32 steps
+ * (enable != 0, the default) or as individual kernel launches.  Both give
+ * identical results; timing (cato_set_timing) always launches individually.
+ * No counterpart in the reference: its loop is host-driven (main.f90:138-198). */
+int cato_set_device_graph(cato_ctx *ctx, int enable);
 int cato_synchronize(cato_ctx *ctx);
 
 /* Step bookkeeping of fluid_t%integrate: calculate_convergence (mod_fluid.f90:1162-1205),

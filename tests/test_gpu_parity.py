@@ -312,6 +312,56 @@ def test_convergence_history_parity(oracle, integrator):
         assert np.array_equal(g[k], f.interior(name)), name
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_flux"])
+def test_device_loop_graph_matches_eager(oracle, precision):
+    """cato_advance / cato_run replay a captured 32-step graph unless the graph is switched off or timing is on
+    (then every kernel is launched on its own): identical states and history rows, also after the stage arguments change between
+    runs (energy source on, diagnostics on: the graph is dropped and captured again).  FP64 also matches the
+    oracle's run loop (main.f90:138-198) bit for bit."""
+    say("device-loop graph", precision)
+    ni, nj = 45, 33
+    # outlet pressure near the interior's: the outlet's Mach switch (outlet_bc.f90:96-123) then never flips, and
+    # a 1e-9 perturbation of the start stays at 1e-9 (at 0.8 it grows to O(1) within 20 steps, even in FP64)
+    cfg = SchemeConfig(flux="SLAU", limiter="MLP5", integrator="ssp_rk3",
+                       bcs=("outflow", "symmetry", "periodic", "periodic"), bc_pressure=0.5, cfl=0.4,
+                       precision=precision)
+    nh = cfg.n_ghost
+    nx, ny = uniform_nodes(ni, nj, nh, (0.0, 1.0), (0.0, 0.8))
+    prims = shock_state(ni, nj, 31)
+    X, Y = np.meshgrid((np.arange(ni) + 0.5) / ni, (np.arange(nj) + 0.5) / nj, indexing="ij")
+    q = np.asfortranarray(30.0 * np.exp(-((X - 0.5) ** 2 + (Y - 0.4) ** 2) / 0.03))
+
+    def drive(mode):
+        with Solver(cfg, nx, ny) as s:
+            s.set_timing(mode == "timed")
+            s.set_device_graph(mode != "eager")
+            s.set_primitive(*prims)
+            s.advance(70)  # one eager step after set_primitive, two 32-step replays, five eager steps
+            first = s.conserved()
+            s.set_energy_source(q)
+            s.advance(40)
+            s.set_diagnostics(True)
+            it, t, err = s.run(1e30, 75)
+            return first, s.conserved(), s.history(), (it, t, err), s.timing()["stage_launches"]
+
+    g1, g2, gh, gr, g_launches = drive("graph")
+    for mode in ("eager", "timed"):
+        e1, e2, eh, er, e_launches = drive(mode)
+        # cato_run launches whole 32-step polls (96 steps for 75; the steps after the last one are no-ops)
+        assert g_launches == 0 and e_launches == (3 * (70 + 40 + 96) if mode == "timed" else 0)
+        assert gr == er and gr[0] == 75 and gr[2] == 0
+        assert np.array_equal(gh, eh) and gh.shape == (75, 7)
+        for k in range(4):
+            assert np.array_equal(g1[k], e1[k]), (mode, "advance", k)
+            assert np.array_equal(g2[k], e2[k]), (mode, "after source + diagnostics", k)
+    if precision == "fp64":
+        oc = oracle_cfg(oracle, cfg, ni, nj)
+        f = oracle.Fluid(oc, oracle.Geometry(oc, nx, ny), *[pad(a, nh) for a in prims])
+        assert f.run(1e300, 70, 1e300)[0] == 70
+        for k, name in enumerate(("rho", "rho_u", "rho_v", "rho_E")):
+            assert np.array_equal(g1[k], f.interior(name)), name
+
+
 @pytest.mark.parametrize("case", ["kh2048", "riemann1024"])
 def test_full_size_workloads_match_oracle(oracle, case):
     """The bench workloads at their full sizes: Kelvin-Helmholtz 2048^2 (SLAU, van Leer, SSP-RK3) for 3 steps
