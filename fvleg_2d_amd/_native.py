@@ -17,7 +17,8 @@ CATO_OK = 0
 FLUX = {"AUSMPW+": 0, "SLAU": 1, "SLAU2": 2, "SD-SLAU": 3, "SD-SLAU2": 4,
         "Roe": 5, "S-Roe": 5, "SP-Roe": 5, "AUSM+": 6, "AUSM+-u": 6, "HLLC": 7, "M-AUSMPW+": 8}
 LIMITER = {"minmod": 0, "superbee": 1, "van_leer": 2, "vanleer": 2, "MLP3": 3, "MLP5": 4}
-BC = {"periodic": 0, "zero_gradient": 1, "symmetry": 2, "mirror": 2, "reflection": 2, "outflow": 3, "outlet": 3}
+BC = {"periodic": 0, "zero_gradient": 1, "symmetry": 2, "mirror": 2, "reflection": 2, "outflow": 3, "outlet": 3,
+      "pressure_input": 4}
 INTEGRATOR = {"ssp_rk2": 0, "ssp_rk3": 1, "ssp_rk33": 1, "rk(1,3)": 2, "rk(1,4)": 3, "rk(1,5)": 4,
               "rk(2,3)": 5, "rk(2,4)": 6, "rk(2,5)": 7}
 PRECISION = {"fp64": 0, "fp32_flux": 1}
@@ -29,6 +30,7 @@ class CatoConfig(C.Structure):
         ("bc_plus_x", C.c_int), ("bc_minus_x", C.c_int), ("bc_plus_y", C.c_int), ("bc_minus_y", C.c_int),
         ("n_ghost_layers", C.c_int), ("apply_low_mach_fix", C.c_int),
         ("gamma", C.c_double), ("cfl", C.c_double), ("tile", C.c_int), ("bc_pressure", C.c_double),
+        ("bc_density", C.c_double),
     ]
 
 
@@ -52,6 +54,8 @@ SIGNATURES = {
     "cato_set_conserved": (C.c_int, [_CTX, _DP, _DP, _DP, _DP, C.c_int, C.c_int]),
     "cato_get_conserved": (C.c_int, [_CTX, _DP, _DP, _DP, _DP, C.c_int, C.c_int]),
     "cato_set_energy_source": (C.c_int, [_CTX, _DP, C.c_int, C.c_int]),
+    "cato_set_pressure_input": (C.c_int, [_CTX, _DP, _DP, _DP, C.c_int]),
+    "cato_set_time": (C.c_int, [_CTX, C.c_double]),
     "cato_get_primitive": (C.c_int, [_CTX, _DP, _DP, _DP, _DP, _DP, C.c_int, C.c_int]),
     "cato_compute_dt": (C.c_int, [_CTX, _DP]),
     "cato_step": (C.c_int, [_CTX, C.c_double, C.POINTER(C.c_int)]),

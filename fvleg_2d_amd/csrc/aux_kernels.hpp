@@ -303,6 +303,179 @@ static __global__ void k_outflow_column(FieldGeom g, double *rho, double *u, dou
   p[k + 1] = q.p;
 }
 
+// ------------------------- +x pressure input boundary ----------------------
+// pressure_input_bc.f90:232-385.  Its ghosts depend on the largest pressure of
+// column ni, so they come from a one-workgroup pass over the column before each
+// stage (the stage kernel then reads them from column ni+1 of its primitives).
+
+// Where a pass takes its boundary time and desired state from.
+struct PressureInput {
+  const double *tab;     // n times, then n pressures, then n densities (null: the constants)
+  int n;
+  double p_const, rho_const;
+  // mode 0: evaluate at t_at; mode 1: a step's stage at (t0 if first_multistage
+  // else the fluid time at the step's start) + dt, with the fluid time and dt
+  // from *loop when set (device-side loop) or t_at / dt_at
+  int mode, first_multistage;
+  double t_at, dt_at, t0;
+  const LoopState *loop;
+};
+
+// finterp's linear_interp_1d%evaluate (linear_interpolation_module.f90:730-762,
+// interval search dintrv 1189-1323): the interval [t_k, t_k+1] holding x, or
+// the first / last interval outside the table, where the line extrapolates.
+__device__ inline double interp_linear(const double *t, const double *f, int n, double x) {
+  int k;
+  if (x >= t[n - 1]) {
+    k = n - 2;
+  } else if (x < t[0]) {
+    k = 0;
+  } else {
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+      int mid = (lo + hi) / 2;
+      if (x < t[mid]) hi = mid; else lo = mid;
+    }
+    k = lo;
+  }
+  double q1 = (x - t[k]) / (t[k + 1] - t[k]);
+  double p1 = 1.0 - q1;
+  return p1 * f[k] + q1 * f[k + 1];
+}
+
+__device__ inline void pressure_input_desired(const PressureInput &a, double &pd, double &rd) {
+  if (!a.tab) {
+    pd = a.p_const;
+    rd = a.rho_const;
+    return;
+  }
+  double t;
+  if (a.mode == 0) {
+    t = a.t_at;
+  } else {
+    double tn = a.loop ? a.loop->master_time : a.t_at;
+    double dt = a.loop ? a.loop->dt : a.dt_at;
+    t = (a.first_multistage ? a.t0 : tn) + dt;
+  }
+  pd = interp_linear(a.tab, a.tab + a.n, a.n, t);
+  rd = interp_linear(a.tab, a.tab + 2 * a.n, a.n, t);
+}
+
+// Ghost state of one row from its domain cell (rho_d, u_d, p_d) (the domain v
+// is taken as 0, pressure_input_bc.f90:312): outlet (u > 0) subsonic_outlet
+// unless supersonic with the column's largest pressure at or above pd, then
+// supersonic_outlet; inlet supersonic: (rd, u_d, 0, pd); else subsonic_inlet
+// with normal (1, 0) and flow angle 0 (inlet_outlet.f90:26-191).  pd or rd
+// below tiny (an error stop in the reference): NaN.
+__device__ inline Prim<double> pressure_input_ghost(double rho_d, double u_d, double p_d, double pd, double rd,
+                                                    double pmax, double gamma) {
+  const double TINY = 2.2250738585072014e-308;
+  const double nan = __builtin_nan("");
+  if (!(pd >= TINY) || !(rd >= TINY)) return Prim<double>{nan, nan, 0.0, nan};
+  const double cs = sqrt(gamma * p_d / rho_d);
+  const double mach_u = u_d / cs;
+  if (mach_u > 0.0) {
+    if (mach_u <= 1.0 || pmax < pd) {
+      double rho_b = rho_d + (pd - p_d) / (cs * cs);
+      double u_b = u_d + (1.0 * (p_d - pd)) / (rho_d * cs);
+      return Prim<double>{rho_b, u_b, 0.0, pd};
+    }
+    return Prim<double>{rho_d, u_d, 0.0, p_d};
+  }
+  if (absd(mach_u) > 1.0) return Prim<double>{rd, u_d, 0.0, pd};
+  // subsonic_inlet: v_dot_n = u*(-1) + 0*(-0), the domain speed |u|, R- and
+  // the boundary sound speed of the total-pressure inlet, isentropic p and rho
+  const double gm1 = gamma - 1.0;
+  double cos_theta = 0.0;
+  double v_dot_n = u_d * (-1.0) + 0.0 * (-0.0);
+  double v_tot_d = sqrt(u_d * u_d + 0.0 * 0.0);
+  if (v_tot_d > 0.0) cos_theta = -(v_dot_n / v_tot_d);
+  double cs_0_sq = cs * cs + (gm1 / 2.0) * (v_tot_d * v_tot_d);
+  double R_minus = v_dot_n - (2.0 * cs / gm1);
+  double den = gm1 * (cos_theta * cos_theta) + 2.0;
+  double first_term = -(R_minus * gm1 / den);
+  double cs_b = first_term;
+  if (absd(cos_theta) > 0.0) {
+    double second_term = (den * cs_0_sq) / (gm1 * (R_minus * R_minus));
+    cs_b = first_term * (1.0 + cos_theta * sqrt(second_term - gm1 / 2.0));
+  }
+  double R_plus = v_dot_n + (2.0 * cs_b / gm1);
+  double v_tot_b = (2.0 * cs_b / gm1) - R_plus;
+  double mach_b = v_tot_b / cs_b;
+  double base = 1.0 + 0.5 * gm1 * (mach_b * mach_b);
+  return Prim<double>{rd * pow(base, -(1.0 / gm1)), v_tot_b * 1.0, 0.0, pd * pow(base, -(gamma / gm1))};
+}
+
+__device__ inline double nan_skipping_max(double m, double x) { return (x == x && !(m >= x)) ? x : m; }
+
+// Largest pressure of a column over the workgroup (1024 threads); values come
+// from get(j), j in [0, n).
+template <typename Get> __device__ inline double column_max(int n, Get &&get) {
+  __shared__ double red[16];
+  const int tid = threadIdx.x;
+  double m = __builtin_nan("");
+  for (int j = tid; j < n; j += blockDim.x) m = nan_skipping_max(m, get(j));
+  for (int o = 32; o > 0; o >>= 1) m = nan_skipping_max(m, __shfl_xor(m, o, 64));
+  if ((tid & 63) == 0) red[tid >> 6] = m;
+  __syncthreads();
+  double r = red[0];
+  for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = nan_skipping_max(r, red[w]);
+  return r;
+}
+
+// Column ni+1 of a pitched primitive set from column ni: of the primitives
+// themselves (src = the same set) or, RECOVER, of the state src = U_k
+// recovered as the precision form's stages recover it.  One workgroup.
+template <bool RECOVER>
+static __global__ __launch_bounds__(1024) void k_pressure_column(FieldGeom g, PressureInput a, const double *s0,
+                                                                 const double *s1, const double *s2,
+                                                                 const double *s3, double *rho, double *u, double *v,
+                                                                 double *p) {
+  if (a.loop && a.loop->done) return;
+  auto cell = [&](int j) {
+    size_t k = (size_t)j * g.pitch + (size_t)(g.ni - 1);
+    if constexpr (RECOVER) {
+      double cs;
+      return cons_to_prim_fast(s0[k], s1[k], s2[k], s3[k], g.gamma, g.gamma - 1.0, 0, &cs);
+    } else {
+      return Prim<double>{s0[k], s1[k], s2[k], s3[k]};
+    }
+  };
+  const double pmax = column_max(g.nj, [&](int j) { return cell(j).p; });
+  double pd, rd;
+  pressure_input_desired(a, pd, rd);
+  for (int j = threadIdx.x; j < g.nj; j += blockDim.x) {
+    Prim<double> d = cell(j);
+    Prim<double> q = pressure_input_ghost(d.rho, d.u, d.p, pd, rd, pmax, g.gamma);
+    size_t k = (size_t)j * g.pitch + (size_t)g.ni;
+    rho[k] = q.rho;
+    u[k] = q.u;
+    v[k] = q.v;
+    p[k] = q.p;
+  }
+}
+
+// The same on halo'd arrays (leading dimension ld) after the other sides are
+// filled: every row of the halo'd extent, every ghost layer (apply order of
+// flux_solver.f90:157-183, the pressure input last at priority 0).
+static __global__ __launch_bounds__(1024) void k_pressure_halo(FieldGeom g, PressureInput a, int ld, double *rho,
+                                                               double *u, double *v, double *p) {
+  const int ny = g.nj + 2 * g.nh, ii = g.ni + g.nh - 1;
+  const double pmax = column_max(ny, [&](int jj) { return p[(size_t)ii + (size_t)jj * ld]; });
+  double pd, rd;
+  pressure_input_desired(a, pd, rd);
+  for (int jj = threadIdx.x; jj < ny; jj += blockDim.x) {
+    size_t k = (size_t)ii + (size_t)jj * ld;
+    Prim<double> q = pressure_input_ghost(rho[k], u[k], p[k], pd, rd, pmax, g.gamma);
+    for (int l = 1; l <= g.nh; ++l) {
+      rho[k + l] = q.rho;
+      u[k + l] = q.u;
+      v[k + l] = q.v;
+      p[k + l] = q.p;
+    }
+  }
+}
+
 // --------------------------- per-operator kernels --------------------------
 // Halo'd column-major arrays: element (i, j), i,j in [1-nh, n+nh], at
 // (i+nh-1) + (j+nh-1)*ld.

@@ -39,6 +39,8 @@ int BatchEngine::init(int n, const cato_config *cfgs, const cato_geom *geoms, in
     c.precision = precision;
     c.tile = tile;
     if (c.integrator != integ) return fail(CATO_ERR_UNSUPPORTED, "all solves of a batch share one integrator");
+    if (c.bc_plus_x == BC_PRESSURE_INPUT)
+      return fail(CATO_ERR_UNSUPPORTED, "pressure_input boundaries run in single-solve contexts only");
     int rc = solves[k].init(c, geoms[k]);
     if (rc) return rc;
   }

@@ -1,6 +1,8 @@
 // Shared enumerations and launch-argument structures of the CATO HIP library.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include "../../include/cato_hip.h"
 
 namespace cato {
@@ -31,8 +33,13 @@ enum BcId {
   BC_PERIODIC = CATO_BC_PERIODIC,
   BC_ZERO_GRADIENT = CATO_BC_ZERO_GRADIENT,
   BC_SYMMETRY = CATO_BC_SYMMETRY,
-  BC_OUTFLOW = CATO_BC_OUTFLOW
+  BC_OUTFLOW = CATO_BC_OUTFLOW,
+  BC_PRESSURE_INPUT = CATO_BC_PRESSURE_INPUT
 };
+
+// +x sides whose ghost layers all read one stored column ni+1 of the primitive
+// arrays (filled per stage by the producer of the primitives).
+__host__ __device__ inline bool ghost_column_bc(int bc) { return bc == BC_OUTFLOW || bc == BC_PRESSURE_INPUT; }
 
 // How a stage kernel combines the residual L(U_k) with the stored states.
 enum StageMode {

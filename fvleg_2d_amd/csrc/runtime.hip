@@ -108,11 +108,17 @@ static int validate_config(const cato_config &c) {
   const int bcs[4] = {c.bc_plus_x, c.bc_minus_x, c.bc_plus_y, c.bc_minus_y};
   bool outflow = false;
   for (int k = 0; k < 4; ++k) {
-    if (bcs[k] < 0 || bcs[k] > 3) return fail(CATO_ERR_INVALID, "unsupported boundary condition type");
+    if (bcs[k] < 0 || bcs[k] > 4) return fail(CATO_ERR_INVALID, "unsupported boundary condition type");
     outflow = outflow || bcs[k] == BC_OUTFLOW;
+    if (k > 0 && bcs[k] == BC_PRESSURE_INPUT)  // pressure_input_bc.f90:377-381
+      return fail(CATO_ERR_INVALID, "Unsupported pressure_bc location (pressure_input is a +x boundary)");
   }
   if (outflow && !(c.bc_pressure > 0.0))
     return fail(CATO_ERR_INVALID, "Outlet ambient pressure is <= 0!");  // outlet_bc.f90:57-62
+  // A y outlet shares the pressure input's priority and is applied after it, over
+  // rows the pressure input has already read (its column maximum would see them).
+  if (c.bc_plus_x == BC_PRESSURE_INPUT && (c.bc_plus_y == BC_OUTFLOW || c.bc_minus_y == BC_OUTFLOW))
+    return fail(CATO_ERR_UNSUPPORTED, "pressure_input at +x with an outlet at +y or -y");
   if ((c.bc_plus_x == BC_PERIODIC) != (c.bc_minus_x == BC_PERIODIC) ||
       (c.bc_plus_y == BC_PERIODIC) != (c.bc_minus_y == BC_PERIODIC))
     return fail(CATO_ERR_UNSUPPORTED, "a periodic boundary must be periodic on both sides of its axis");
@@ -140,7 +146,7 @@ int Solve::init(const cato_config &c, const cato_geom &g) {
   nj = g.nj;
   nh = c.n_ghost_layers;
   // a +x outflow side keeps its ghost column (ni+1) in the primitive arrays
-  pitch = (ni + (c.bc_plus_x == BC_OUTFLOW ? 1 : 0) + 15) & ~15;
+  pitch = (ni + (ghost_column_bc(c.bc_plus_x) ? 1 : 0) + 15) & ~15;
   rc = build_geometry(ni, nj, nh, g.node_x, g.node_y, geo);
   if (rc) return rc;
   tile = default_tile(c, ni, nj);
@@ -452,7 +458,10 @@ struct cato_ctx {
   double *diag = nullptr;
   std::vector<double> history;
   long long diag_iter = 0;
-  double diag_time = 0.0;
+  // fluid_t%time (the host copy; after cato_advance it is current only in
+  // loop->master_time until read back) and the time the solve started from
+  double t_fluid = 0.0, t0 = 0.0;
+  bool t_on_device = false;
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<std::pair<hipEvent_t, hipEvent_t>, int>> pending;  // (start, stop), is_stage
   // kGraphSteps steps of the device-side loop, captured once and replayed by
@@ -520,12 +529,68 @@ template <typename F> int timed(cato_ctx *c, bool is_stage, F &&launch) {
   return rc;
 }
 
+// Arguments of a pressure-input pass: mode 0 evaluates at the fluid time, mode 1
+// at the boundary time of stage m of a step (dt: the host dt when loop is null).
+PressureInput pressure_args(const cato_ctx *c, int mode, int m, const LoopState *loop, double dt) {
+  const Solve &s = c->s;
+  PressureInput a{};
+  a.tab = s.pi_n ? s.pi_tab : nullptr;
+  a.n = s.pi_n;
+  a.p_const = s.cfg.bc_pressure;
+  a.rho_const = s.cfg.bc_density;
+  a.mode = mode;
+  const bool multistage = s.cfg.integrator != CATO_RK_SSP2 && s.cfg.integrator != CATO_RK_SSP3;
+  a.first_multistage = (mode == 1 && multistage && m == 0) ? 1 : 0;
+  a.t_at = c->t_fluid;
+  a.dt_at = dt;
+  a.t0 = c->t0;
+  a.loop = loop;
+  return a;
+}
+
+// The constant inputs are checked where the reference checks them: when a
+// boundary fill uses them (pressure_input_bc.f90:204-206, 227-229).
+int check_pressure_constants(const Solve &s) {
+  if (s.cfg.bc_plus_x != BC_PRESSURE_INPUT || s.pi_n) return CATO_OK;
+  if (!(s.cfg.bc_pressure >= 2.2250738585072014e-308))
+    return fail(CATO_ERR_INVALID, "Error in pressure_input_bc_t%get_desired_pressure: desired_pressure < tiny(1.0_rk)");
+  if (!(s.cfg.bc_density >= 2.2250738585072014e-308))
+    return fail(CATO_ERR_INVALID, "Error in pressure_input_bc_t%get_desired_density: desired_density < tiny(1.0_rk)");
+  return CATO_OK;
+}
+
+// Ghost column ni+1 of the primitives a stage reads, for a +x pressure input:
+// from those primitives (FP64 form, residual) or recovered from the stage's
+// input state (precision form).
+int launch_pressure_column(cato_ctx *c, const StageArgs &a, const PressureInput &pa) {
+  Solve &s = c->s;
+  FieldGeom g = s.fgeom();
+  double *d[4];
+  for (int v = 0; v < 4; ++v) d[v] = const_cast<double *>(a.prim[v]);
+  const bool recover = s.cfg.precision != CATO_PRECISION_FP64 && a.mode != MODE_RESIDUAL;
+  return timed(c, false, [&] {
+    if (recover)
+      hipLaunchKernelGGL(k_pressure_column<true>, dim3(1), dim3(1024), 0, c->stream, g, pa, a.uk[0], a.uk[1],
+                         a.uk[2], a.uk[3], d[0], d[1], d[2], d[3]);
+    else
+      hipLaunchKernelGGL(k_pressure_column<false>, dim3(1), dim3(1024), 0, c->stream, g, pa, a.prim[0], a.prim[1],
+                         a.prim[2], a.prim[3], d[0], d[1], d[2], d[3]);
+  });
+}
+
 int launch_stages(cato_ctx *c, const std::vector<StageArgs> &st) {
   Solve &s = c->s;
   dim3 grid((unsigned)(s.tiles_x * s.tiles_y));
+  const bool pin = s.cfg.bc_plus_x == BC_PRESSURE_INPUT;
+  int m = 0;
   for (const StageArgs &a : st) {
     int rc = validate_stage_args(a);
     if (rc) return rc;
+    if (pin) {
+      if ((rc = check_pressure_constants(s))) return rc;
+      if ((rc = launch_pressure_column(c, a, pressure_args(c, 1, m, a.loop, a.dt_value)))) return rc;
+    }
+    ++m;
     rc = timed(c, true, [&] { s.launcher(grid, c->stream, a); });
     if (rc) return rc;
     if (a.mode == MODE_RESIDUAL) continue;
@@ -566,6 +631,17 @@ int launch_cfl(cato_ctx *c, int loop_mode) {
 }
 
 int check_ctx(cato_ctx *c) { return c ? CATO_OK : fail(CATO_ERR_INVALID, "null context"); }
+
+// The host copy of the fluid time, read back after cato_advance left it on the device.
+int sync_fluid_time(cato_ctx *c) {
+  if (!c->t_on_device) return CATO_OK;
+  LoopState H{};
+  HIPCK(hipMemcpyAsync(&H, c->s.loop, sizeof(H), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  c->t_fluid = H.master_time;
+  c->t_on_device = false;
+  return CATO_OK;
+}
 
 // Host <-> device copies of interior fields with (ld, halo) host layout.
 int upload_field(cato_ctx *c, double *dst, const double *src, int ld, int halo, double *staging) {
@@ -659,6 +735,7 @@ int cato_ctx_destroy(cato_ctx *c) {
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   drop_steps_graph(c);
   if (c->s.src_E) (void)hipFree(c->s.src_E);
+  if (c->s.pi_tab) (void)hipFree(c->s.pi_tab);
   if (c->diag) (void)hipFree(c->diag);
   if (c->mem) (void)hipFree(c->mem);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -684,7 +761,8 @@ int cato_set_primitive(cato_ctx *c, const double *rho, const double *u, const do
   s.prims_valid = true;
   s.prims_stale = false;
   c->diag_iter = 0;
-  c->diag_time = 0.0;
+  c->t_fluid = c->t0;
+  c->t_on_device = false;
   return hip_check(hipStreamSynchronize(c->stream), "set_primitive");
 }
 
@@ -702,7 +780,8 @@ int cato_set_conserved(cato_ctx *c, const double *r, const double *ru, const dou
   s.prims_valid = false;
   s.prims_stale = false;
   c->diag_iter = 0;
-  c->diag_time = 0.0;
+  c->t_fluid = c->t0;
+  c->t_on_device = false;
   return hip_check(hipStreamSynchronize(c->stream), "set_conserved");
 }
 
@@ -726,6 +805,43 @@ int cato_set_energy_source(cato_ctx *c, const double *q, int ld, int halo) {
     return rc;
   if ((rc = upload_field(c, s.src_E, q, ld, halo, s.U[(s.cur + 1) % 3][0]))) return rc;
   return hip_check(hipStreamSynchronize(c->stream), "set_energy_source");
+}
+
+int cato_set_pressure_input(cato_ctx *c, const double *time, const double *pressure, const double *density, int n) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  Solve &s = c->s;
+  if (s.cfg.bc_plus_x != BC_PRESSURE_INPUT) return fail(CATO_ERR_INVALID, "no pressure_input boundary at +x");
+  if (n != 0 && (!time || !pressure || !density)) return fail(CATO_ERR_INVALID, "null table");
+  // initialize_1d / check_inputs (linear_interpolation_module.f90:451-478, 1743-1810)
+  if (n == 1 || n < 0) return fail(CATO_ERR_INVALID, "cannot use linear interpolation for only one point");
+  for (int k = 1; k < n; ++k)
+    if (!(time[k] > time[k - 1])) return fail(CATO_ERR_INVALID, "pressure input times are not strictly increasing");
+  HIPCK(hipStreamSynchronize(c->stream));
+  drop_steps_graph(c);  // the pressure-input passes carry the table
+  if (s.pi_tab) (void)hipFree(s.pi_tab);
+  s.pi_tab = nullptr;
+  s.pi_n = 0;
+  if (n == 0) return CATO_OK;
+  std::vector<double> tab((size_t)3 * n);
+  std::copy(time, time + n, tab.begin());
+  std::copy(pressure, pressure + n, tab.begin() + n);
+  std::copy(density, density + n, tab.begin() + 2 * n);
+  if ((rc = hip_check(hipMalloc((void **)&s.pi_tab, tab.size() * sizeof(double)), "alloc pressure input"))) return rc;
+  s.pi_n = n;
+  HIPCK(hipMemcpyAsync(s.pi_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  return hip_check(hipStreamSynchronize(c->stream), "set_pressure_input");
+}
+
+int cato_set_time(cato_ctx *c, double time) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  HIPCK(hipStreamSynchronize(c->stream));
+  drop_steps_graph(c);  // the pressure-input passes carry the start time
+  c->t0 = time;
+  c->t_fluid = time;
+  c->t_on_device = false;
+  return CATO_OK;
 }
 
 int cato_get_conserved(cato_ctx *c, double *r, double *ru, double *rv, double *rE, int ld, int halo) {
@@ -776,6 +892,7 @@ int cato_step(cato_ctx *c, double dt, int *error_code) {
   Solve &s = c->s;
   if (!(dt >= 2.2250738585072014e-308))
     return fail(CATO_ERR_INVALID, "The timestep dt is < tiny(1.0_rk)");  // slau_solver.fypp:162-166
+  if ((rc = sync_fluid_time(c))) return rc;
   if ((rc = launch_stages(c, s.step_stages(nullptr, dt)))) return rc;
   s.prims_valid = false;
   if ((rc = launch_reduce(c, 0, true))) return rc;
@@ -783,11 +900,11 @@ int cato_step(cato_ctx *c, double dt, int *error_code) {
   HIPCK(hipMemcpyAsync(r, s.result, sizeof(r), hipMemcpyDeviceToHost, c->stream));
   if (c->diag) HIPCK(hipMemcpyAsync(nrm, s.conv_norms, sizeof(nrm), hipMemcpyDeviceToHost, c->stream));
   HIPCK(hipStreamSynchronize(c->stream));
+  // fluid_t%integrate: time + dt (ssp_rk_3_3:844), iteration + 1 (mod_fluid.f90:422)
+  c->t_fluid = c->t_fluid + dt;
   if (c->diag) {
-    // fluid_t%integrate: iteration + 1 (mod_fluid.f90:422), time + dt (ssp_rk_3_3:844)
     c->diag_iter += 1;
-    c->diag_time = c->diag_time + dt;
-    const double row[kHistRow] = {(double)c->diag_iter, c->diag_time, dt, nrm[0], nrm[1], nrm[2], nrm[3]};
+    const double row[kHistRow] = {(double)c->diag_iter, c->t_fluid, dt, nrm[0], nrm[1], nrm[2], nrm[3]};
     c->history.insert(c->history.end(), row, row + kHistRow);
   }
   if (c->timing) collect_timing(c);
@@ -848,18 +965,20 @@ static int enqueue_steps(cato_ctx *c, long long n) {
 static int run_impl(cato_ctx *c, double max_time, long long max_iter, double contour_dt, double constant_dt,
                     bool poll, long long fixed_steps, long long *iters, double *time, int *err) {
   Solve &s = c->s;
+  int rc = sync_fluid_time(c);
+  if (rc) return rc;
+  // main.f90:97-102: the driver clock starts at the fluid's time
   LoopState L{};
-  L.time = 0.0;
-  L.master_time = 0.0;
-  L.next_output_time = 0.0 + contour_dt;
+  L.time = c->t_fluid;
+  L.master_time = c->t_fluid;
+  L.next_output_time = c->t_fluid + contour_dt;
   L.contour_dt = contour_dt;
   L.max_time = max_time;
   L.constant_dt = constant_dt;
   L.iteration = 0;
   L.max_iterations = max_iter;
   HIPCK(hipMemcpyAsync(s.loop, &L, sizeof(L), hipMemcpyHostToDevice, c->stream));
-  int rc = launch_cfl(c, 1);
-  if (rc) return rc;
+  if ((rc = launch_cfl(c, 1))) return rc;
   long long launched = 0;
   const long long chunk = kGraphSteps;
   static_assert(kGraphSteps < Solve::kHistCap, "a poll must find every row of its chunk in the ring");
@@ -883,14 +1002,17 @@ static int run_impl(cato_ctx *c, double max_time, long long max_iter, double con
     for (long long it = seen + 1; c->diag && it <= H.iteration; ++it) {
       const double *row = &ring[(size_t)((it - 1) % Solve::kHistCap) * kHistRow];
       c->history.push_back((double)(c->diag_iter + it));
-      c->history.push_back(c->diag_time + row[1]);
+      c->history.push_back(row[1]);
       c->history.insert(c->history.end(), row + 2, row + kHistRow);
     }
     if (H.done) break;
   }
-  if (!poll) return CATO_OK;
+  if (!poll) {
+    c->t_on_device = true;
+    return CATO_OK;
+  }
   c->diag_iter += H.iteration;
-  c->diag_time = c->diag_time + H.master_time;
+  c->t_fluid = H.master_time;
   if (iters) *iters = H.iteration;
   if (time) *time = H.time;
   if (err) *err = H.error;
@@ -930,7 +1052,6 @@ int cato_set_diagnostics(cato_ctx *c, int enable) {
   drop_steps_graph(c);  // the last stage's arguments carry the diagnostics buffers
   c->history.clear();
   c->diag_iter = 0;
-  c->diag_time = 0.0;
   if (!enable) {
     if (c->diag) (void)hipFree(c->diag);
     c->diag = nullptr;
@@ -1031,6 +1152,13 @@ int cato_apply_bc(cato_ctx *c, double *rho, double *u, double *v, double *p, int
   hipLaunchKernelGGL(k_apply_bc, dim3((unsigned)((nx * ny + 255) / 256)), dim3(256), 0, c->stream, g, ld, D, D + n,
                      D + 2 * n, D + 3 * n);
   HIPCK(hipGetLastError());
+  if (s.cfg.bc_plus_x == BC_PRESSURE_INPUT) {
+    // last, at priority 0, over the rows the other sides filled; at the fluid time
+    if ((rc = check_pressure_constants(s)) || (rc = sync_fluid_time(c))) return rc;
+    hipLaunchKernelGGL(k_pressure_halo, dim3(1), dim3(1024), 0, c->stream, g, pressure_args(c, 0, 0, nullptr, 0.0),
+                       ld, D, D + n, D + 2 * n, D + 3 * n);
+    HIPCK(hipGetLastError());
+  }
   for (int k = 0; k < 4; ++k) HIPCK(hipMemcpyAsync(H[k], D + k * n, n * 8, hipMemcpyDeviceToHost, c->stream));
   return hip_check(hipStreamSynchronize(c->stream), "apply_bc");
 }
@@ -1121,6 +1249,9 @@ int cato_residual(cato_ctx *c, double *rho, double *u, double *v, double *p, dou
   a.block_min = nullptr;
   a.block_err = nullptr;
   if ((rc = validate_stage_args(a))) return rc;
+  if (s.cfg.bc_plus_x == BC_PRESSURE_INPUT &&
+      (rc = launch_pressure_column(c, a, pressure_args(c, 0, 0, nullptr, 0.0))))
+    return rc;
   dim3 grid((unsigned)(s.tiles_x * s.tiles_y));
   s.launcher(grid, c->stream, a);
   HIPCK(hipGetLastError());

@@ -40,6 +40,8 @@ struct Solve {
   int *block_err = nullptr;
   LoopState *loop = nullptr;
   double *src_E = nullptr;  // energy source field (pitched like U; owned by the caller of bind())
+  double *pi_tab = nullptr; // pressure input table: n times, pressures, densities (owned by the caller)
+  int pi_n = 0;
   // convergence diagnostics (owned by the caller of bind()): 8 arrays of block
   // sums, the 4 norms of the last step, and a ring of history rows
   double *conv_sums = nullptr, *conv_norms = nullptr, *conv_hist = nullptr;

@@ -368,6 +368,8 @@ int cato_server_submit(cato_server *S, const cato_request *req, double arrival_t
   int nh = req->cfg.n_ghost_layers;
   if (g.ni <= 0 || g.nj <= 0 || nh <= 0 || !g.node_x || !g.node_y || !req->rho || !req->u || !req->v || !req->p)
     return fail(CATO_ERR_INVALID, "incomplete request");
+  if (req->cfg.bc_plus_x == BC_PRESSURE_INPUT)
+    return fail(CATO_ERR_UNSUPPORTED, "pressure_input boundaries run in single-solve contexts only");
   if (S->integrator < 0) {
     S->integrator = req->cfg.integrator;
     Solve probe;

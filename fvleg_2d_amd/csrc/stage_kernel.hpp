@@ -46,13 +46,13 @@ __device__ inline int map_axis(int g, int n, int bc_hi, int bc_lo, bool &flip) {
   flip = false;
   if (g < 1) {
     if (bc_lo == BC_PERIODIC) return g + n;
-    if (bc_lo == BC_ZERO_GRADIENT || bc_lo == BC_OUTFLOW) return 1;
+    if (bc_lo == BC_ZERO_GRADIENT || bc_lo == BC_OUTFLOW || bc_lo == BC_PRESSURE_INPUT) return 1;
     flip = true;
     return 1 - g;
   }
   if (g > n) {
     if (bc_hi == BC_PERIODIC) return g - n;
-    if (bc_hi == BC_ZERO_GRADIENT || bc_hi == BC_OUTFLOW) return n;
+    if (bc_hi == BC_ZERO_GRADIENT || bc_hi == BC_OUTFLOW || bc_hi == BC_PRESSURE_INPUT) return n;
     flip = true;
     return 2 * n + 1 - g;
   }
@@ -106,12 +106,15 @@ __device__ inline int wave_or(int v) {
 __device__ inline Prim<double> load_prim(const StageArgs &a, int gi, int gj) {
   bool fu, fv;
   int si = map_axis(gi, a.ni, a.bc[0], a.bc[1], fu);
-  // +x outflow: every ghost layer reads column ni+1 of the primitive arrays,
-  // which the producer of the state filled with outflow_ghost of column ni.
+  // +x outflow / pressure input: every ghost layer reads column ni+1 of the
+  // primitive arrays, which the producer of the state filled from column ni.
   // outflow_ghost commutes exactly with the v negation of a symmetry fill in
-  // y, so the y-mapped corners equal the reference's fill-after-mirror.
-  if (gi > a.ni && a.bc[0] == BC_OUTFLOW) si = a.ni + 1;
+  // y, so the y-mapped corners equal the reference's fill-after-mirror; the
+  // pressure input ignores the domain v and writes v = +0 on every row.
+  const bool gcol = gi > a.ni && ghost_column_bc(a.bc[0]);
+  if (gcol) si = a.ni + 1;
   int sj = map_axis(gj, a.nj, a.bc[2], a.bc[3], fv);
+  if (gcol && a.bc[0] == BC_PRESSURE_INPUT) fv = false;
   size_t off = (size_t)(sj - 1) * a.pitch + (size_t)(si - 1);
   Prim<double> pr;
   pr.rho = a.prim[0][off];
@@ -146,6 +149,9 @@ template <bool NT> __device__ inline void st_cell(double *p, double x) {
 // fill included.  The form stores no primitive arrays: 64 B/cell less HBM
 // traffic per stage for the price of the recovery on halo cells.
 __device__ inline Prim<double> load_recovered(const StageArgs &a, int gi, int gj) {
+  // the pressure input's ghosts need the whole column (its largest pressure):
+  // a pass before the stage stores them in column ni+1 of the primitive arrays
+  if (gi > a.ni && a.bc[0] == BC_PRESSURE_INPUT) return load_prim(a, gi, gj);
   bool fu, fv;
   int si = map_axis(gi, a.ni, a.bc[0], a.bc[1], fu);
   int sj = map_axis(gj, a.nj, a.bc[2], a.bc[3], fv);

@@ -11,11 +11,13 @@ import numpy as np
 from . import h5io
 
 
-def write_deck(pb, directory: str, name: str = "cato", max_iterations: int | None = None) -> str:
+def write_deck(pb, directory: str, name: str = "cato", max_iterations: int | None = None,
+               pressure_input_file: str | None = None, pressure_scale_factor: float = 1.0) -> str:
     """Write ``<name>.h5`` and ``<name>.ini`` for problem ``pb`` into ``directory``; returns the ini path.
 
     Ghost cells of the deck are edge copies of the interior (the solver refills them through the
-    boundary conditions before any use)."""
+    boundary conditions before any use).  A +x ``pressure_input`` side reads ``pressure_input_file``
+    when given, else the constant ``cfg.bc_pressure`` / ``cfg.bc_density`` (mod_input.f90:366-382)."""
     os.makedirs(directory, exist_ok=True)
     cfg = pb.cfg
     nh = cfg.n_ghost
@@ -36,13 +38,24 @@ def write_deck(pb, directory: str, name: str = "cato", max_iterations: int | Non
         "[initial_conditions]", "read_from_file = true", f"initial_condition_file = '{h5}'", "",
         "[boundary_conditions]", f"plus_x = '{cfg.bcs[0]}'", f"minus_x = '{cfg.bcs[1]}'",
         f"plus_y = '{cfg.bcs[2]}'", f"minus_y = '{cfg.bcs[3]}'",
-        *([f"outflow_ambient_pressure = {cfg.bc_pressure!r}"] if "outflow" in cfg.bcs else []), "",
+        *([f"outflow_ambient_pressure = {cfg.bc_pressure!r}"] if "outflow" in cfg.bcs else []),
+        *_pressure_input_lines(cfg, pressure_input_file, pressure_scale_factor), "",
         "[io]", f"contour_interval_dt = {pb.contour_interval_dt or pb.max_time!r}", "",
     ]
     ini = os.path.join(directory, name + ".ini")
     with open(ini, "w") as f:
         f.write("\n".join(lines))
     return ini
+
+
+def _pressure_input_lines(cfg, path, scale):
+    if cfg.bcs[0] != "pressure_input":
+        return []
+    if path:
+        return ["apply_constant_bc_pressure = false", f"bc_pressure_input_file = '{path}'",
+                f"bc_pressure_scale_factor = {float(scale)!r}"]
+    return ["apply_constant_bc_pressure = true", f"constant_bc_pressure_value = {cfg.bc_pressure!r}",
+            f"bc_density = {cfg.bc_density!r}"]
 
 
 def read_state(path: str) -> dict:

@@ -12,7 +12,8 @@ module cato_deck
   private
 
   integer, parameter :: rk = c_double
-  public :: cato_input_t, read_input_ini, read_initial_conditions, read_restart, write_state, write_contour
+  public :: cato_input_t, read_input_ini, read_initial_conditions, read_restart, write_state, write_contour, &
+            read_pressure_input
 
   !> Deck entries of the hot path (names as in input.ini)
   type :: cato_input_t
@@ -48,6 +49,12 @@ module cato_deck
     ! [boundary_conditions]
     character(len=32) :: plus_x = 'periodic', minus_x = 'periodic', plus_y = 'periodic', minus_y = 'periodic'
     real(rk) :: outflow_ambient_pressure = 0.0_rk
+    ! pressure_input (mod_input.f90:366-382)
+    logical :: apply_constant_bc_pressure = .false.
+    real(rk) :: constant_bc_pressure_value = 0.0_rk
+    real(rk) :: bc_density = 0.0_rk
+    character(len=256) :: bc_pressure_input_file = ''
+    real(rk) :: bc_pressure_scale_factor = 1.0_rk
     ! [io]
     real(rk) :: contour_interval_dt = huge(1.0_rk)
     ! [grid]
@@ -248,6 +255,53 @@ contains
   end function
 
   !> Read the [section] key = value pairs the hot path uses (mod_input.f90:200-410).
+  !> pressure_input_bc_t%read_pressure_input (pressure_input_bc.f90:93-178): count the records (a first
+  !> one starting with '#' is a header), then one list-directed read of time, pressure and density per
+  !> row; pressure and density times the scale factor (code units: this host takes unit reference scales).
+  subroutine read_pressure_input(path, scale, t, p, r)
+    character(len=*), intent(in) :: path
+    real(rk), intent(in) :: scale
+    real(rk), allocatable, intent(out) :: t(:), p(:), r(:)
+    character(len=300) :: buf
+    logical :: header, exists
+    integer :: u, ios, nlines, k
+    inquire(file=path, exist=exists)
+    if (.not. exists) then
+      write(*, '(a)') "Error: Pressure input file '"//trim(path)//"' not found"
+      error stop 1
+    end if
+    header = .false.
+    nlines = 0
+    open(newunit=u, file=path, status='old', action='read')
+    do
+      buf = ''
+      read(u, *, iostat=ios) buf
+      if (nlines == 0) then
+        buf = adjustl(buf)
+        if (buf(1:1) == '#') header = .true.
+      end if
+      if (ios /= 0) exit
+      nlines = nlines + 1
+    end do
+    close(u)
+    open(newunit=u, file=path, status='old', action='read')
+    if (header) then
+      read(u, *, iostat=ios)
+      nlines = nlines - 1
+    end if
+    allocate(t(max(nlines, 0)), p(max(nlines, 0)), r(max(nlines, 0)))
+    t = 0.0_rk
+    p = 0.0_rk
+    r = 0.0_rk
+    do k = 1, nlines
+      read(u, *, iostat=ios) t(k), p(k), r(k)
+      if (ios /= 0) exit
+    end do
+    close(u)
+    p = (p * 1.0_rk) * scale
+    r = (r * 1.0_rk) * scale
+  end subroutine
+
   subroutine read_input_ini(filename, inp)
     character(len=*), intent(in) :: filename
     type(cato_input_t), intent(out) :: inp
@@ -304,6 +358,11 @@ contains
       case ('boundary_conditions.plus_y'); inp%plus_y = val
       case ('boundary_conditions.minus_y'); inp%minus_y = val
       case ('boundary_conditions.outflow_ambient_pressure'); inp%outflow_ambient_pressure = to_real(val, key)
+      case ('boundary_conditions.apply_constant_bc_pressure'); inp%apply_constant_bc_pressure = to_logical(val)
+      case ('boundary_conditions.constant_bc_pressure_value'); inp%constant_bc_pressure_value = to_real(val, key)
+      case ('boundary_conditions.bc_density'); inp%bc_density = to_real(val, key)
+      case ('boundary_conditions.bc_pressure_input_file'); inp%bc_pressure_input_file = val
+      case ('boundary_conditions.bc_pressure_scale_factor'); inp%bc_pressure_scale_factor = to_real(val, key)
       case ('io.contour_interval_dt'); inp%contour_interval_dt = to_real(val, key)
       case ('grid.n_ghost_layers'); inp%n_ghost_layers = int(to_real(val, key))
       end select

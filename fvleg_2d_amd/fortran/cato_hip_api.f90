@@ -29,6 +29,7 @@ module cato_hip_api
     real(c_double) :: cfl = 0.1_c_double
     integer(c_int) :: tile = -1
     real(c_double) :: bc_pressure = 0.0_c_double
+    real(c_double) :: bc_density = 0.0_c_double
   end type
 
   !> cato_geom: interior cells and the node coordinates incl. ghost nodes
@@ -41,7 +42,8 @@ module cato_hip_api
   public :: cato_abi_version, cato_ctx_create, cato_ctx_destroy, cato_set_primitive, cato_set_conserved, &
             cato_get_conserved, cato_get_primitive, cato_set_energy_source, cato_set_diagnostics, &
             cato_get_convergence, cato_get_history, cato_compute_dt, cato_step, cato_run, cato_apply_bc, &
-            cato_interpolate_edges, cato_edge_fluxes, cato_residual, cato_error_message
+            cato_interpolate_edges, cato_edge_fluxes, cato_residual, cato_error_message, &
+            cato_set_pressure_input, cato_set_time
 
   interface
     function cato_abi_version() bind(C, name="cato_abi_version") result(v)
@@ -114,6 +116,24 @@ module cato_hip_api
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, q_dot
       integer(c_int), value :: ld, halo
+      integer(c_int) :: rc
+    end function
+
+    !> pressure_input table rows (time, pressure, density), scaled; n = 0 returns to the constants
+    function cato_set_pressure_input(ctx, time, pressure, density, n) bind(C, name="cato_set_pressure_input") &
+      result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: time(*), pressure(*), density(*)
+      integer(c_int), value :: n
+      integer(c_int) :: rc
+    end function
+
+    !> fluid time (and the time the solve starts from)
+    function cato_set_time(ctx, time) bind(C, name="cato_set_time") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: time
       integer(c_int) :: rc
     end function
 

@@ -26,6 +26,7 @@ program cato_hip_host
   type(cato_config_t) :: cfg
   real(rk), allocatable :: x(:, :), y(:, :), rho(:, :), u(:, :), v(:, :), p(:, :)
   real(rk), allocatable :: r(:, :), ru(:, :), rv(:, :), rE(:, :), cs(:, :)
+  real(rk), allocatable :: pi_t(:), pi_p(:), pi_r(:)
   character(len=512) :: ini_file, out_file, arg
   logical :: device_loop, dry_run, residual_hist, constant_dt, from_checkpoint
   real(rk) :: restart_time
@@ -109,13 +110,19 @@ program cato_hip_host
     write(*, '(a, 8(1x, i0))') 'config: flux limiter integrator bcs(+x -x +y -y) n_ghost =', cfg%flux, cfg%limiter, &
       cfg%integrator, cfg%bc_plus_x, cfg%bc_minus_x, cfg%bc_plus_y, cfg%bc_minus_y, cfg%n_ghost_layers
     write(*, '(a, 3(1x, es23.16))') 'gamma cfl max_time =', cfg%gamma, cfg%cfl, input%max_time
-    write(*, '(a, 1x, es23.16)') 'bc_pressure =', cfg%bc_pressure
+    write(*, '(a, 2(1x, es23.16))') 'bc_pressure bc_density =', cfg%bc_pressure, cfg%bc_density
     write(*, '(a, 2(1x, es23.16))') 'x(1-nh,1-nh) y(ni+nh+1,nj+nh+1) =', grid%node_x(1 - nh, 1 - nh), &
       grid%node_y(grid%ni + nh + 1, grid%nj + nh + 1)
     write(*, '(a, 2(1x, es23.16))') 'density(1,1) pressure(ni,nj) =', rho(1 + nh, 1 + nh), p(grid%ni + nh, grid%nj + nh)
     stop
   end if
   call dev%create(cfg, grid)
+  if (trim(input%plus_x) == 'pressure_input' .and. .not. input%apply_constant_bc_pressure) then
+    ! pressure_input_bc_constructor: the time-dependent input from its file
+    call read_pressure_input(trim(input%bc_pressure_input_file), input%bc_pressure_scale_factor, pi_t, pi_p, pi_r)
+    call check_rc(cato_set_pressure_input(dev%ctx, pi_t, pi_p, pi_r, int(size(pi_t), c_int)), &
+                  'pressure input '//trim(input%bc_pressure_input_file))
+  end if
   restart_time = 0.0_rk
   restart_iteration = 0
   if (input%restart_from_file) then
@@ -136,6 +143,8 @@ program cato_hip_host
       fluid%time = restart_time
       fluid%iteration = restart_iteration
     end if
+    ! the fluid (and its flux solver) start from the restart time (master_puppeteer.f90:106-132)
+    call check_rc(cato_set_time(dev%ctx, restart_time), 'cato_set_time')
     write(*, '(a, es22.15, a, i0)') 'Restart from '//trim(input%restart_file)//': time = ', restart_time, &
       ', iteration ', restart_iteration
   else

@@ -184,9 +184,9 @@ contains
     character(len=16), parameter :: limiters(6) = [character(len=16) :: 'minmod', 'superbee', 'van_leer', &
                                                     'vanleer', 'MLP3', 'MLP5']
     integer, parameter :: limiter_ids(6) = [0, 1, 2, 2, 3, 4]
-    character(len=16), parameter :: bcs(7) = [character(len=16) :: 'periodic', 'zero_gradient', 'symmetry', &
-                                               'mirror', 'reflection', 'outflow', 'outlet']
-    integer, parameter :: bc_ids(7) = [0, 1, 2, 2, 2, 3, 3]
+    character(len=16), parameter :: bcs(8) = [character(len=16) :: 'periodic', 'zero_gradient', 'symmetry', &
+                                               'mirror', 'reflection', 'outflow', 'outlet', 'pressure_input']
+    integer, parameter :: bc_ids(8) = [0, 1, 2, 2, 2, 3, 3, 4]
     character(len=16), parameter :: rks(9) = [character(len=16) :: 'ssp_rk2', 'ssp_rk3', 'ssp_rk33', 'rk(1,3)', &
                                                'rk(1,4)', 'rk(1,5)', 'rk(2,3)', 'rk(2,4)', 'rk(2,5)']
     integer, parameter :: rk_ids(9) = [0, 1, 1, 2, 3, 4, 5, 6, 7]
@@ -208,6 +208,11 @@ contains
     cfg%cfl = input%cfl
     cfg%tile = -1
     cfg%bc_pressure = input%outflow_ambient_pressure
+    if (trim(input%plus_x) == 'pressure_input') then
+      ! pressure_input_bc_constructor (pressure_input_bc.f90:81-89): the constants of the constant mode
+      cfg%bc_pressure = input%constant_bc_pressure_value
+      cfg%bc_density = input%bc_density
+    end if
   end function
 
   ! ------------------------------------------------------------ device solve

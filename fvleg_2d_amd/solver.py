@@ -27,9 +27,11 @@ class SchemeConfig:
 
     def __init__(self, flux="AUSMPW+", limiter="minmod", integrator="ssp_rk2",
                  bcs=("periodic", "periodic", "periodic", "periodic"), gamma=1.4, cfl=0.1,
-                 apply_low_mach_fix=True, precision="fp64", tile=-1, bc_pressure=0.0):
+                 apply_low_mach_fix=True, precision="fp64", tile=-1, bc_pressure=0.0, bc_density=0.0):
         self.flux, self.limiter, self.integrator = flux, limiter, integrator
-        self.bc_pressure = float(bc_pressure)  # outflow_ambient_pressure
+        # outflow_ambient_pressure, or the pressure_input's constant_bc_pressure_value
+        self.bc_pressure = float(bc_pressure)
+        self.bc_density = float(bc_density)  # pressure_input: bc_density
         self.bcs = tuple(bcs)  # (+x, -x, +y, -y)
         self.gamma, self.cfl = float(gamma), float(cfl)
         self.apply_low_mach_fix = bool(apply_low_mach_fix)
@@ -45,7 +47,7 @@ class SchemeConfig:
             return N.CatoConfig(N.FLUX[self.flux], N.LIMITER[self.limiter], N.INTEGRATOR[self.integrator],
                                 N.PRECISION[self.precision], N.BC[self.bcs[0]], N.BC[self.bcs[1]],
                                 N.BC[self.bcs[2]], N.BC[self.bcs[3]], self.n_ghost, int(self.apply_low_mach_fix),
-                                self.gamma, self.cfl, self.tile, self.bc_pressure)
+                                self.gamma, self.cfl, self.tile, self.bc_pressure, self.bc_density)
         except KeyError as e:
             raise ValueError(f"unknown scheme option {e}") from None
 
@@ -115,6 +117,21 @@ class Solver:
         q = _f64f(q_dot)
         ld, halo = self._ld_halo(q)
         N.check(self.lib.cato_set_energy_source(self.h, _p(q), ld, halo), "cato_set_energy_source")
+
+    def set_pressure_input(self, table):
+        """Time-dependent pressure input (pressure_input_bc_t%read_pressure_input): table = (time, pressure,
+        density) rows, already scaled; None returns to the constant bc_pressure / bc_density."""
+        if table is None:
+            N.check(self.lib.cato_set_pressure_input(self.h, None, None, None, 0), "cato_set_pressure_input")
+            return
+        t, p, r = (np.ascontiguousarray(a, dtype=np.float64) for a in table)
+        if not (len(t) == len(p) == len(r)):
+            raise ValueError("time, pressure and density need the same length")
+        N.check(self.lib.cato_set_pressure_input(self.h, _p(t), _p(p), _p(r), len(t)), "cato_set_pressure_input")
+
+    def set_time(self, t: float):
+        """Fluid time (and the time the solve starts from): fluid_t%set_time on an initialized fluid."""
+        N.check(self.lib.cato_set_time(self.h, float(t)), "cato_set_time")
 
     def conserved(self):
         out = [np.zeros((self.ni, self.nj), order="F") for _ in range(4)]

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define CATO_HIP_ABI_VERSION 2
+#define CATO_HIP_ABI_VERSION 3
 
 /* flux_solver names of the input deck (mod_fluid.f90:223-238) */
 enum {
@@ -63,8 +63,10 @@ enum {
   CATO_BC_PERIODIC = 0,      /* periodic_bc.fypp (priority 2)                                  */
   CATO_BC_ZERO_GRADIENT = 1, /* zero_gradient_bc.f90 (priority 1)                              */
   CATO_BC_SYMMETRY = 2,      /* symmetry_bc.f90, also 'mirror' / 'reflection' (priority 1)     */
-  CATO_BC_OUTFLOW = 3        /* outlet_bc.f90, 'outflow' / 'outlet' (priority 0): at +x the    */
+  CATO_BC_OUTFLOW = 3,       /* outlet_bc.f90, 'outflow' / 'outlet' (priority 0): at +x the    */
                              /* characteristic extrapolation to bc_pressure, elsewhere a copy  */
+  CATO_BC_PRESSURE_INPUT = 4 /* pressure_input_bc.f90 (priority 0, +x only): inlet/outlet      */
+                             /* states of inlet_outlet.f90 for a desired pressure and density  */
 };
 /* integration_strategy names (mod_fluid.f90:424-446) */
 enum {
@@ -100,7 +102,11 @@ typedef struct cato_config {
   double cfl;             /* [time] cfl */
   int tile;               /* -1: library choice; otherwise a tile id from cato_tile_shape */
   double bc_pressure;     /* outflow ambient pressure ([boundary_conditions] outflow_ambient_pressure,
-                             mod_input.f90:384-390); must be > 0 when a side is CATO_BC_OUTFLOW */
+                             mod_input.f90:384-390); must be > 0 when a side is CATO_BC_OUTFLOW.
+                             CATO_BC_PRESSURE_INPUT: constant_bc_pressure_value (mod_input.f90:375) */
+  double bc_density;      /* CATO_BC_PRESSURE_INPUT: bc_density (mod_input.f90:372), used with the
+                             constant pressure; both must be >= tiny when a stage uses them
+                             (get_desired_pressure / _density, pressure_input_bc.f90:204-229) */
 } cato_config;
 
 typedef struct cato_geom {
@@ -141,8 +147,10 @@ int cato_compute_dt(cato_ctx *ctx, double *dt);
 int cato_step(cato_ctx *ctx, double dt, int *error_code);
 /* main.f90 time loop on the device: while time < max_time and iteration <
  * max_iterations, dt = CFL (or constant_dt > 0), integrate, advance the driver
- * time with the contour-interval clip.  Returns iterations, driver time and the
- * first error code.  The host only polls every few steps. */
+ * time with the contour-interval clip.  The driver clock starts at the fluid time
+ * (main.f90:100-102; 0 unless cato_set_time or earlier steps moved it).  Returns
+ * iterations, driver time and the first error code.  The host only polls every
+ * few steps. */
 int cato_run(cato_ctx *ctx, double max_time, long long max_iterations, double contour_interval_dt, double constant_dt,
              long long *iterations, double *time, int *error_code);
 /* Energy source of fluid_t%time_derivative (mod_fluid.f90:500-509, with the field
@@ -151,6 +159,19 @@ int cato_run(cato_ctx *ctx, double max_time, long long max_iterations, double co
  * stage of every following cato_step / cato_run / cato_advance until it is
  * replaced or cleared (q_dot = NULL).  cato_residual (solver%solve) never adds it. */
 int cato_set_energy_source(cato_ctx *ctx, const double *q_dot, int ld, int halo);
+/* pressure_input_bc_t%read_pressure_input (pressure_input_bc.f90:93-178) after the file is read:
+ * n >= 2 rows of time (strictly increasing), pressure and density, already scaled
+ * (bc_pressure_scale_factor) and in code units.  Every following stage evaluates them
+ * by linear interpolation at its boundary time (finterp's linear_interp_1d, extrapolating
+ * beyond the ends), replacing the constant bc_pressure / bc_density; n = 0 returns to
+ * the constants.  Errors as initialize_1d: n < 2 or times not increasing. */
+int cato_set_pressure_input(cato_ctx *ctx, const double *time, const double *pressure, const double *density, int n);
+/* fluid_t%set_time + the time the flux solver starts from (fluid_t%initialize,
+ * mod_fluid.f90:169,240; master_puppeteer%set_time): the fluid time every following
+ * step advances, the start of cato_run's driver clock, and the origin of the boundary
+ * times of time-dependent boundaries.  cato_set_primitive / cato_set_conserved return
+ * the fluid to this time (default 0). */
+int cato_set_time(cato_ctx *ctx, double time);
 /* Enqueue exactly n steps with CFL dt and no host synchronisation (benchmarks). */
 int cato_advance(cato_ctx *ctx, long long n);
 /* Device-side loop of cato_run / cato_advance as a captured hipGraph of 32 steps

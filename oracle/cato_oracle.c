@@ -182,7 +182,8 @@ void or_derived_quantities(const or_config *c, const double *rho, const double *
 /* Boundary conditions on one image                                           */
 /* ------------------------------------------------------------------------- */
 /* periodic_bc.fypp:65, zero_gradient_bc.f90:50, symmetry_bc.f90:52, outlet_bc.f90:54 */
-static int bc_priority(int t) { return t == OR_BC_PERIODIC ? 2 : t == OR_BC_OUTFLOW ? 0 : 1; }
+/* pressure_input_bc.f90:78 */
+static int bc_priority(int t) { return t == OR_BC_PERIODIC ? 2 : (t == OR_BC_OUTFLOW || t == OR_BC_PRESSURE_INPUT) ? 0 : 1; }
 
 /* periodic_bc.fypp:78-283: either x side fills both x ghost strips (rows jlo:jhi);
  * corners only when all four sides are periodic. */
@@ -278,8 +279,128 @@ static void bc_outflow_plus_x(const or_config *c, double *rho, double *u, double
     }
 }
 
+/* linear_interp_1d%evaluate: interp_1d (linear_interpolation_module.f90:730-762) on the interval dintrv
+ * (1189-1323) returns for strictly increasing knots, [t_k, t_k+1] with t_k <= x < t_k+1, and the first or
+ * last interval outside the table, where the same formula extrapolates. */
+double or_interp_linear(int n, const double *t, const double *f, double x) {
+  int k;
+  if (x >= t[n - 1]) {
+    k = n - 2;
+  } else if (x < t[0]) {
+    k = 0;
+  } else {
+    int lo = 0, hi = n - 1; /* t[lo] <= x < t[hi] */
+    while (hi - lo > 1) {
+      int mid = (lo + hi) / 2;
+      if (x < t[mid]) hi = mid; else lo = mid;
+    }
+    k = lo;
+  }
+  double q1 = (x - t[k]) / (t[k + 1] - t[k]);
+  double p1 = 1.0 - q1;
+  return p1 * f[k] + q1 * f[k + 1];
+}
+
+/* subsonic_inlet (inlet_outlet.f90:26-140) for the +x boundary: normal (1, 0), flow angle 0, and the
+ * domain v that pressure_input_bc passes (0).  Operation order as written there; x**2 is x*x and
+ * x**y is pow (what gfortran emits). */
+static void subsonic_inlet_px(double gamma, double rho, double u, double p, double rho_0, double p_0,
+                              double *rho_b, double *u_b, double *p_b) {
+  const double n_x = 1.0, n_y = 0.0, v = 0.0, theta = 0.0;
+  double cos_theta = 0.0;
+  double v_dot_n = u * (-n_x) + v * (-n_y);
+  double v_tot_d = sqrt(u * u + v * v);
+  if (v_tot_d > 0.0) cos_theta = -(v_dot_n / v_tot_d);
+  double cs_d = sqrt(gamma * p / rho);
+  double cs_0_sq = cs_d * cs_d + ((gamma - 1.0) / 2.0) * (v_tot_d * v_tot_d);
+  double R_minus = v_dot_n - (2.0 * cs_d / (gamma - 1.0));
+  double first_term = -(R_minus * (gamma - 1.0) / ((gamma - 1.0) * (cos_theta * cos_theta) + 2.0));
+  double cs_b;
+  if (fabs(cos_theta) > 0.0) {
+    double second_term =
+        (((gamma - 1.0) * (cos_theta * cos_theta) + 2.0) * cs_0_sq) / ((gamma - 1.0) * (R_minus * R_minus));
+    double third_term = (gamma - 1.0) / 2.0;
+    cs_b = first_term * (1.0 + cos_theta * sqrt(second_term - third_term));
+  } else {
+    cs_b = first_term;
+  }
+  double R_plus = v_dot_n + (2.0 * cs_b / (gamma - 1.0));
+  double v_tot_b = (2.0 * cs_b / (gamma - 1.0)) - R_plus;
+  double mach_b = v_tot_b / cs_b;
+  double base = 1.0 + 0.5 * (gamma - 1.0) * (mach_b * mach_b);
+  *u_b = v_tot_b * cos(theta);
+  *p_b = p_0 * pow(base, -(gamma / (gamma - 1.0)));
+  *rho_b = rho_0 * pow(base, -(1.0 / (gamma - 1.0)));
+}
+
+/* pressure_input_bc_t%apply at +x (pressure_input_bc.f90:232-385): the desired boundary pressure and
+ * density at bc_time (get_desired_pressure / get_desired_density, 181-230: the constant inputs, or the
+ * table), then for every row j of the halo'd extent the state of mod_inlet_outlet chosen by the domain
+ * Mach number u/cs of cell (ni, j) and, for an outlet, by the largest pressure of column ni; every ghost
+ * layer of the row takes it with v = 0.  The reference stops with an error when the desired pressure or
+ * density is below tiny; this restatement fills the ghosts with NaN instead, which the step reports as
+ * NaN (error code 3), as the HIP path does.  maxval skips NaNs as gfortran's does. */
+static void bc_pressure_input_plus_x(const or_config *c, double *rho, double *u, double *v, double *p) {
+  const int ni = c->ni, nj = c->nj, nh = c->nh;
+  const double gamma = c->gamma;
+  double pd, rd;
+  if (c->pi_n >= 2) {
+    pd = or_interp_linear(c->pi_n, c->pi_t, c->pi_p, c->bc_time);
+    rd = or_interp_linear(c->pi_n, c->pi_t, c->pi_rho, c->bc_time);
+  } else {
+    pd = c->bc_pressure;
+    rd = c->bc_density;
+  }
+  const int bad = !(pd >= TINY) || !(rd >= TINY);
+  double pmax = NAN;
+  for (int j = 1 - nh; j <= nj + nh; ++j) {
+    double x = p[CI(ni, j)];
+    if (x == x && !(pmax >= x)) pmax = x;
+  }
+  for (int j = 1 - nh; j <= nj + nh; ++j) {
+    const double rho_d = rho[CI(ni, j)], u_d = u[CI(ni, j)], p_d = p[CI(ni, j)];
+    double gr, gu, gp;
+    if (bad) {
+      gr = gu = gp = NAN;
+    } else {
+      double cs = sqrt(gamma * p_d / rho_d);
+      double mach_u = u_d / cs;
+      if (mach_u > 0.0) {
+        if (mach_u <= 1.0 || pmax < pd) {
+          /* subsonic_outlet (inlet_outlet.f90:142-177), exit pressure pd, normal (1, 0) */
+          double cs_d = sqrt(gamma * p_d / rho_d);
+          gr = rho_d + (pd - p_d) / (cs_d * cs_d);
+          gu = u_d + (1.0 * (p_d - pd)) / (rho_d * cs_d);
+          gp = pd;
+        } else { /* supersonic_outlet (186-191) */
+          gr = rho_d;
+          gu = u_d;
+          gp = p_d;
+        }
+      } else if (fabs(mach_u) > 1.0) { /* pressure_input_bc.f90:335-336 */
+        gr = rd;
+        gu = u_d;
+        gp = pd;
+      } else {
+        subsonic_inlet_px(gamma, rho_d, u_d, p_d, rd, pd, &gr, &gu, &gp);
+      }
+    }
+    for (int i = ni + 1; i <= ni + nh; ++i) {
+      rho[CI(i, j)] = gr;
+      u[CI(i, j)] = gu;
+      v[CI(i, j)] = 0.0;
+      p[CI(i, j)] = gp;
+    }
+  }
+}
+
 static void bc_apply_one(const or_config *c, int type, int loc, double *rho, double *u, double *v, double *p) {
   int xdir = (loc == 0 || loc == 1);
+  if (type == OR_BC_PRESSURE_INPUT) {
+    /* pressure_input_bc.f90:377-381: +x only (the host rejects the other sides) */
+    if (loc == 0) bc_pressure_input_plus_x(c, rho, u, v, p);
+    return;
+  }
   if (type == OR_BC_OUTFLOW) {
     if (loc == 0) {
       bc_outflow_plus_x(c, rho, u, v, p);
@@ -1193,7 +1314,23 @@ static void convergence(const or_config *c, const stage_buf *first, const stage_
   }
 }
 
-int or_integrate(const or_config *c, const or_geometry *g, or_fluid *f, double dt) {
+/* Boundary time of each stage, from the flux solver objects the stages call (each solve first adds its dt
+ * to the solver's time, slau_solver.fypp:168 and the other solvers alike):
+ * - ssp_rk_2_2 / ssp_rk_3_3: stage 1 uses U's own solver, whose time has gained every step's dt since
+ *   fluid_t%initialize (the same sum as fluid_t%time); later stages use a stage fluid assigned from an
+ *   expression led by U, whose solver restarts from U%time (assign_fluid, mod_fluid.f90:707-711;
+ *   add_fluid / fluid_mul_real allocate from the left operand, 617, 635).  All: time + dt.
+ * - rk_multistage: stage 1 uses U_prev, a fresh copy of U each step (a local allocatable, 896-898, 952),
+ *   whose solver still holds the time U's solver started from, since U%t is never called there: t0 + dt;
+ *   later stages U_prev = U_curr (assign_fluid, U_curr led by U_0 = U): time + dt. */
+static double stage_bc_time(const or_config *c, const or_fluid *f, double dt, int stage) {
+  int multistage = c->integrator != OR_RK_SSP2 && c->integrator != OR_RK_SSP3;
+  return (multistage && stage == 0) ? f->t0 + dt : f->time + dt;
+}
+
+int or_integrate(const or_config *c_in, const or_geometry *g, or_fluid *f, double dt) {
+  or_config cb = *c_in;
+  const or_config *c = &cb;
   size_t n = ncell(c);
   stage_buf U, U1, U2, U3;
   alloc_stage(c, &U);
@@ -1208,10 +1345,12 @@ int or_integrate(const or_config *c, const or_geometry *g, or_fluid *f, double d
   if (c->integrator == OR_RK_SSP2) {
     /* ssp_rk_2_2 (mod_fluid.f90:1115-1129) */
     fluid_to_stage(f, &U1, n);
+    cb.bc_time = stage_bc_time(c, f, dt, 0);
     time_derivative(c, g, &U, L, f->src_E);
     combine1(c, &U, L, dt, &U1);
     err |= assign_stage(c, &U1);
     fluid_to_stage(f, &U2, n);
+    cb.bc_time = stage_bc_time(c, f, dt, 1);
     time_derivative(c, g, &U1, L, f->src_E);
     combine2(c, &U, 0.5, &U1, 0.5, L, 0.5 * dt, &U2);
     err |= assign_stage(c, &U2);
@@ -1220,14 +1359,17 @@ int or_integrate(const or_config *c, const or_geometry *g, or_fluid *f, double d
   } else if (c->integrator == OR_RK_SSP3) {
     /* ssp_rk_3_3 (mod_fluid.f90:773-802) */
     fluid_to_stage(f, &U1, n);
+    cb.bc_time = stage_bc_time(c, f, dt, 0);
     time_derivative(c, g, &U, L, f->src_E);
     combine1(c, &U, L, dt, &U1);
     err |= assign_stage(c, &U1);
     fluid_to_stage(f, &U2, n);
+    cb.bc_time = stage_bc_time(c, f, dt, 1);
     time_derivative(c, g, &U1, L, f->src_E);
     combine2(c, &U, 3.0 / 4.0, &U1, 1.0 / 4.0, L, (1.0 / 4.0) * dt, &U2);
     err |= assign_stage(c, &U2);
     fluid_to_stage(f, &U3, n);
+    cb.bc_time = stage_bc_time(c, f, dt, 2);
     time_derivative(c, g, &U2, L, f->src_E);
     combine2(c, &U, 1.0 / 3.0, &U2, 2.0 / 3.0, L, (2.0 / 3.0) * dt, &U3);
     err |= assign_stage(c, &U3);
@@ -1252,6 +1394,7 @@ int or_integrate(const or_config *c, const or_geometry *g, or_fluid *f, double d
     fluid_to_stage(f, &U1, n); /* U_prev */
     fluid_to_stage(f, &U2, n); /* U_curr */
     for (int m = 0; m < ns; ++m) {
+      cb.bc_time = stage_bc_time(c, f, dt, m);
       time_derivative(c, g, &U1, L, f->src_E);
       combine1(c, &U, L, dt * alpha[m], &U2);
       err |= assign_stage(c, &U2);
@@ -1265,6 +1408,7 @@ int or_integrate(const or_config *c, const or_geometry *g, or_fluid *f, double d
   memcpy(f->rho_u, fin->a[1], n * sizeof(double));
   memcpy(f->rho_v, fin->a[2], n * sizeof(double));
   memcpy(f->rho_E, fin->a[3], n * sizeof(double));
+  f->time = f->time + dt; /* ssp_rk_3_3:844, ssp_rk_2_2 and rk_multistage alike */
   int code = 0;
   if (err & 3) code = 3;
   or_derived_quantities(c, f->rho, f->rho_u, f->rho_v, f->rho_E, f->u, f->v, f->p, f->cs);
@@ -1281,9 +1425,10 @@ int or_integrate(const or_config *c, const or_geometry *g, or_fluid *f, double d
 
 long or_run(const or_config *c, const or_geometry *g, or_fluid *f, double max_time, long max_iterations,
             double contour_interval_dt, double *final_time, int *error_code) {
-  /* main.f90:97-198 with no I/O and no constant-dt / startup options */
-  double time = 0.0, next_output_time = 0.0 + contour_interval_dt;
-  double master_time = 0.0; /* master_puppeteer.f90:206 accumulates dt without the driver's clip */
+  /* main.f90:97-198 with no I/O and no constant-dt / startup options, starting at the fluid's time
+   * (main.f90:100-102 on a restart; 0 otherwise) */
+  double time = f->time, next_output_time = time + contour_interval_dt;
+  double master_time = time; /* master_puppeteer.f90:206 accumulates dt without the driver's clip */
   long iteration = 0;
   *error_code = 0;
   while (time < max_time && iteration < max_iterations) {

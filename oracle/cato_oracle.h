@@ -35,7 +35,7 @@ enum {
 enum { OR_LIM_MINMOD = 0, OR_LIM_SUPERBEE = 1, OR_LIM_VAN_LEER = 2, OR_LIM_MLP3 = 3, OR_LIM_MLP5 = 4 };
 
 /* Boundary condition ids (bc_factory.f90:48-64). */
-enum { OR_BC_PERIODIC = 0, OR_BC_ZERO_GRADIENT = 1, OR_BC_SYMMETRY = 2, OR_BC_OUTFLOW = 3 };
+enum { OR_BC_PERIODIC = 0, OR_BC_ZERO_GRADIENT = 1, OR_BC_SYMMETRY = 2, OR_BC_OUTFLOW = 3, OR_BC_PRESSURE_INPUT = 4 };
 
 /* Time integrators (mod_fluid.f90:424-446). */
 enum {
@@ -50,7 +50,14 @@ typedef struct {
   int apply_low_mach_fix;
   double gamma;
   double cfl;
-  double bc_pressure; /* outflow ambient pressure (outlet_bc.f90:55) */
+  double bc_pressure; /* outflow ambient pressure (outlet_bc.f90:55); pressure_input constant pressure */
+  /* pressure_input (+x, pressure_input_bc.f90): constant density (apply_constant_bc_pressure), or, with
+   * pi_n >= 2, pressure and density linear in time over the table (pi_t strictly increasing); bc_time is
+   * the time the boundary is evaluated at (boundary_condition_t%time, set per stage by or_integrate). */
+  double bc_density;
+  int pi_n;
+  const double *pi_t, *pi_p, *pi_rho;
+  double bc_time;
 } or_config;
 
 /* Per-cell geometry of the halo'd block (grid_block_2d.f90:352-391). */
@@ -82,6 +89,9 @@ void or_sound_speed(const or_config *c, const double *rho, const double *p, doub
 
 /* flux_solver.f90:138-185 with periodic_bc.fypp / zero_gradient_bc.f90 / symmetry_bc.f90 on one image. */
 void or_apply_bcs(const or_config *c, double *rho, double *u, double *v, double *p);
+
+/* linear_interp_1d%evaluate (src/third_party/finterp/linear_interpolation_module.f90:730-762). */
+double or_interp_linear(int n, const double *t, const double *f, double x);
 
 /* One boundary_condition_t%apply call; loc 0:+x 1:-x 2:+y 3:-y. */
 void or_apply_bc_single(const or_config *c, int type, int loc, double *rho, double *u, double *v, double *p);
@@ -116,6 +126,9 @@ typedef struct {
   double *u, *v, *p, *cs;
   const double *src_E;
   double conv[4];
+  /* fluid_t%time (advanced by dt per or_integrate) and the time its flux solver started from
+   * (fluid_t%initialize, mod_fluid.f90:169,240): the boundary times of a step derive from both */
+  double time, t0;
 } or_fluid;
 
 /* fluid_t%initialize from primitive interior values (mod_fluid.f90:248-252, 339-363). */

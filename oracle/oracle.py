@@ -27,7 +27,8 @@ FLUX_IDS = {
     "Roe": 5, "S-Roe": 5, "SP-Roe": 5, "AUSM+": 6, "AUSM+-u": 6, "HLLC": 7, "M-AUSMPW+": 8,
 }
 LIMITER_IDS = {"minmod": 0, "superbee": 1, "van_leer": 2, "vanleer": 2, "MLP3": 3, "MLP5": 4}
-BC_IDS = {"periodic": 0, "zero_gradient": 1, "symmetry": 2, "mirror": 2, "reflection": 2, "outflow": 3, "outlet": 3}
+BC_IDS = {"periodic": 0, "zero_gradient": 1, "symmetry": 2, "mirror": 2, "reflection": 2, "outflow": 3, "outlet": 3,
+          "pressure_input": 4}
 RK_IDS = {
     "ssp_rk2": 0, "ssp_rk3": 1, "ssp_rk33": 1,
     "rk(1,3)": 2, "rk(1,4)": 3, "rk(1,5)": 4, "rk(2,3)": 5, "rk(2,4)": 6, "rk(2,5)": 7,
@@ -41,6 +42,8 @@ class _Config(C.Structure):
         ("bc_plus_x", C.c_int), ("bc_minus_x", C.c_int), ("bc_plus_y", C.c_int), ("bc_minus_y", C.c_int),
         ("apply_low_mach_fix", C.c_int),
         ("gamma", C.c_double), ("cfl", C.c_double), ("bc_pressure", C.c_double),
+        ("bc_density", C.c_double), ("pi_n", C.c_int), ("pi_t", C.POINTER(C.c_double)),
+        ("pi_p", C.POINTER(C.c_double)), ("pi_rho", C.POINTER(C.c_double)), ("bc_time", C.c_double),
     ]
 
 
@@ -54,7 +57,7 @@ class _Geometry(C.Structure):
 
 class _Fluid(C.Structure):
     _fields_ = [(n, C.POINTER(C.c_double)) for n in ("rho", "rho_u", "rho_v", "rho_E", "u", "v", "p", "cs")] + \
-        [("src_E", C.POINTER(C.c_double)), ("conv", C.c_double * 4)]
+        [("src_E", C.POINTER(C.c_double)), ("conv", C.c_double * 4), ("time", C.c_double), ("t0", C.c_double)]
 
 
 def build(force: bool = False) -> str:
@@ -93,6 +96,8 @@ def lib():
         L.or_run.argtypes = [C.POINTER(_Config), C.POINTER(_Geometry), C.POINTER(_Fluid), C.c_double, C.c_long,
                              C.c_double, dp, C.POINTER(C.c_int)]
         L.or_run.restype = C.c_long
+        L.or_interp_linear.argtypes = [C.c_int, dp, dp, C.c_double]
+        L.or_interp_linear.restype = C.c_double
         _lib = L
     return _lib
 
@@ -120,9 +125,16 @@ class Config:
 
     def __init__(self, ni, nj, flux="AUSMPW+", limiter="minmod", integrator="ssp_rk2",
                  bcs=("periodic", "periodic", "periodic", "periodic"), gamma=1.4, cfl=0.1,
-                 apply_low_mach_fix=True, n_ghost=None, bc_pressure=0.0):
+                 apply_low_mach_fix=True, n_ghost=None, bc_pressure=0.0, bc_density=0.0, pressure_table=None,
+                 bc_time=0.0):
         self.ni, self.nj = int(ni), int(nj)
         self.bc_pressure = float(bc_pressure)
+        # pressure_input: constant bc_pressure / bc_density, or pressure_table = (time, pressure, density)
+        self.bc_density = float(bc_density)
+        self.bc_time = float(bc_time)
+        self.pressure_table = None
+        if pressure_table is not None:
+            self.pressure_table = tuple(np.ascontiguousarray(a, dtype=np.float64) for a in pressure_table)
         self.flux, self.limiter, self.integrator = flux, limiter, integrator
         self.bcs = tuple(bcs)  # (+x, -x, +y, -y)
         self.gamma, self.cfl = float(gamma), float(cfl)
@@ -134,9 +146,13 @@ class Config:
         return (self.ni + 2 * self.nh, self.nj + 2 * self.nh)
 
     def c(self) -> _Config:
+        tab = self.pressure_table
+        dp = C.POINTER(C.c_double)
         return _Config(self.ni, self.nj, self.nh, FLUX_IDS[self.flux], LIMITER_IDS[self.limiter],
                        RK_IDS[self.integrator], BC_IDS[self.bcs[0]], BC_IDS[self.bcs[1]], BC_IDS[self.bcs[2]],
-                       BC_IDS[self.bcs[3]], int(self.apply_low_mach_fix), self.gamma, self.cfl, self.bc_pressure)
+                       BC_IDS[self.bcs[3]], int(self.apply_low_mach_fix), self.gamma, self.cfl, self.bc_pressure,
+                       self.bc_density, len(tab[0]) if tab else 0, _p(tab[0]) if tab else dp(),
+                       _p(tab[1]) if tab else dp(), _p(tab[2]) if tab else dp(), self.bc_time)
 
     def field(self):
         return np.zeros(self.shape, dtype=np.float64, order="F")
@@ -172,6 +188,13 @@ def uniform_nodes(ni, nj, nh, xrange=(0.0, 1.0), yrange=(0.0, 1.0)):
     y = np.linspace(yrange[0] - nh * dy, yrange[1] + nh * dy, (nj + 1) + 2 * nh, dtype=np.float64)
     y2, x2 = np.meshgrid(y, x)
     return np.asfortranarray(x2), np.asfortranarray(y2)
+
+
+def interp_linear(t, f, x: float) -> float:
+    """linear_interp_1d%evaluate (finterp's interp_1d) at x over knots t."""
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    return lib().or_interp_linear(len(t), _p(t), _p(f), float(x))
 
 
 def apply_bcs(cfg: Config, rho, u, v, p):
@@ -253,6 +276,15 @@ class Fluid:
             self._src = np.asfortranarray(src, dtype=np.float64)
             assert self._src.shape == self.cfg.shape
             self._f.src_E = _p(self._src)
+
+    def set_time(self, t: float):
+        """fluid_t time and the time its flux solver starts from (a fluid initialized at time t)."""
+        self._f.time = float(t)
+        self._f.t0 = float(t)
+
+    @property
+    def time(self) -> float:
+        return float(self._f.time)
 
     def convergence(self):
         """L2-norm convergence [rho, rho u, rho v, rho E] of the last integrate()."""

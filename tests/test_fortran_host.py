@@ -76,7 +76,7 @@ def test_host_reads_outflow_deck(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     cfg_line = [l for l in r.stdout.splitlines() if l.startswith("config:")][0]
     assert cfg_line.split("=")[1].split()[3:7] == ["3", "1", "0", "0"]
-    assert float([l for l in r.stdout.splitlines() if l.startswith("bc_pressure")][0].split("=")[1]) == 0.25
+    assert float([l for l in r.stdout.splitlines() if l.startswith("bc_pressure")][0].split("=")[1].split()[0]) == 0.25
 
 
 @pytest.mark.parametrize("extra, message", [
