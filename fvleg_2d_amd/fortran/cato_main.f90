@@ -127,10 +127,6 @@ program cato_hip_host
   restart_iteration = 0
   if (input%restart_from_file) then
     ! main.f90:100-104 + master_puppeteer.f90:106-121: state, time and iteration from the restart file
-    if (device_loop) then
-      write(*, '(a)') 'Error: --device-loop starts at time 0; a restart needs the host-side loop'
-      error stop 1
-    end if
     call read_restart(trim(input%restart_file), restart_time, restart_iteration, from_checkpoint, rho, u, v, p)
     if (from_checkpoint) then
       call fluid%initialize_conserved(dev, rho, u, v, p, restart_time, restart_iteration)
@@ -177,10 +173,11 @@ program cato_hip_host
     write(hist_unit, '(a)') 'iteration,time,dt,rho,rho_u,rho_v,rho_E'
   end if
   if (device_loop) then
-    call check_rc(cato_run(dev%ctx, input%max_time, int(input%max_iterations, c_long_long), contour_interval_dt, &
-                           merge(input%initial_delta_t, 0.0_rk, input%use_constant_delta_t), iters, time, e), &
-                  'cato_run')
-    iteration = int(iters)
+    ! the device loop starts at the fluid time (the restart time) and counts its own iterations
+    call check_rc(cato_run(dev%ctx, input%max_time, int(input%max_iterations - restart_iteration, c_long_long), &
+                           contour_interval_dt, merge(input%initial_delta_t, 0.0_rk, input%use_constant_delta_t), &
+                           iters, time, e), 'cato_run')
+    iteration = restart_iteration + int(iters)
     error_code = e
     if (residual_hist) call drain_history()
   else
@@ -283,7 +280,7 @@ contains
     do
       call check_rc(cato_get_history(dev%ctx, rows, 256_c_long_long, n), 'cato_get_history')
       do r = 1, int(n)
-        write(hist_unit, '(i0, ",", 6(es16.6, ","))') nint(rows(1, r)), rows(2:7, r)
+        write(hist_unit, '(i0, ",", 6(es16.6, ","))') restart_iteration + nint(rows(1, r)), rows(2:7, r)
       end do
       if (n < 256) exit
     end do

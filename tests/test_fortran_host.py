@@ -79,6 +79,37 @@ def test_host_reads_outflow_deck(tmp_path):
     assert float([l for l in r.stdout.splitlines() if l.startswith("bc_pressure")][0].split("=")[1].split()[0]) == 0.25
 
 
+def drive_problem(tmp_path, scale=2.0):
+    """A gas at rest driven from +x by the pressure_input boundary with a pulse from a table file."""
+    pb = sod_problem(ni=120, nj=4, flux="AUSM+", limiter="minmod", integrator="ssp_rk3", cfl=0.4, max_time=0.15)
+    pb.cfg.bcs = ("pressure_input", "zero_gradient", "periodic", "periodic")
+    pb.rho[...] = 1.0
+    pb.p[...] = 0.1
+    table = np.array([[0.0, 0.5, 0.25], [0.05, 0.5, 0.25], [0.06, 0.1, 0.25], [1.0, 0.1, 0.25]])
+    path = os.path.join(str(tmp_path), "drive.dat")
+    with open(path, "w") as fh:
+        fh.write("# time [s]  pressure  density\n")
+        fh.writelines(f"{float(t)!r} {float(p)!r}   {float(r)!r}\n" for t, p, r in table)
+    return pb, path, (table[:, 0].copy(), (table[:, 1] * 1.0) * scale, (table[:, 2] * 1.0) * scale)
+
+
+def test_host_reads_pressure_input_deck(tmp_path):
+    """'pressure_input' at +x: the constant mode's constant_bc_pressure_value and bc_density reach the
+    library's config (mod_input.f90:366-382); the boundary id is 4."""
+    from fvleg_2d_amd.deck import write_deck
+
+    pb, _, _ = drive_problem(tmp_path)
+    pb.cfg.bc_pressure, pb.cfg.bc_density = 0.75, 0.125
+    ini = write_deck(pb, str(tmp_path), "drive")
+    r = subprocess.run([host_or_fail(), ini, "--dry-run"], capture_output=True, text=True, timeout=60,
+                       cwd=str(tmp_path))
+    assert r.returncode == 0, r.stdout + r.stderr
+    cfg_line = [l for l in r.stdout.splitlines() if l.startswith("config:")][0]
+    assert cfg_line.split("=")[1].split()[3:7] == ["4", "1", "0", "0"]
+    vals = [l for l in r.stdout.splitlines() if l.startswith("bc_pressure")][0].split("=")[1].split()
+    assert [float(x) for x in vals] == [0.75, 0.125]
+
+
 @pytest.mark.parametrize("extra, message", [
     ("[time]\nallow_subcycle = true\n", "allow_subcycle = true is not supported"),
     ("[source_terms]\nenable_source_terms = true\n", "[source_terms] from the deck are not supported"),
@@ -268,9 +299,12 @@ def run_host(args, cwd):
 
 
 @pytest.mark.gpu
-def test_host_restart_from_checkpoint_is_bit_exact(tmp_path):
+@pytest.mark.parametrize("device_loop", [False, True])
+def test_host_restart_from_checkpoint_is_bit_exact(tmp_path, device_loop):
     """[restart] (main.f90:100-107, master_puppeteer.f90:106-121) from the conserved checkpoint the host writes:
-    40 steps, then a restart to t_end, lands on exactly the state, time and iteration of the uninterrupted run."""
+    40 steps, then a restart to t_end, lands on exactly the state, time and iteration of the uninterrupted run.
+    The device-side loop resumes from the restart time and iteration too (its driver clock starts at the
+    fluid time); the residual history continues the iteration count."""
     from fvleg_2d_amd.deck import read_state, write_deck
 
     pb = sod_problem(ni=200, nj=4, flux="SLAU", limiter="minmod", integrator="ssp_rk3", cfl=0.4, max_time=0.1)
@@ -282,8 +316,10 @@ def test_host_restart_from_checkpoint_is_bit_exact(tmp_path):
     rest = write_deck(pb, d, "rest")
     with open(rest, "a") as fh:
         fh.write(f"\n[restart]\nrestart_from_file = true\nrestart_file = '{os.path.join(d, 'b1.h5')}'\n")
-    out = run_host([rest, os.path.join(d, "b2.h5")], d)
+    out = run_host([rest, os.path.join(d, "b2.h5")] + (["--device-loop"] if device_loop else []), d)
     assert "Restart from" in out and "iteration 40" in out
+    hist = read_residual_hist(os.path.join(d, "residual_hist.csv"))
+    assert hist[0, 0] == 41 and hist[-1, 0] == read_state(os.path.join(d, "b2.h5"))["iteration"]
     a, b1, b2 = (read_state(os.path.join(d, n)) for n in ("a.h5", "b1.h5", "b2.h5"))
     assert b1["iteration"] == 40 and a["iteration"] > 40
     # same steps; the driver time differs by the reference's own rule: a restart sets next_output_time to
@@ -337,3 +373,34 @@ def test_host_restart_from_contour_matches_oracle(tmp_path, oracle):
     assert got["iteration"] == it and got["time"] == time
     for key, name in (("density", "rho"), ("rho_u", "rho_u"), ("rho_v", "rho_v"), ("rho_E", "rho_E")):
         assert np.array_equal(got[key], f.interior(name)), key
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("device_loop", [False, True])
+def test_host_pressure_input_file_matches_oracle(tmp_path, oracle, device_loop):
+    """A time-dependent pressure drive read from its file (header line, three columns, scale factor 2 on
+    pressure and density, pressure_input_bc.f90:93-178) through the host, host- and device-side loops,
+    against the oracle's main.f90 loop; within 1e-12 (the inlet rows' pow)."""
+    from fvleg_2d_amd.deck import read_state, write_deck
+
+    pb, path, table = drive_problem(tmp_path, scale=2.0)
+    ini = write_deck(pb, str(tmp_path), "drive", pressure_input_file=path, pressure_scale_factor=2.0)
+    out = os.path.join(str(tmp_path), "final.h5")
+    cmd = [host_or_fail(), ini, out, "--no-residual-hist"] + (["--device-loop"] if device_loop else [])
+    print("host:", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    print(r.stdout[-1500:], flush=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = read_state(out)
+    cfg = pb.cfg
+    nh = cfg.n_ghost
+    oc = oracle.Config(pb.ni, pb.nj, flux=cfg.flux, limiter=cfg.limiter, integrator=cfg.integrator, bcs=cfg.bcs,
+                       gamma=cfg.gamma, cfl=cfg.cfl, pressure_table=table)
+    f = oracle.Fluid(oc, oracle.Geometry(oc, pb.node_x, pb.node_y),
+                     *[np.pad(a, nh) for a in (pb.rho, pb.u, pb.v, pb.p)])
+    n, t, e = f.run(pb.max_time, 2 ** 31 - 1, pb.max_time)
+    assert e == 0 and got["iteration"] == n and abs(got["time"] - t) <= 1e-12 * t
+    for key, name in (("density", "rho"), ("rho_u", "rho_u"), ("rho_v", "rho_v"), ("rho_E", "rho_E")):
+        ref = f.interior(name)
+        assert np.max(np.abs(got[key] - ref)) <= 1e-12 * np.max(np.abs(ref)), key
+    assert np.max(f.interior("rho")) > 1.5  # the drive launched a shock into the gas
