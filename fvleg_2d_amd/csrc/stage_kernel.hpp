@@ -149,10 +149,13 @@ template <bool NT> __device__ inline void st_cell(double *p, double x) {
 // fill included.  The form stores no primitive arrays: 64 B/cell less HBM
 // traffic per stage for the price of the recovery on halo cells.
 __device__ inline Prim<double> load_recovered(const StageArgs &a, int gi, int gj) {
+  bool fu, fv;
   // the pressure input's ghosts need the whole column (its largest pressure):
   // a pass before the stage stores them in column ni+1 of the primitive arrays
-  if (gi > a.ni && a.bc[0] == BC_PRESSURE_INPUT) return load_prim(a, gi, gj);
-  bool fu, fv;
+  if (gi > a.ni && a.bc[0] == BC_PRESSURE_INPUT) {
+    size_t off = (size_t)(map_axis(gj, a.nj, a.bc[2], a.bc[3], fv) - 1) * a.pitch + (size_t)a.ni;
+    return Prim<double>{a.prim[0][off], a.prim[1][off], a.prim[2][off], a.prim[3][off]};
+  }
   int si = map_axis(gi, a.ni, a.bc[0], a.bc[1], fu);
   int sj = map_axis(gj, a.nj, a.bc[2], a.bc[3], fv);
   size_t off = (size_t)(sj - 1) * a.pitch + (size_t)(si - 1);
