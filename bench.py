@@ -219,6 +219,23 @@ def roofline(pb, stage_avg_ms, launches, precision, n):
     }
 
 
+def valu_figure(stage_avg_ms, precision, n, tile):
+    """Vector-issue utilisation of the stage kernel: the VALU wave-instructions one launch issues (SQ_INSTS_VALU,
+    profiles/stage_kernel_counters.json, counted per tile for the same workload) over this run's mean launch time,
+    against 1024 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction.  None when no count matches."""
+    cpath = os.path.join(ROOT, "profiles", "stage_kernel_counters.json")
+    try:
+        c = json.load(open(cpath)).get(f"{n}_{precision}", {}).get(str(tile))
+    except Exception:
+        c = None
+    if not c:
+        return None
+    achieved = c["valu_insts_per_dispatch"] / (stage_avg_ms * 1e-3)
+    peak = 1024 * 2.4e9 / 4
+    return {"achieved": achieved, "peak": peak, "unit": "wave-instructions/s", "frac": achieved / peak,
+            "insts_per_launch": c["valu_insts_per_dispatch"], "source": "profiles/stage_kernel_counters.json"}
+
+
 def riemann_forms():
     """Config: 1024^2 four-quadrant Riemann, HLLC + MLP5, SSP-RK3, CFL 0.4, to t = 0.25."""
     from fvleg_2d_amd import Solver
@@ -294,6 +311,7 @@ def main():
             "autotune": pb.meta.get("autotune"),
         },
         "roofline": roofline(pb, stage_ms, launches, args.precision, args.n),
+        "valu": valu_figure(stage_ms, args.precision, args.n, pb.cfg.tile),
     }
     if args.forms == "all" and args.precision == "fp64":
         say("[bench] precision form (FP32 flux, FP64 state) on the same workload")
@@ -305,6 +323,7 @@ def main():
             "tile": pb32.cfg.tile,
             "autotune": pb32.meta.get("autotune"),
             "roofline": roofline(pb32, ms32, l32, "fp32_flux", args.n),
+            "valu": valu_figure(ms32, "fp32_flux", args.n, pb32.cfg.tile),
             "difference_from_fp64_state": rel_errors(state, state32),
         }
         say("[bench] one GPU's share of the 2048^2 solve split over 8 GPUs: 512x1024")
