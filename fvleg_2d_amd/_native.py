@@ -56,6 +56,8 @@ SIGNATURES = {
     "cato_set_energy_source": (C.c_int, [_CTX, _DP, C.c_int, C.c_int]),
     "cato_set_pressure_input": (C.c_int, [_CTX, _DP, _DP, _DP, C.c_int]),
     "cato_set_time": (C.c_int, [_CTX, C.c_double]),
+    "cato_set_subcycle": (C.c_int, [_CTX, C.c_int, C.c_double]),
+    "cato_last_dt": (C.c_int, [_CTX, _DP]),
     "cato_get_primitive": (C.c_int, [_CTX, _DP, _DP, _DP, _DP, _DP, C.c_int, C.c_int]),
     "cato_compute_dt": (C.c_int, [_CTX, _DP]),
     "cato_step": (C.c_int, [_CTX, C.c_double, C.POINTER(C.c_int)]),

@@ -283,7 +283,7 @@ static void prim_cycle(int m, int ns, int &pin, int &pout) {
 // ssp_rk_3_3 (773-802) and rk_multistage (957-966).  State buffers: cur holds
 // U^n, the two others hold intermediate stages; the last stage overwrites U^n
 // in place (it reads U^n only at its own cell).
-std::vector<StageArgs> Solve::step_stages(const LoopState *lp, double dt_value) const {
+std::vector<StageArgs> Solve::step_stages(const LoopState *lp, double dt_value, bool keep_un) const {
   std::vector<StageArgs> st;
   const int n = cur, A = (cur + 1) % 3, B = (cur + 2) % 3;
   // The first stage after cato_set_primitive reads the caller's primitives
@@ -295,7 +295,9 @@ std::vector<StageArgs> Solve::step_stages(const LoopState *lp, double dt_value) 
       prim_cycle(0, 2, pi, po);
       st.push_back(stage_args(MODE_FIRST, 0, 0, 1.0, n, -1, A, pi, po, false, lp, dt_value, chk));
       prim_cycle(1, 2, pi, po);
-      st.push_back(stage_args(MODE_COMBINE, 0.5, 0.5, 0.5, A, n, n, pi, po, true, lp, dt_value, false));
+      // keep_un: U^(n+1) into B and its primitives into P[2], both unused by the step
+      st.push_back(stage_args(MODE_COMBINE, 0.5, 0.5, 0.5, A, n, keep_un ? B : n, pi, keep_un ? 2 : po, true, lp,
+                              dt_value, false));
       break;
     case CATO_RK_SSP3:
       prim_cycle(0, 3, pi, po);
@@ -304,8 +306,10 @@ std::vector<StageArgs> Solve::step_stages(const LoopState *lp, double dt_value) 
       st.push_back(
           stage_args(MODE_COMBINE, 3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0, A, n, B, pi, po, false, lp, dt_value, false));
       prim_cycle(2, 3, pi, po);
-      st.push_back(
-          stage_args(MODE_COMBINE, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, B, n, n, pi, po, true, lp, dt_value, false));
+      // keep_un: U^(n+1) into A (U_1 there is read only at the cell, before the write, for the
+      // convergence norms) and its primitives into P[1], which stage 2 has read
+      st.push_back(stage_args(MODE_COMBINE, 1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0, B, n, keep_un ? A : n, pi,
+                              keep_un ? 1 : po, true, lp, dt_value, false));
       break;
     default: {
       static const double a13[3] = {0.1481, 0.4, 1.0}, a14[4] = {0.0833, 0.2069, 0.4265, 1.0},
@@ -462,6 +466,14 @@ struct cato_ctx {
   // loop->master_time until read back) and the time the solve started from
   double t_fluid = 0.0, t0 = 0.0;
   bool t_on_device = false;
+  // the clock of the flux solver the first SSP stage calls (it gains the dt of
+  // every attempt, retried ones included; the fluid time only the kept one's)
+  double t_solver = 0.0;
+  // subcycle_check: retry a cato_step with dt / 10 while a norm exceeds the tolerance
+  bool subcycle = false;
+  double subcycle_tol = 1e-3;
+  double last_dt = 0.0;
+  bool hist_on = false;  // the caller asked for convergence rows (cato_set_diagnostics)
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<std::pair<hipEvent_t, hipEvent_t>, int>> pending;  // (start, stop), is_stage
   // kGraphSteps steps of the device-side loop, captured once and replayed by
@@ -541,7 +553,7 @@ PressureInput pressure_args(const cato_ctx *c, int mode, int m, const LoopState 
   a.mode = mode;
   const bool multistage = s.cfg.integrator != CATO_RK_SSP2 && s.cfg.integrator != CATO_RK_SSP3;
   a.first_multistage = (mode == 1 && multistage && m == 0) ? 1 : 0;
-  a.t_at = c->t_fluid;
+  a.t_at = (mode == 1 && !multistage && m == 0) ? c->t_solver : c->t_fluid;
   a.dt_at = dt;
   a.t0 = c->t0;
   a.loop = loop;
@@ -639,6 +651,7 @@ int sync_fluid_time(cato_ctx *c) {
   HIPCK(hipMemcpyAsync(&H, c->s.loop, sizeof(H), hipMemcpyDeviceToHost, c->stream));
   HIPCK(hipStreamSynchronize(c->stream));
   c->t_fluid = H.master_time;
+  c->t_solver = H.master_time;
   c->t_on_device = false;
   return CATO_OK;
 }
@@ -762,6 +775,7 @@ int cato_set_primitive(cato_ctx *c, const double *rho, const double *u, const do
   s.prims_stale = false;
   c->diag_iter = 0;
   c->t_fluid = c->t0;
+  c->t_solver = c->t0;
   c->t_on_device = false;
   return hip_check(hipStreamSynchronize(c->stream), "set_primitive");
 }
@@ -781,6 +795,7 @@ int cato_set_conserved(cato_ctx *c, const double *r, const double *ru, const dou
   s.prims_stale = false;
   c->diag_iter = 0;
   c->t_fluid = c->t0;
+  c->t_solver = c->t0;
   c->t_on_device = false;
   return hip_check(hipStreamSynchronize(c->stream), "set_conserved");
 }
@@ -840,6 +855,7 @@ int cato_set_time(cato_ctx *c, double time) {
   drop_steps_graph(c);  // the pressure-input passes carry the start time
   c->t0 = time;
   c->t_fluid = time;
+  c->t_solver = time;
   c->t_on_device = false;
   return CATO_OK;
 }
@@ -893,16 +909,37 @@ int cato_step(cato_ctx *c, double dt, int *error_code) {
   if (!(dt >= 2.2250738585072014e-308))
     return fail(CATO_ERR_INVALID, "The timestep dt is < tiny(1.0_rk)");  // slau_solver.fypp:162-166
   if ((rc = sync_fluid_time(c))) return rc;
-  if ((rc = launch_stages(c, s.step_stages(nullptr, dt)))) return rc;
-  s.prims_valid = false;
-  if ((rc = launch_reduce(c, 0, true))) return rc;
+  const bool ssp = s.cfg.integrator == CATO_RK_SSP2 || s.cfg.integrator == CATO_RK_SSP3;
+  // ssp_rk_2_2 / ssp_rk_3_3 with subcycle_check (mod_fluid.f90:816-823,856-887): attempts keep U^n and
+  // P[0] intact, and the kept one's buffers become U^n's; rk_multistage has no retry loop
+  const bool sub = c->subcycle && ssp;
   double r[2], nrm[4];
-  HIPCK(hipMemcpyAsync(r, s.result, sizeof(r), hipMemcpyDeviceToHost, c->stream));
-  if (c->diag) HIPCK(hipMemcpyAsync(nrm, s.conv_norms, sizeof(nrm), hipMemcpyDeviceToHost, c->stream));
-  HIPCK(hipStreamSynchronize(c->stream));
-  // fluid_t%integrate: time + dt (ssp_rk_3_3:844), iteration + 1 (mod_fluid.f90:422)
+  for (;;) {
+    std::vector<StageArgs> st = s.step_stages(nullptr, dt, sub);
+    if ((rc = launch_stages(c, st))) return rc;
+    if (ssp) c->t_solver = c->t_solver + dt;  // the first stage's solver: time + dt per solve call
+    s.prims_valid = false;
+    if ((rc = launch_reduce(c, 0, true))) return rc;
+    HIPCK(hipMemcpyAsync(r, s.result, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+    if (c->diag) HIPCK(hipMemcpyAsync(nrm, s.conv_norms, sizeof(nrm), hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    if (!sub) break;
+    bool retry = false;
+    for (int v = 0; v < 4; ++v) retry = retry || nrm[v] > c->subcycle_tol;
+    if (!retry) {
+      const int kept = s.cfg.integrator == CATO_RK_SSP2 ? (s.cur + 2) % 3 : (s.cur + 1) % 3;
+      const int kept_p = s.cfg.integrator == CATO_RK_SSP2 ? 2 : 1;
+      s.cur = kept;
+      for (int v = 0; v < 4; ++v) std::swap(s.P[0][v], s.P[kept_p][v]);
+      drop_steps_graph(c);  // a captured device loop holds the old buffer roles
+      break;
+    }
+    dt = dt / 10.0;
+  }
+  // fluid_t%integrate: time + dt (ssp_rk_3_3:844), iteration + 1 (mod_fluid.f90:422), and the dt it used
+  c->last_dt = dt;
   c->t_fluid = c->t_fluid + dt;
-  if (c->diag) {
+  if (c->hist_on) {
     c->diag_iter += 1;
     const double row[kHistRow] = {(double)c->diag_iter, c->t_fluid, dt, nrm[0], nrm[1], nrm[2], nrm[3]};
     c->history.insert(c->history.end(), row, row + kHistRow);
@@ -965,6 +1002,8 @@ static int enqueue_steps(cato_ctx *c, long long n) {
 static int run_impl(cato_ctx *c, double max_time, long long max_iter, double contour_dt, double constant_dt,
                     bool poll, long long fixed_steps, long long *iters, double *time, int *err) {
   Solve &s = c->s;
+  if (c->subcycle && (s.cfg.integrator == CATO_RK_SSP2 || s.cfg.integrator == CATO_RK_SSP3))
+    return fail(CATO_ERR_UNSUPPORTED, "subcycling (allow_subcycle) runs in host-driven steps (cato_step) only");
   int rc = sync_fluid_time(c);
   if (rc) return rc;
   // main.f90:97-102: the driver clock starts at the fluid's time
@@ -991,7 +1030,7 @@ static int run_impl(cato_ctx *c, double max_time, long long max_iter, double con
     long long seen = H.iteration;
     HIPCK(hipMemcpyAsync(&H, s.loop, sizeof(H), hipMemcpyDeviceToHost, c->stream));
     std::vector<double> ring;
-    if (c->diag) {
+    if (c->hist_on) {
       ring.resize((size_t)Solve::kHistCap * kHistRow);
       HIPCK(hipMemcpyAsync(ring.data(), s.conv_hist, ring.size() * sizeof(double), hipMemcpyDeviceToHost,
                            c->stream));
@@ -999,7 +1038,7 @@ static int run_impl(cato_ctx *c, double max_time, long long max_iter, double con
     HIPCK(hipStreamSynchronize(c->stream));
     if (c->timing) collect_timing(c);
     // rows of the steps since the last poll (at most `chunk` < kHistCap of them)
-    for (long long it = seen + 1; c->diag && it <= H.iteration; ++it) {
+    for (long long it = seen + 1; c->hist_on && it <= H.iteration; ++it) {
       const double *row = &ring[(size_t)((it - 1) % Solve::kHistCap) * kHistRow];
       c->history.push_back((double)(c->diag_iter + it));
       c->history.push_back(row[1]);
@@ -1013,6 +1052,7 @@ static int run_impl(cato_ctx *c, double max_time, long long max_iter, double con
   }
   c->diag_iter += H.iteration;
   c->t_fluid = H.master_time;
+  c->t_solver = H.master_time;
   if (iters) *iters = H.iteration;
   if (time) *time = H.time;
   if (err) *err = H.error;
@@ -1044,6 +1084,8 @@ int cato_synchronize(cato_ctx *c) {
   return CATO_OK;
 }
 
+static int set_conv_buffers(cato_ctx *c, bool need);
+
 int cato_set_diagnostics(cato_ctx *c, int enable) {
   int rc = check_ctx(c);
   if (rc) return rc;
@@ -1052,7 +1094,15 @@ int cato_set_diagnostics(cato_ctx *c, int enable) {
   drop_steps_graph(c);  // the last stage's arguments carry the diagnostics buffers
   c->history.clear();
   c->diag_iter = 0;
-  if (!enable) {
+  c->hist_on = enable != 0;
+  return set_conv_buffers(c, c->hist_on || c->subcycle);
+}
+
+// The convergence sums, norms and history ring (one device allocation), kept
+// while the diagnostics or the subcycling need them.
+static int set_conv_buffers(cato_ctx *c, bool need) {
+  Solve &s = c->s;
+  if (!need) {
     if (c->diag) (void)hipFree(c->diag);
     c->diag = nullptr;
     s.conv_sums = s.conv_norms = s.conv_hist = nullptr;
@@ -1067,6 +1117,25 @@ int cato_set_diagnostics(cato_ctx *c, int enable) {
   s.conv_norms = c->diag + 8 * nb;
   s.conv_hist = s.conv_norms + 4;
   return hip_check(hipStreamSynchronize(c->stream), "set_diagnostics");
+}
+
+int cato_set_subcycle(cato_ctx *c, int enable, double tolerance) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (enable && !(tolerance > 0.0)) return fail(CATO_ERR_INVALID, "subcycle_error_tolerance must be > 0");
+  HIPCK(hipStreamSynchronize(c->stream));
+  drop_steps_graph(c);
+  c->subcycle = enable != 0;
+  if (c->subcycle) c->subcycle_tol = tolerance;
+  return set_conv_buffers(c, c->hist_on || c->subcycle);
+}
+
+int cato_last_dt(cato_ctx *c, double *dt) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (!dt) return fail(CATO_ERR_INVALID, "null output pointer");
+  *dt = c->last_dt;
+  return CATO_OK;
 }
 
 int cato_get_convergence(cato_ctx *c, double *norms) {

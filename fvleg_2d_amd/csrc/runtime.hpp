@@ -58,7 +58,9 @@ struct Solve {
   FieldGeom fgeom() const;
   StageArgs stage_args(int mode, double a, double b, double c, int in_buf, int u0_buf, int out_buf, int p_in,
                        int p_out, bool last, const LoopState *lp, double dt_value, bool check_input) const;
-  std::vector<StageArgs> step_stages(const LoopState *lp, double dt_value) const;
+  // keep_un: the last stage writes a free state buffer and primitive set instead
+  // of U^n and P[0] (a step that may be retried from U^n, subcycle_check)
+  std::vector<StageArgs> step_stages(const LoopState *lp, double dt_value, bool keep_un = false) const;
   ConvOut conv_out() const { return ConvOut{conv_sums, conv_norms, conv_hist, kHistCap}; }
 };
 

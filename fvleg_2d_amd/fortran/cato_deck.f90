@@ -28,6 +28,7 @@ module cato_deck
     logical :: enable_startup_time = .false.
     real(rk) :: startup_period = 0.0_rk, startup_delta_t = 0.0_rk
     logical :: allow_subcycle = .false.
+    real(rk) :: subcycle_error_tolerance = 1e-3_rk
     ! [source_terms]
     logical :: enable_source_terms = .false.
     ! [restart]
@@ -339,6 +340,7 @@ contains
       case ('time.startup_period'); inp%startup_period = to_real(val, key)
       case ('time.startup_delta_t'); inp%startup_delta_t = to_real(val, key)
       case ('time.allow_subcycle'); inp%allow_subcycle = to_logical(val)
+      case ('time.subcycle_error_tolerance'); inp%subcycle_error_tolerance = to_real(val, key)
       case ('source_terms.enable_source_terms'); inp%enable_source_terms = to_logical(val)
       case ('restart.restart_from_file'); inp%restart_from_file = to_logical(val)
       case ('restart.restart_file'); inp%restart_file = val

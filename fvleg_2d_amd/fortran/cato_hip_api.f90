@@ -43,7 +43,7 @@ module cato_hip_api
             cato_get_conserved, cato_get_primitive, cato_set_energy_source, cato_set_diagnostics, &
             cato_get_convergence, cato_get_history, cato_compute_dt, cato_step, cato_run, cato_apply_bc, &
             cato_interpolate_edges, cato_edge_fluxes, cato_residual, cato_error_message, &
-            cato_set_pressure_input, cato_set_time
+            cato_set_pressure_input, cato_set_time, cato_set_subcycle, cato_last_dt
 
   interface
     function cato_abi_version() bind(C, name="cato_abi_version") result(v)
@@ -126,6 +126,23 @@ module cato_hip_api
       type(c_ptr), value :: ctx
       real(c_double), intent(in) :: time(*), pressure(*), density(*)
       integer(c_int), value :: n
+      integer(c_int) :: rc
+    end function
+
+    !> [time] allow_subcycle / subcycle_error_tolerance (host-driven steps)
+    function cato_set_subcycle(ctx, enable, tolerance) bind(C, name="cato_set_subcycle") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: enable
+      real(c_double), value :: tolerance
+      integer(c_int) :: rc
+    end function
+
+    !> the dt the last cato_step used
+    function cato_last_dt(ctx, dt) bind(C, name="cato_last_dt") result(rc)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: dt
       integer(c_int) :: rc
     end function
 

@@ -80,10 +80,6 @@ program cato_hip_host
     write(*, '(a)') 'Error: reference_state scaling other than 1 is not supported by this host'
     error stop 1
   end if
-  if (input%allow_subcycle) then
-    write(*, '(a)') 'Error: allow_subcycle = true is not supported (no reference deck sets it)'
-    error stop 1
-  end if
   if (input%enable_source_terms) then
     write(*, '(a)') 'Error: [source_terms] from the deck are not supported; the library takes the step''s '// &
       'source field through fluid%set_energy_source'
@@ -117,6 +113,14 @@ program cato_hip_host
     stop
   end if
   call dev%create(cfg, grid)
+  if (input%allow_subcycle) then
+    ! fluid_t%initialize (mod_fluid.f90:171-172); ssp_rk_2_2 / ssp_rk_3_3 retry a step with dt / 10
+    if (device_loop .and. (cfg%integrator == 0 .or. cfg%integrator == 1)) then
+      write(*, '(a)') 'Error: --device-loop does not support allow_subcycle (its retries are host-driven)'
+      error stop 1
+    end if
+    call check_rc(cato_set_subcycle(dev%ctx, 1_c_int, input%subcycle_error_tolerance), 'cato_set_subcycle')
+  end if
   if (trim(input%plus_x) == 'pressure_input' .and. .not. input%apply_constant_bc_pressure) then
     ! pressure_input_bc_constructor: the time-dependent input from its file
     call read_pressure_input(trim(input%bc_pressure_input_file), input%bc_pressure_scale_factor, pi_t, pi_p, pi_r)

@@ -350,12 +350,16 @@ contains
   end function
 
   !> fluid_t%integrate (mod_fluid.f90:411-450): one step of the configured scheme
+  !> dt returns the step's dt: smaller than given when subcycling retried it (mod_fluid.f90:411-448)
   subroutine fluid_integrate(self, dt, error_code)
     class(hip_fluid_t), intent(inout) :: self
-    real(rk), intent(in) :: dt
+    real(rk), intent(inout) :: dt
     integer, intent(out) :: error_code
     integer(c_int) :: e
+    real(c_double) :: used
     call check(cato_step(self%dev%ctx, real(dt, c_double), e), 'fluid%integrate')
+    call check(cato_last_dt(self%dev%ctx, used), 'fluid%integrate')
+    dt = used
     error_code = e
     self%time = self%time + dt
     self%iteration = self%iteration + 1

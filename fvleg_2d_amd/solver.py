@@ -129,6 +129,17 @@ class Solver:
             raise ValueError("time, pressure and density need the same length")
         N.check(self.lib.cato_set_pressure_input(self.h, _p(t), _p(p), _p(r), len(t)), "cato_set_pressure_input")
 
+    def set_subcycle(self, on: bool, tolerance: float = 1e-3):
+        """[time] allow_subcycle / subcycle_error_tolerance: step() retries with dt / 10 while a convergence
+        norm exceeds the tolerance (SSP-RK2/3, host-driven steps)."""
+        N.check(self.lib.cato_set_subcycle(self.h, int(on), float(tolerance)), "cato_set_subcycle")
+
+    def last_dt(self) -> float:
+        """The dt the last step() used."""
+        dt = C.c_double()
+        N.check(self.lib.cato_last_dt(self.h, C.byref(dt)), "cato_last_dt")
+        return dt.value
+
     def set_time(self, t: float):
         """Fluid time (and the time the solve starts from): fluid_t%set_time on an initialized fluid."""
         N.check(self.lib.cato_set_time(self.h, float(t)), "cato_set_time")

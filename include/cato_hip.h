@@ -199,6 +199,14 @@ int cato_get_convergence(cato_ctx *ctx, double norms[4]);
  * reports the number of rows waiting in *n_rows. */
 int cato_get_history(cato_ctx *ctx, double *rows, long long max_rows, long long *n_rows);
 
+/* [time] allow_subcycle / subcycle_error_tolerance (ssp_rk_2_2 / ssp_rk_3_3 with
+ * subcycle_check, mod_fluid.f90:816-823,856-887; deck keys mod_input.f90:255-256):
+ * while any convergence norm of a cato_step exceeds the tolerance, the step is
+ * recomputed from the same state with dt / 10.  SSP-RK2/3 only (rk_multistage has
+ * no retry loop); host-driven steps only (cato_run / cato_advance refuse an SSP
+ * context with it on).  cato_last_dt: the dt the last cato_step used (fluid_t%dt). */
+int cato_set_subcycle(cato_ctx *ctx, int enable, double tolerance);
+int cato_last_dt(cato_ctx *ctx, double *dt);
 /* Per-launch timing of the fused stage kernel, measured with HIP events on the
  * context's stream while enabled. */
 int cato_set_timing(cato_ctx *ctx, int enable);

@@ -1325,7 +1325,8 @@ static void convergence(const or_config *c, const stage_buf *first, const stage_
  *   later stages U_prev = U_curr (assign_fluid, U_curr led by U_0 = U): time + dt. */
 static double stage_bc_time(const or_config *c, const or_fluid *f, double dt, int stage) {
   int multistage = c->integrator != OR_RK_SSP2 && c->integrator != OR_RK_SSP3;
-  return (multistage && stage == 0) ? f->t0 + dt : f->time + dt;
+  if (stage == 0) return multistage ? f->t0 + dt : f->t_solver + dt;
+  return f->time + dt;
 }
 
 int or_integrate(const or_config *c_in, const or_geometry *g, or_fluid *f, double dt) {
@@ -1342,39 +1343,43 @@ int or_integrate(const or_config *c_in, const or_geometry *g, or_fluid *f, doubl
   fluid_to_stage(f, &U, n);
   int err = 0;
   stage_buf *fin = NULL;
-  if (c->integrator == OR_RK_SSP2) {
-    /* ssp_rk_2_2 (mod_fluid.f90:1115-1129) */
-    fluid_to_stage(f, &U1, n);
-    cb.bc_time = stage_bc_time(c, f, dt, 0);
-    time_derivative(c, g, &U, L, f->src_E);
-    combine1(c, &U, L, dt, &U1);
-    err |= assign_stage(c, &U1);
-    fluid_to_stage(f, &U2, n);
-    cb.bc_time = stage_bc_time(c, f, dt, 1);
-    time_derivative(c, g, &U1, L, f->src_E);
-    combine2(c, &U, 0.5, &U1, 0.5, L, 0.5 * dt, &U2);
-    err |= assign_stage(c, &U2);
-    convergence(c, &U1, &U2, f->conv); /* mod_fluid.f90:1132 */
-    fin = &U2;
-  } else if (c->integrator == OR_RK_SSP3) {
-    /* ssp_rk_3_3 (mod_fluid.f90:773-802) */
-    fluid_to_stage(f, &U1, n);
-    cb.bc_time = stage_bc_time(c, f, dt, 0);
-    time_derivative(c, g, &U, L, f->src_E);
-    combine1(c, &U, L, dt, &U1);
-    err |= assign_stage(c, &U1);
-    fluid_to_stage(f, &U2, n);
-    cb.bc_time = stage_bc_time(c, f, dt, 1);
-    time_derivative(c, g, &U1, L, f->src_E);
-    combine2(c, &U, 3.0 / 4.0, &U1, 1.0 / 4.0, L, (1.0 / 4.0) * dt, &U2);
-    err |= assign_stage(c, &U2);
-    fluid_to_stage(f, &U3, n);
-    cb.bc_time = stage_bc_time(c, f, dt, 2);
-    time_derivative(c, g, &U2, L, f->src_E);
-    combine2(c, &U, 1.0 / 3.0, &U2, 2.0 / 3.0, L, (2.0 / 3.0) * dt, &U3);
-    err |= assign_stage(c, &U3);
-    convergence(c, &U1, &U3, f->conv); /* mod_fluid.f90:805 */
-    fin = &U3;
+  if (c->integrator == OR_RK_SSP2 || c->integrator == OR_RK_SSP3) {
+    /* ssp_rk_2_2 (mod_fluid.f90:1115-1142) / ssp_rk_3_3 (773-827): the step, its convergence norms, and with
+     * allow_subcycle a retry with dt / 10 while any norm exceeds the tolerance (subcycle_check, 856-887).
+     * U is untouched until an attempt is kept; every attempt advances U's own solver clock (stage 1). */
+    for (;;) {
+      err = 0;
+      fluid_to_stage(f, &U1, n);
+      cb.bc_time = stage_bc_time(c, f, dt, 0);
+      f->t_solver = f->t_solver + dt;
+      time_derivative(c, g, &U, L, f->src_E);
+      combine1(c, &U, L, dt, &U1);
+      err |= assign_stage(c, &U1);
+      fluid_to_stage(f, &U2, n);
+      cb.bc_time = stage_bc_time(c, f, dt, 1);
+      time_derivative(c, g, &U1, L, f->src_E);
+      if (c->integrator == OR_RK_SSP2) {
+        combine2(c, &U, 0.5, &U1, 0.5, L, 0.5 * dt, &U2);
+        err |= assign_stage(c, &U2);
+        convergence(c, &U1, &U2, f->conv); /* mod_fluid.f90:1132 */
+        fin = &U2;
+      } else {
+        combine2(c, &U, 3.0 / 4.0, &U1, 1.0 / 4.0, L, (1.0 / 4.0) * dt, &U2);
+        err |= assign_stage(c, &U2);
+        fluid_to_stage(f, &U3, n);
+        cb.bc_time = stage_bc_time(c, f, dt, 2);
+        time_derivative(c, g, &U2, L, f->src_E);
+        combine2(c, &U, 1.0 / 3.0, &U2, 2.0 / 3.0, L, (2.0 / 3.0) * dt, &U3);
+        err |= assign_stage(c, &U3);
+        convergence(c, &U1, &U3, f->conv); /* mod_fluid.f90:805 */
+        fin = &U3;
+      }
+      int retry = 0;
+      if (c->allow_subcycle)
+        for (int v = 0; v < 4; ++v) retry |= f->conv[v] > c->subcycle_tolerance;
+      if (!retry) break;
+      dt = dt / 10.0;
+    }
   } else {
     /* rk_multistage (mod_fluid.f90:889-1007) */
     static const double a13[3] = {0.1481, 0.4, 1.0}, a14[4] = {0.0833, 0.2069, 0.4265, 1.0},
@@ -1390,7 +1395,8 @@ int or_integrate(const or_config *c_in, const or_geometry *g, or_fluid *f, doubl
       case OR_RK_2_4: alpha = a24; ns = 4; break;
       default: alpha = a25; ns = 5; break;
     }
-    /* U_0 = U; U_prev starts as a copy of U */
+    /* U_0 = U; U_prev starts as a copy of U.  rk_multistage has no subcycling loop (mod_fluid.f90:974-990 is
+     * commented out); U's own solver is never called, so its clock stays put. */
     fluid_to_stage(f, &U1, n); /* U_prev */
     fluid_to_stage(f, &U2, n); /* U_curr */
     for (int m = 0; m < ns; ++m) {
@@ -1409,6 +1415,7 @@ int or_integrate(const or_config *c_in, const or_geometry *g, or_fluid *f, doubl
   memcpy(f->rho_v, fin->a[2], n * sizeof(double));
   memcpy(f->rho_E, fin->a[3], n * sizeof(double));
   f->time = f->time + dt; /* ssp_rk_3_3:844, ssp_rk_2_2 and rk_multistage alike */
+  f->last_dt = dt;        /* fluid_t%integrate returns the dt it used (mod_fluid.f90:448) */
   int code = 0;
   if (err & 3) code = 3;
   or_derived_quantities(c, f->rho, f->rho_u, f->rho_v, f->rho_E, f->u, f->v, f->p, f->cs);
@@ -1434,6 +1441,7 @@ long or_run(const or_config *c, const or_geometry *g, or_fluid *f, double max_ti
   while (time < max_time && iteration < max_iterations) {
     double dt = or_get_timestep(c, g, f);
     int err = or_integrate(c, g, f, dt);
+    dt = f->last_dt; /* master%dt: the dt the step used (a subcycled step's last one) */
     master_time = master_time + dt;
     double new_time = master_time;
     if (err) {

@@ -61,6 +61,9 @@ typedef struct {
   int pi_n;
   const double *pi_t, *pi_p, *pi_rho;
   double bc_time;
+  /* [time] allow_subcycle / subcycle_error_tolerance (mod_input.f90:255-256, mod_fluid.f90:171-172) */
+  int allow_subcycle;
+  double subcycle_tolerance;
 } or_config;
 
 /* Per-cell geometry of the halo'd block (grid_block_2d.f90:352-391). */
@@ -132,6 +135,9 @@ typedef struct {
   /* fluid_t%time (advanced by dt per or_integrate) and the time its flux solver started from
    * (fluid_t%initialize, mod_fluid.f90:169,240): the boundary times of a step derive from both */
   double time, t0;
+  /* the time of the flux solver the first stage of ssp_rk_2_2 / ssp_rk_3_3 calls (U's own): it gains the
+   * dt of every attempt, retried ones included; and the dt the last or_integrate used */
+  double t_solver, last_dt;
 } or_fluid;
 
 /* fluid_t%initialize from primitive interior values (mod_fluid.f90:248-252, 339-363). */

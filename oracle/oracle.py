@@ -44,6 +44,7 @@ class _Config(C.Structure):
         ("gamma", C.c_double), ("cfl", C.c_double), ("bc_pressure", C.c_double),
         ("bc_density", C.c_double), ("pi_n", C.c_int), ("pi_t", C.POINTER(C.c_double)),
         ("pi_p", C.POINTER(C.c_double)), ("pi_rho", C.POINTER(C.c_double)), ("bc_time", C.c_double),
+        ("allow_subcycle", C.c_int), ("subcycle_tolerance", C.c_double),
     ]
 
 
@@ -57,7 +58,8 @@ class _Geometry(C.Structure):
 
 class _Fluid(C.Structure):
     _fields_ = [(n, C.POINTER(C.c_double)) for n in ("rho", "rho_u", "rho_v", "rho_E", "u", "v", "p", "cs")] + \
-        [("src_E", C.POINTER(C.c_double)), ("conv", C.c_double * 4), ("time", C.c_double), ("t0", C.c_double)]
+        [("src_E", C.POINTER(C.c_double)), ("conv", C.c_double * 4), ("time", C.c_double), ("t0", C.c_double),
+         ("t_solver", C.c_double), ("last_dt", C.c_double)]
 
 
 def build(force: bool = False) -> str:
@@ -126,12 +128,13 @@ class Config:
     def __init__(self, ni, nj, flux="AUSMPW+", limiter="minmod", integrator="ssp_rk2",
                  bcs=("periodic", "periodic", "periodic", "periodic"), gamma=1.4, cfl=0.1,
                  apply_low_mach_fix=True, n_ghost=None, bc_pressure=0.0, bc_density=0.0, pressure_table=None,
-                 bc_time=0.0):
+                 bc_time=0.0, allow_subcycle=False, subcycle_tolerance=1e-3):
         self.ni, self.nj = int(ni), int(nj)
         self.bc_pressure = float(bc_pressure)
         # pressure_input: constant bc_pressure / bc_density, or pressure_table = (time, pressure, density)
         self.bc_density = float(bc_density)
         self.bc_time = float(bc_time)
+        self.allow_subcycle, self.subcycle_tolerance = bool(allow_subcycle), float(subcycle_tolerance)
         self.pressure_table = None
         if pressure_table is not None:
             self.pressure_table = tuple(np.ascontiguousarray(a, dtype=np.float64) for a in pressure_table)
@@ -152,7 +155,8 @@ class Config:
                        RK_IDS[self.integrator], BC_IDS[self.bcs[0]], BC_IDS[self.bcs[1]], BC_IDS[self.bcs[2]],
                        BC_IDS[self.bcs[3]], int(self.apply_low_mach_fix), self.gamma, self.cfl, self.bc_pressure,
                        self.bc_density, len(tab[0]) if tab else 0, _p(tab[0]) if tab else dp(),
-                       _p(tab[1]) if tab else dp(), _p(tab[2]) if tab else dp(), self.bc_time)
+                       _p(tab[1]) if tab else dp(), _p(tab[2]) if tab else dp(), self.bc_time,
+                       int(self.allow_subcycle), self.subcycle_tolerance)
 
     def field(self):
         return np.zeros(self.shape, dtype=np.float64, order="F")
@@ -281,10 +285,16 @@ class Fluid:
         """fluid_t time and the time its flux solver starts from (a fluid initialized at time t)."""
         self._f.time = float(t)
         self._f.t0 = float(t)
+        self._f.t_solver = float(t)
 
     @property
     def time(self) -> float:
         return float(self._f.time)
+
+    @property
+    def last_dt(self) -> float:
+        """The dt the last integrate() used (smaller than the one given after subcycling)."""
+        return float(self._f.last_dt)
 
     def convergence(self):
         """L2-norm convergence [rho, rho u, rho v, rho E] of the last integrate()."""

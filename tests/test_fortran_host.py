@@ -111,7 +111,6 @@ def test_host_reads_pressure_input_deck(tmp_path):
 
 
 @pytest.mark.parametrize("extra, message", [
-    ("[time]\nallow_subcycle = true\n", "allow_subcycle = true is not supported"),
     ("[source_terms]\nenable_source_terms = true\n", "[source_terms] from the deck are not supported"),
 ])
 def test_host_rejects_unsupported_deck_options(tmp_path, extra, message):
@@ -404,3 +403,36 @@ def test_host_pressure_input_file_matches_oracle(tmp_path, oracle, device_loop):
         ref = f.interior(name)
         assert np.max(np.abs(got[key] - ref)) <= 1e-12 * np.max(np.abs(ref)), key
     assert np.max(f.interior("rho")) > 1.5  # the drive launched a shock into the gas
+
+
+@pytest.mark.gpu
+def test_host_subcycling_matches_oracle(tmp_path, oracle):
+    """[time] allow_subcycle with a tolerance that the first steps of a Sod tube exceed: the host-side loop retries
+    those steps with dt / 10 (mod_fluid.f90:816-823,856-887) and lands on the oracle's main.f90 loop bit for bit;
+    the residual history shows the kept dt; --device-loop refuses the option."""
+    from fvleg_2d_amd.deck import read_state, write_deck
+
+    pb = sod_problem(ni=100, nj=4, flux="AUSM+", limiter="minmod", integrator="ssp_rk3", cfl=0.4, max_time=0.05)
+    tol = 3e-3  # above the rho u norm of later steps (~2.5e-3), below the first ones' (~3.3e-3)
+    ini = write_deck(pb, str(tmp_path), "sub")
+    with open(ini, "a") as fh:
+        fh.write(f"\n[time]\nallow_subcycle = true\nsubcycle_error_tolerance = {tol!r}\n")
+    out = os.path.join(str(tmp_path), "final.h5")
+    run_host([ini, out], str(tmp_path))
+    got = read_state(out)
+    hist = read_residual_hist(os.path.join(str(tmp_path), "residual_hist.csv"))
+    cfg = pb.cfg
+    nh = cfg.n_ghost
+    oc = oracle.Config(pb.ni, pb.nj, flux=cfg.flux, limiter=cfg.limiter, integrator=cfg.integrator, bcs=cfg.bcs,
+                       gamma=cfg.gamma, cfl=cfg.cfl, allow_subcycle=True, subcycle_tolerance=tol)
+    og = oracle.Geometry(oc, pb.node_x, pb.node_y)
+    f = oracle.Fluid(oc, og, *[np.pad(a, nh) for a in (pb.rho, pb.u, pb.v, pb.p)])
+    n, t, e = f.run(pb.max_time, 2 ** 31 - 1, pb.max_time)
+    assert e == 0 and got["iteration"] == n and got["time"] == t
+    for key, name in (("density", "rho"), ("rho_u", "rho_u"), ("rho_v", "rho_v"), ("rho_E", "rho_E")):
+        assert np.array_equal(got[key], f.interior(name)), key
+    # the first step was retried: its kept dt is a tenth (or less) of the CFL dt of the next steps
+    assert hist[0, 2] < 0.2 * hist[-1, 2] and np.all(hist[:, 3:] <= tol)
+    r = subprocess.run([host_or_fail(), ini, out, "--device-loop"], capture_output=True, text=True, timeout=60,
+                       cwd=str(tmp_path))
+    assert r.returncode != 0 and "does not support allow_subcycle" in r.stdout

@@ -362,6 +362,45 @@ def test_device_loop_graph_matches_eager(oracle, precision):
             assert np.array_equal(g1[k], f.interior(name)), name
 
 
+@pytest.mark.parametrize("integrator", ["ssp_rk2", "ssp_rk3", "rk(1,3)"])
+def test_subcycle_parity(oracle, integrator):
+    """[time] allow_subcycle (subcycle_check, mod_fluid.f90:856-887): cato_step retries with dt / 10 while a
+    convergence norm exceeds the tolerance, from the same state, and keeps the last attempt's buffers; steps and
+    the dt each used match the oracle bit for bit (rk(1,3) has no retry loop).  The device loop refuses it."""
+    say("subcycle", integrator)
+    from fvleg_2d_amd.problems import sod
+    from fvleg_2d_amd._native import CatoError
+
+    pb = sod(ni=100, nj=4, flux="AUSM+", limiter="minmod", integrator=integrator, cfl=0.4)
+    cfg = pb.cfg
+    nh = cfg.n_ghost
+    tol = 3e-3
+    oc = oracle_cfg(oracle, cfg, pb.ni, pb.nj)
+    oc.allow_subcycle, oc.subcycle_tolerance = True, tol
+    f = oracle.Fluid(oc, oracle.Geometry(oc, pb.node_x, pb.node_y), *[pad(a, nh) for a in (pb.rho, pb.u, pb.v, pb.p)])
+    retried = 0
+    with Solver(cfg, pb.node_x, pb.node_y) as s:
+        s.set_primitive(pb.rho, pb.u, pb.v, pb.p)
+        s.set_subcycle(True, tol)
+        s.set_diagnostics(True)
+        for k in range(12):
+            dt = s.compute_dt()
+            assert dt == f.get_timestep(), k
+            assert s.step(dt) == 0 and f.integrate(dt) == 0
+            assert s.last_dt() == f.last_dt, k
+            retried += s.last_dt() < dt
+            assert np.allclose(s.convergence(), f.convergence(), rtol=1e-12, atol=0), k
+        g = s.conserved()
+        h = s.history()
+        if integrator != "rk(1,3)":
+            with pytest.raises(CatoError, match="host-driven"):
+                s.run(1.0, 5)
+    for k, name in enumerate(("rho", "rho_u", "rho_v", "rho_E")):
+        assert np.array_equal(g[k], f.interior(name)), name
+    assert retried > 0 if integrator != "rk(1,3)" else retried == 0
+    assert h.shape == (12, 7) and h[-1, 1] == f.time
+
+
 @pytest.mark.parametrize("case", ["kh2048", "riemann1024"])
 def test_full_size_workloads_match_oracle(oracle, case):
     """The bench workloads at their full sizes: Kelvin-Helmholtz 2048^2 (SLAU, van Leer, SSP-RK3) for 3 steps

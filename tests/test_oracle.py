@@ -519,3 +519,48 @@ def test_outflow_bc_priority_and_other_sides(oracle):
     oracle.apply_bc_single(one, "outflow", "+x", *m)
     for k in range(4):
         assert np.array_equal(f[k][nh + ni:, nh + nj:], m[k][nh + ni:, nh + nj:]), k
+
+
+@pytest.mark.parametrize("integrator", ["ssp_rk2", "ssp_rk3", "rk(2,4)"])
+def test_subcycle_retries_with_a_tenth_of_dt(oracle, integrator):
+    """subcycle_check (mod_fluid.f90:856-887): with a tolerance between the convergence norms of the step at dt and
+    at dt/10, integrate(dt) retries once, reports dt/10 and lands bit for bit on a plain integrate(dt/10) of the same
+    state (the retry starts again from U); a tolerance above the first norms changes nothing; rk_multistage has no
+    retry loop."""
+    ni, nj = 24, 16
+    rng = np.random.default_rng(17)
+    base = dict(flux="SLAU", limiter="minmod", integrator=integrator,
+                bcs=("zero_gradient", "zero_gradient", "periodic", "periodic"), cfl=0.4)
+    prims = [rng.uniform(0.5, 1.5, (ni, nj)), rng.uniform(-0.5, 0.5, (ni, nj)), rng.uniform(-0.5, 0.5, (ni, nj)),
+             rng.uniform(0.5, 1.5, (ni, nj))]
+
+    def fluid(**kw):
+        cfg = oracle.Config(ni, nj, **base, **kw)
+        nx, ny = oracle.uniform_nodes(ni, nj, cfg.nh)
+        return oracle.Fluid(cfg, oracle.Geometry(cfg, nx, ny), *[np.pad(a, cfg.nh) for a in prims])
+
+    def state(f):
+        return [f.interior(n) for n in ("rho", "rho_u", "rho_v", "rho_E")]
+
+    fa = fluid()
+    dt = fa.get_timestep()
+    assert fa.integrate(dt) == 0 and fa.last_dt == dt
+    n1 = max(fa.convergence())
+    fb = fluid()
+    assert fb.integrate(dt / 10.0) == 0
+    n2 = max(fb.convergence())
+    assert n2 < 0.5 * n1  # the norms shrink with dt
+    tol = 0.5 * (n1 + n2)
+    fc = fluid(allow_subcycle=True, subcycle_tolerance=tol)
+    assert fc.integrate(dt) == 0
+    retried = integrator != "rk(2,4)"
+    assert fc.last_dt == (dt / 10.0 if retried else dt)
+    assert fc.time == fc.last_dt
+    want = fb if retried else fa
+    for a, b in zip(state(fc), state(want)):
+        assert np.array_equal(a, b)
+    assert fc.convergence() == want.convergence()
+    fd = fluid(allow_subcycle=True, subcycle_tolerance=2.0 * n1)
+    assert fd.integrate(dt) == 0 and fd.last_dt == dt
+    for a, b in zip(state(fd), state(fa)):
+        assert np.array_equal(a, b)
