@@ -58,6 +58,9 @@ module cato_deck
     real(rk) :: bc_pressure_scale_factor = 1.0_rk
     ! [io]
     real(rk) :: contour_interval_dt = huge(1.0_rk)
+    logical :: plot_64bit = .true., plot_volume = .true.   ! mod_input.f90:432,443
+    ! [general] units: the contour writer's unit system (units.f90:94-143)
+    character(len=8) :: unit_system = 'cgs'
     ! [grid]
     integer :: n_ghost_layers = 0
   end type
@@ -67,6 +70,7 @@ module cato_deck
   integer(c_int64_t), bind(C, name="H5T_NATIVE_DOUBLE_g") :: h5t_native_double
   integer(c_int64_t), bind(C, name="H5T_NATIVE_INT_g") :: h5t_native_int
   integer(c_int64_t), bind(C, name="H5T_C_S1_g") :: h5t_c_s1
+  integer(c_int64_t), bind(C, name="H5T_NATIVE_FLOAT_g") :: h5t_native_float
   integer(c_int), parameter :: H5S_SCALAR = 0
 
   interface
@@ -214,6 +218,19 @@ module cato_deck
     function H5Aclose(a) bind(C, name="H5Aclose") result(r)
       import :: c_int, c_int64_t
       integer(c_int64_t), value :: a
+      integer(c_int) :: r
+    end function
+    function H5Acreate_by_name(loc, obj, attr, atype, space, acpl, aapl, lapl) bind(C, name="H5Acreate_by_name") &
+      result(h)
+      import :: c_int64_t, c_char
+      integer(c_int64_t), value :: loc, atype, space, acpl, aapl, lapl
+      character(kind=c_char), intent(in) :: obj(*), attr(*)
+      integer(c_int64_t) :: h
+    end function
+    function H5Awrite(a, mtype, buf) bind(C, name="H5Awrite") result(r)
+      import :: c_int, c_int64_t, c_ptr
+      integer(c_int64_t), value :: a, mtype
+      type(c_ptr), value :: buf
       integer(c_int) :: r
     end function
   end interface
@@ -366,6 +383,9 @@ contains
       case ('boundary_conditions.bc_pressure_input_file'); inp%bc_pressure_input_file = val
       case ('boundary_conditions.bc_pressure_scale_factor'); inp%bc_pressure_scale_factor = to_real(val, key)
       case ('io.contour_interval_dt'); inp%contour_interval_dt = to_real(val, key)
+      case ('io.plot_64bit'); inp%plot_64bit = to_logical(val)
+      case ('io.plot_volume'); inp%plot_volume = to_logical(val)
+      case ('general.units'); inp%unit_system = val
       case ('grid.n_ghost_layers'); inp%n_ghost_layers = int(to_real(val, key))
       end select
     end do
@@ -573,19 +593,90 @@ contains
     r = H5Tclose(t)
   end subroutine
 
+  !> A fixed-length string attribute (as h5fortran's writeattr stores them).
+  subroutine write_attr(f, obj, name, text)
+    integer(c_int64_t), intent(in) :: f
+    character(len=*), intent(in) :: obj, name, text
+    character(kind=c_char, len=max(len_trim(text), 1)), target :: buf
+    integer(c_int64_t) :: t, sp, a
+    integer(c_int) :: r
+    buf = trim(text)
+    if (len_trim(text) == 0) buf = ' '
+    t = H5Tcopy(h5t_c_s1)
+    r = H5Tset_size(t, int(len(buf), c_size_t))
+    sp = H5Screate(H5S_SCALAR)
+    a = H5Acreate_by_name(f, cstr(obj), cstr(name), t, sp, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT)
+    if (a < 0) error stop 'contour: cannot create attribute '//trim(name)//' of '//trim(obj)
+    r = H5Awrite(a, t, c_loc(buf))
+    r = H5Aclose(a)
+    r = H5Sclose(sp)
+    r = H5Tclose(t)
+  end subroutine
+
+  !> contour_writer_t%write_2d_real_data (contour_writer.f90:544-587): FP64, or FP32 with values beyond
+  !> huge(real32) clipped to +-huge and magnitudes below tiny(real32) written as 0; description and units.
+  subroutine write_field(f, name, a, plot_64bit, description, units)
+    integer(c_int64_t), intent(in) :: f
+    character(len=*), intent(in) :: name, description, units
+    real(rk), intent(in) :: a(:, :)
+    logical, intent(in) :: plot_64bit
+    real(c_float), allocatable, target :: b(:, :)
+    integer(c_int64_t) :: s, d, dims(2)
+    integer(c_int) :: r
+    integer :: i, j
+    if (plot_64bit) then
+      call write_2d(f, name, a)
+    else
+      allocate(b(size(a, 1), size(a, 2)))
+      do j = 1, size(a, 2)
+        do i = 1, size(a, 1)
+          if (abs(a(i, j)) > huge(1.0_c_float)) then
+            b(i, j) = sign(huge(1.0_c_float), real(a(i, j), c_float))
+          else if (abs(a(i, j)) < tiny(1.0_c_float)) then
+            b(i, j) = 0.0_c_float
+          else
+            b(i, j) = real(a(i, j), c_float)
+          end if
+        end do
+      end do
+      dims = [int(size(b, 2), c_int64_t), int(size(b, 1), c_int64_t)]
+      s = H5Screate_simple(2_c_int, dims, c_null_ptr)
+      d = H5Dcreate2(f, cstr(name), h5t_native_float, s, H5P_DEFAULT, H5P_DEFAULT, H5P_DEFAULT)
+      if (d < 0) error stop 'output: cannot create dataset '//trim(name)
+      r = H5Dwrite(d, h5t_native_float, H5S_ALL, H5S_ALL, H5P_DEFAULT, c_loc(b))
+      r = H5Dclose(d)
+      r = H5Sclose(s)
+    end if
+    call write_attr(f, name, 'description', description)
+    call write_attr(f, name, 'units', units)
+  end subroutine
+
   !> contour_writer_t%write_hdf5 (src/lib/io/contour_writer.f90:193-384): results/step_NNNNNNN.h5 with the
-  !> scalars and the halo'd node and cell fields (ghost cells included, as plot_ghost_cells defaults to true).
-  !> Units attributes and the xdmf companion are not written.
-  subroutine write_contour(folder, title, iteration, time, delta_t, n_ghost, x, y, rho, u, v, p, cs)
-    character(len=*), intent(in) :: folder, title
+  !> scalars and the halo'd node and cell fields (ghost cells included, as plot_ghost_cells defaults to true)
+  !> in the deck's output unit system (set_output_unit_system, units.f90:94-143: 'cgs' or 'icf'; the code
+  !> units are cgs with the unit reference state this host takes), each with its description and units
+  !> attributes; FP32 unless plot_64bit.  The xdmf companion is not written.
+  subroutine write_contour(folder, title, iteration, time, delta_t, n_ghost, x, y, rho, u, v, p, cs, volume, &
+                           unit_system, plot_64bit)
+    character(len=*), intent(in) :: folder, title, unit_system
     integer, intent(in) :: iteration, n_ghost
     real(rk), intent(in) :: time, delta_t
     real(rk), intent(in) :: x(:, :), y(:, :), rho(:, :), u(:, :), v(:, :), p(:, :), cs(:, :)
+    real(rk), intent(in), optional :: volume(:, :)
+    logical, intent(in) :: plot_64bit
     character(len=512) :: path
     integer(c_int64_t) :: f
     integer(c_int), target :: ival
     real(rk), target :: rval
     integer(c_int) :: r
+    real(rk) :: tu, lu, vu, pu
+    character(len=10) :: tl, ll, vl, pl
+    select case (trim(unit_system))
+    case ('icf')
+      tu = 1e9_rk; tl = 'ns'; lu = 1e4_rk; ll = 'um'; vu = 1e-5_rk; vl = 'km/s'; pu = 1e-12_rk; pl = 'Mbar'
+    case default
+      tu = 1.0_rk; tl = 's'; lu = 1.0_rk; ll = 'cm'; vu = 1.0_rk; vl = 'cm/s'; pu = 1.0_rk; pl = 'barye'
+    end select
     write(path, '(a, a, i0.7, a)') trim(folder), '/step_', iteration, '.h5'
     r = H5open()
     f = H5Fcreate(cstr(path), H5F_ACC_TRUNC, H5P_DEFAULT, H5P_DEFAULT)
@@ -593,19 +684,26 @@ contains
     call write_string(f, '/title', title)
     ival = iteration
     call write_scalar_value(f, '/iteration', h5t_native_int, c_loc(ival))
-    rval = time
+    call write_attr(f, '/iteration', 'description', 'Iteration Count')
+    call write_attr(f, '/iteration', 'units', 'dimensionless')
+    rval = time * tu
     call write_scalar_value(f, '/time', h5t_native_double, c_loc(rval))
+    call write_attr(f, '/time', 'description', 'Simulation Time')
+    call write_attr(f, '/time', 'units', tl)
     rval = delta_t
     call write_scalar_value(f, '/delta_t', h5t_native_double, c_loc(rval))
+    call write_attr(f, '/delta_t', 'description', 'Simulation Timestep')
+    call write_attr(f, '/delta_t', 'units', 'seconds')
     ival = n_ghost
     call write_scalar_value(f, '/n_ghost_layers', h5t_native_int, c_loc(ival))
-    call write_2d(f, '/x', x)
-    call write_2d(f, '/y', y)
-    call write_2d(f, '/density', rho)
-    call write_2d(f, '/x_velocity', u)
-    call write_2d(f, '/y_velocity', v)
-    call write_2d(f, '/pressure', p)
-    call write_2d(f, '/sound_speed', cs)
+    call write_field(f, '/x', x * lu, plot_64bit, 'X Coordinate', ll)
+    call write_field(f, '/y', y * lu, plot_64bit, 'Y Coordinate', ll)
+    call write_field(f, '/density', rho, plot_64bit, 'Cell Density', 'g/cc')
+    call write_field(f, '/x_velocity', u * vu, plot_64bit, 'Cell X Velocity', vl)
+    call write_field(f, '/y_velocity', v * vu, plot_64bit, 'Cell Y Velocity', vl)
+    call write_field(f, '/pressure', p * pu, plot_64bit, 'Cell Pressure', pl)
+    call write_field(f, '/sound_speed', cs * vu, plot_64bit, 'Cell Sound Speed', vl)
+    if (present(volume)) call write_field(f, '/volume', volume, plot_64bit, 'Cell Volume', 'cc')
     r = H5Fclose(f)
   end subroutine
 

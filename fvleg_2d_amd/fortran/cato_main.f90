@@ -264,8 +264,33 @@ contains
     call bcs%apply(hr, hu, hv, hp)
     where (hr > 0.0_rk .and. hp >= 0.0_rk) hc = sqrt(cfg%gamma * hp / hr)
     hc(nh + 1:nh + grid%ni, nh + 1:nh + grid%nj) = ic
-    call write_contour(trim(contour_dir), trim(input%title), it, t, dt_last, nh, x, y, hr, hu, hv, hp, hc)
+    if (input%plot_volume) then
+      call write_contour(trim(contour_dir), trim(input%title), it, t, dt_last, nh, x, y, hr, hu, hv, hp, hc, &
+                         cell_volumes(), unit_system=input%unit_system, plot_64bit=input%plot_64bit)
+    else
+      call write_contour(trim(contour_dir), trim(input%title), it, t, dt_last, nh, x, y, hr, hu, hv, hp, hc, &
+                         unit_system=input%unit_system, plot_64bit=input%plot_64bit)
+    end if
   end subroutine
+
+  !> quad_cell_t volumes of the halo'd cells (quad_cell.f90:78-91 round-off fixes, 160-175 the volume)
+  function cell_volumes() result(vol)
+    real(rk), allocatable :: vol(:, :)
+    real(rk) :: qx(4), qy(4)
+    integer :: i, j
+    allocate(vol(size(x, 1) - 1, size(x, 2) - 1))
+    do j = 1, size(vol, 2)
+      do i = 1, size(vol, 1)
+        qx = [x(i, j), x(i + 1, j), x(i + 1, j + 1), x(i, j + 1)]
+        qy = [y(i, j), y(i + 1, j), y(i + 1, j + 1), y(i, j + 1)]
+        if (abs(qy(2) - qy(1)) < epsilon(1.0_rk)) qy(2) = qy(1)
+        if (abs(qy(4) - qy(3)) < epsilon(1.0_rk)) qy(4) = qy(3)
+        if (abs(qx(3) - qx(2)) < epsilon(1.0_rk)) qx(3) = qx(2)
+        if (abs(qx(4) - qx(1)) < epsilon(1.0_rk)) qx(4) = qx(1)
+        vol(i, j) = (qx(2) - qx(1)) * (qy(4) - qy(1)) - (qx(4) - qx(1)) * (qy(2) - qy(1))
+      end do
+    end do
+  end function
 
   !> write_residual_history (mod_fluid.f90:1207-1223) for the step just taken
   subroutine history_row(it, t, dt)

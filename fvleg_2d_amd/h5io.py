@@ -69,6 +69,17 @@ def lib():
         L.H5Sclose.argtypes = [_hid]
         L.H5Lexists.restype = C.c_int
         L.H5Lexists.argtypes = [_hid, C.c_char_p, _hid]
+        L.H5Aopen_by_name.restype = _hid
+        L.H5Aopen_by_name.argtypes = [_hid, C.c_char_p, C.c_char_p, _hid, _hid]
+        L.H5Aget_type.restype = _hid
+        L.H5Aget_type.argtypes = [_hid]
+        L.H5Aread.argtypes = [_hid, _hid, C.c_void_p]
+        L.H5Aclose.argtypes = [_hid]
+        L.H5Dget_type.restype = _hid
+        L.H5Dget_type.argtypes = [_hid]
+        L.H5Tget_size.restype = C.c_size_t
+        L.H5Tget_size.argtypes = [_hid]
+        L.H5Tclose.argtypes = [_hid]
         _lib = L
     return _lib
 
@@ -124,6 +135,37 @@ class H5File:
         if out.ndim == 2:
             return np.asfortranarray(out.T)
         return out
+
+    def attr(self, name: str, attr: str) -> str:
+        """A fixed-length string attribute of a dataset."""
+        L = lib()
+        a = L.H5Aopen_by_name(self.h, name.encode(), attr.encode(), 0, 0)
+        if a < 0:
+            raise KeyError(f"{name}:{attr}")
+        try:
+            t = L.H5Aget_type(a)
+            n = L.H5Tget_size(t)
+            buf = C.create_string_buffer(n + 1)
+            if L.H5Aread(a, t, buf) < 0:
+                raise OSError(f"read of {name}:{attr} failed")
+            L.H5Tclose(t)
+        finally:
+            L.H5Aclose(a)
+        return buf.raw[:n].split(b"\0")[0].decode().rstrip()
+
+    def itemsize(self, name: str) -> int:
+        """Bytes per element as stored (4 for FP32 datasets, 8 for FP64)."""
+        L = lib()
+        d = L.H5Dopen2(self.h, name.encode(), 0)
+        if d < 0:
+            raise KeyError(name)
+        try:
+            t = L.H5Dget_type(d)
+            n = int(L.H5Tget_size(t))
+            L.H5Tclose(t)
+        finally:
+            L.H5Dclose(d)
+        return n
 
     def write(self, name: str, data):
         """Write a scalar int or a 2D (i, j) float64 array (stored transposed, as pycato does)."""

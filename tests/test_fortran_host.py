@@ -436,3 +436,54 @@ def test_host_subcycling_matches_oracle(tmp_path, oracle):
     r = subprocess.run([host_or_fail(), ini, out, "--device-loop"], capture_output=True, text=True, timeout=60,
                        cwd=str(tmp_path))
     assert r.returncode != 0 and "does not support allow_subcycle" in r.stdout
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("units, plot_64bit", [("cgs", True), ("icf", False)])
+def test_host_contour_units_and_precision(tmp_path, units, plot_64bit):
+    """contour_writer_t%write_hdf5 (contour_writer.f90:193-384) in the deck's unit system (units.f90:94-143): icf
+    writes ns, um, km/s and Mbar; every field carries description and units attributes; plot_64bit = false stores
+    FP32 (write_2d_real_data, :544-587); /volume holds the halo'd cells' volumes.  A restart reads the icf contour
+    back through its units attributes."""
+    from fvleg_2d_amd import h5io
+    from fvleg_2d_amd.deck import read_contour, read_state, write_deck
+
+    pb = sod_problem(ni=60, nj=4, flux="AUSM+", limiter="minmod", integrator="ssp_rk2", cfl=0.4, max_time=0.05)
+    d = str(tmp_path)
+    ini = write_deck(pb, d, "u")
+    with open(ini) as fh:
+        text = fh.read()
+    text = text.replace("[general]\n", f"[general]\nunits = '{units}'\n", 1)
+    text += f"\n[io]\nplot_64bit = {'true' if plot_64bit else 'false'}\ncontour_interval_dt = 0.025\n"
+    with open(ini, "w") as fh:
+        fh.write(text)
+    run_host([ini, os.path.join(d, "out.h5"), "--contours", os.path.join(d, "c")], d)
+    out = read_state(os.path.join(d, "out.h5"))
+    path = sorted(os.listdir(os.path.join(d, "c")))[-1]
+    c = read_contour(os.path.join(d, "c", path))
+    want = {"cgs": ("s", "cm", "cm/s", "barye"), "icf": ("ns", "um", "km/s", "Mbar")}[units]
+    with h5io.H5File(os.path.join(d, "c", path)) as f:
+        assert f.attr("/time", "units") == want[0] and f.attr("/time", "description") == "Simulation Time"
+        assert f.attr("/delta_t", "units") == "seconds" and f.attr("/iteration", "units") == "dimensionless"
+        assert f.attr("/x", "units") == want[1] and f.attr("/density", "units") == "g/cc"
+        assert f.attr("/x_velocity", "units") == want[2] and f.attr("/sound_speed", "units") == want[2]
+        assert f.attr("/pressure", "units") == want[3] and f.attr("/pressure", "description") == "Cell Pressure"
+        assert f.itemsize("/pressure") == (8 if plot_64bit else 4) and f.itemsize("/volume") == (8 if plot_64bit else 4)
+        vol = f.read("/volume")
+    scale = {"cgs": (1.0, 1.0, 1.0, 1.0), "icf": (1e9, 1e4, 1e-5, 1e-12)}[units]
+    nh = pb.cfg.n_ghost
+    # the contour carries the master time (unclipped, master_puppeteer%time), the output the driver's
+    assert 0.0 <= c["time"] / scale[0] - out["time"] <= c["delta_t"] * (1 + 1e-12)
+    inner = (slice(nh, nh + pb.ni), slice(nh, nh + pb.nj))
+    rtol = 1e-15 if plot_64bit else 1e-7
+    assert np.allclose(c["pressure"][inner] / scale[3], out["pressure"], rtol=rtol, atol=0)
+    assert np.allclose(c["x_velocity"][inner] / scale[2], out["x_velocity"], rtol=rtol, atol=1e-30)
+    assert np.allclose(c["x"] / scale[1], pb.node_x, rtol=rtol, atol=1e-12)
+    assert vol.shape == (pb.ni + 2 * nh, pb.nj + 2 * nh)
+    assert np.allclose(vol, (1.0 / pb.ni) * (0.01 / pb.nj), rtol=1e-6)
+    if units == "icf":
+        rest = write_deck(pb, d, "r")
+        with open(rest, "a") as fh:
+            fh.write(f"\n[restart]\nrestart_from_file = true\nrestart_file = '{os.path.join(d, 'c', path)}'\n")
+        out2 = run_host([rest, os.path.join(d, "r.h5")], d)
+        assert "Restart from" in out2
