@@ -13,7 +13,7 @@ module cato_deck
 
   integer, parameter :: rk = c_double
   public :: cato_input_t, read_input_ini, read_initial_conditions, read_restart, write_state, write_contour, &
-            read_pressure_input
+            read_pressure_input, write_xdmf
 
   !> Deck entries of the hot path (names as in input.ini)
   type :: cato_input_t
@@ -59,6 +59,7 @@ module cato_deck
     ! [io]
     real(rk) :: contour_interval_dt = huge(1.0_rk)
     logical :: plot_64bit = .true., plot_volume = .true.   ! mod_input.f90:432,443
+    character(len=8) :: contour_format = 'xdmf'            ! [io] format (mod_input.f90:429-430)
     ! [general] units: the contour writer's unit system (units.f90:94-143)
     character(len=8) :: unit_system = 'cgs'
     ! [grid]
@@ -384,6 +385,7 @@ contains
       case ('boundary_conditions.bc_pressure_scale_factor'); inp%bc_pressure_scale_factor = to_real(val, key)
       case ('io.contour_interval_dt'); inp%contour_interval_dt = to_real(val, key)
       case ('io.plot_64bit'); inp%plot_64bit = to_logical(val)
+      case ('io.format'); inp%contour_format = val
       case ('io.plot_volume'); inp%plot_volume = to_logical(val)
       case ('general.units'); inp%unit_system = val
       case ('grid.n_ghost_layers'); inp%n_ghost_layers = int(to_real(val, key))
@@ -705,6 +707,83 @@ contains
     call write_field(f, '/sound_speed', cs * vu, plot_64bit, 'Cell Sound Speed', vl)
     if (present(volume)) call write_field(f, '/volume', volume, plot_64bit, 'Cell Volume', 'cc')
     r = H5Fclose(f)
+  end subroutine
+
+  !> contour_writer_t%write_xdmf (contour_writer.f90:386-528): the XDMF companion of step_NNNNNNN.h5 that
+  !> points a viewer at its datasets (structured 2D mesh from /x, /y; density, sound speed, pressure, the
+  !> velocity vector, the Mach-number function and the volume as cell attributes, with their unit labels).
+  subroutine write_xdmf(folder, iteration, time, nx_cells, ny_cells, unit_system, plot_volume)
+    character(len=*), intent(in) :: folder, unit_system
+    integer, intent(in) :: iteration, nx_cells, ny_cells
+    real(rk), intent(in) :: time
+    logical, intent(in) :: plot_volume
+    character(len=512) :: path
+    character(len=32) :: h5
+    character(len=40) :: cells, nodes
+    character(len=10) :: tl, vl, pl
+    real(rk) :: tu
+    integer :: u
+    select case (trim(unit_system))
+    case ('icf')
+      tu = 1e9_rk; tl = 'ns'; vl = 'km/s'; pl = 'Mbar'
+    case default
+      tu = 1.0_rk; tl = 's'; vl = 'cm/s'; pl = 'barye'
+    end select
+    write(h5, '(a, i0.7, a)') 'step_', iteration, '.h5'
+    write(path, '(a, a, i0.7, a)') trim(folder), '/step_', iteration, '.xdmf'
+    write(cells, '(i0, 1x, i0)') ny_cells, nx_cells
+    write(nodes, '(i0, 1x, i0)') ny_cells + 1, nx_cells + 1
+    open(newunit=u, file=trim(path), status='replace', action='write')
+    write(u, '(a)') '<?xml version="1.0" ?>'
+    write(u, '(a)') '<Xdmf version="2.2">'
+    write(u, '(a)') '  <Domain>'
+    write(u, '(a)') '    <Grid GridType="Uniform" Name="grid">'
+    write(u, '(a, g0.3, a)') '      <Time Value="', time * tu, ' '//trim(tl)//'"/>'
+    write(u, '(a)') '      <Topology NumberOfElements="'//trim(nodes)//'" TopologyType="2DSMesh"/>'
+    write(u, '(a)') '      <Geometry GeometryType="X_Y">'
+    call item('        ', nodes, '/x')
+    call item('        ', nodes, '/y')
+    write(u, '(a)') '      </Geometry>'
+    call scalar('Density [g/cc]', '/density')
+    call scalar('Sound Speed ['//trim(vl)//']', '/sound_speed')
+    call scalar('Pressure ['//trim(pl)//']', '/pressure')
+    write(u, '(a)') '      <Attribute AttributeType="Vector" Center="Cell" Name="Velocity ['//trim(vl)// &
+      ']" Dimensions="'//trim(cells)//' 2">'
+    write(u, '(a)') '        <DataItem Dimensions="'//trim(cells)//' 2" ItemType="Function" Function="JOIN($0, $1)">'
+    call func_item('/x_velocity')
+    call func_item('/y_velocity')
+    write(u, '(a)') '        </DataItem>'
+    write(u, '(a)') '      </Attribute>'
+    write(u, '(a)') '      <Attribute AttributeType="Vector" Center="Cell" Name="Mach" Dimensions="'//trim(cells)//' 2">'
+    write(u, '(a)') '        <DataItem Dimensions="'//trim(cells)// &
+      ' 2" ItemType="Function" Function="JOIN(ABS($0/$2), ABS($1/$2))">'
+    call func_item('/x_velocity')
+    call func_item('/y_velocity')
+    call func_item('/sound_speed')
+    write(u, '(a)') '        </DataItem>'
+    write(u, '(a)') '      </Attribute>'
+    if (plot_volume) call scalar('Volume [cc]', '/volume')
+    write(u, '(a)') '    </Grid>'
+    write(u, '(a)') '  </Domain>'
+    write(u, '(a)') '</Xdmf>'
+    close(u)
+  contains
+    subroutine item(indent, dims, ds)
+      character(len=*), intent(in) :: indent, dims, ds
+      write(u, '(a)') indent//'<DataItem Dimensions="'//trim(dims)//'" Format="HDF" NumberType="Float" Precision="4">'// &
+        trim(h5)//':'//ds//'</DataItem>'
+    end subroutine
+    subroutine scalar(label, ds)
+      character(len=*), intent(in) :: label, ds
+      write(u, '(a)') '      <Attribute AttributeType="Scalar" Center="Cell" Name="'//label//'">'
+      call item('        ', cells, ds)
+      write(u, '(a)') '      </Attribute>'
+    end subroutine
+    subroutine func_item(ds)
+      character(len=*), intent(in) :: ds
+      write(u, '(a)') '          <DataItem DataType="Float" Dimensions="'//trim(cells)//'" Format="HDF" Precision="4">'// &
+        trim(h5)//':'//ds//'</DataItem>'
+    end subroutine
   end subroutine
 
   !> Interior state of a solve: conserved and primitive fields, time and iteration count.

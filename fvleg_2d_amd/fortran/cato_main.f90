@@ -153,6 +153,13 @@ program cato_hip_host
   bcs%dev => dev
   bcs%name = 'deck boundary conditions'
   if (len_trim(contour_dir) > 0) then
+    select case (trim(input%contour_format))
+    case ('xdmf', 'hdf5', 'h5')
+    case default
+      write(*, '(a)') "Error: Unsupported I/O contour format '"//trim(input%contour_format)// &
+        "' (must be .xdmf .h5 or .hdf5)"
+      error stop 1
+    end select
     if (device_loop) then
       write(*, '(a)') 'Error: --contours needs the host-side loop (drop --device-loop)'
       error stop 1
@@ -271,6 +278,9 @@ contains
       call write_contour(trim(contour_dir), trim(input%title), it, t, dt_last, nh, x, y, hr, hu, hv, hp, hc, &
                          unit_system=input%unit_system, plot_64bit=input%plot_64bit)
     end if
+    ! contour_writer_t%write_contour (contour_writer.f90:178-190): 'xdmf' adds the companion file
+    if (trim(input%contour_format) == 'xdmf') &
+      call write_xdmf(trim(contour_dir), it, t, n1, n2, input%unit_system, input%plot_volume)
   end subroutine
 
   !> quad_cell_t volumes of the halo'd cells (quad_cell.f90:78-91 round-off fixes, 160-175 the volume)

@@ -455,12 +455,36 @@ def test_host_contour_units_and_precision(tmp_path, units, plot_64bit):
         text = fh.read()
     text = text.replace("[general]\n", f"[general]\nunits = '{units}'\n", 1)
     text += f"\n[io]\nplot_64bit = {'true' if plot_64bit else 'false'}\ncontour_interval_dt = 0.025\n"
+    if units == "icf":
+        text += "format = 'h5'\n"  # [io] format (mod_input.f90:429): no XDMF companion
     with open(ini, "w") as fh:
         fh.write(text)
     run_host([ini, os.path.join(d, "out.h5"), "--contours", os.path.join(d, "c")], d)
     out = read_state(os.path.join(d, "out.h5"))
-    path = sorted(os.listdir(os.path.join(d, "c")))[-1]
+    files = sorted(os.listdir(os.path.join(d, "c")))
+    path = [f for f in files if f.endswith(".h5")][-1]
     c = read_contour(os.path.join(d, "c", path))
+    xdmf = [f for f in files if f.endswith(".xdmf")]
+    if units == "icf":
+        assert not xdmf
+    else:
+        # contour_writer_t%write_xdmf: every DataItem names a dataset of its step file with matching dimensions
+        import xml.etree.ElementTree as ET
+
+        assert len(xdmf) == len([f for f in files if f.endswith(".h5")])
+        root = ET.parse(os.path.join(d, "c", path.replace(".h5", ".xdmf"))).getroot()
+        grid = root.find("Domain/Grid")
+        assert grid.find("Time").get("Value").endswith(" s")
+        names = [a.get("Name") for a in grid.findall("Attribute")]
+        assert names == ["Density [g/cc]", "Sound Speed [cm/s]", "Pressure [barye]", "Velocity [cm/s]", "Mach",
+                         "Volume [cc]"]
+        with h5io.H5File(os.path.join(d, "c", path)) as f:
+            for it in root.iter("DataItem"):
+                if it.text and ":" in it.text:
+                    fname, ds = it.text.strip().split(":")
+                    assert fname == path
+                    shape = tuple(int(k) for k in it.get("Dimensions").split())
+                    assert f.read(ds).T.shape == shape, ds
     want = {"cgs": ("s", "cm", "cm/s", "barye"), "icf": ("ns", "um", "km/s", "Mbar")}[units]
     with h5io.H5File(os.path.join(d, "c", path)) as f:
         assert f.attr("/time", "units") == want[0] and f.attr("/time", "description") == "Simulation Time"
