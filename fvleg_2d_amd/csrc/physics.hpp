@@ -163,6 +163,22 @@ template <typename R> __device__ inline R tvd_phi(int lim, R r) {
   return (r + absd(r)) / (R(1) + absd(r));  // :200
 }
 
+// Precision form: phi(b / (a + 1e-16)) * a, the limited slope, written without
+// the ratio: minmod and superbee as their min/max forms (no division), van Leer
+// as (|a| b + a |b|) / (|a + 1e-16| + |b|) (one division instead of two).  With
+// the eps flush a is 0 or |a| >= 2.2e-16, so a + 1e-16 keeps the sign of a and
+// the forms agree with the ratio form up to 1e-16 / |a| relative.
+template <typename R, int LIMC> __device__ inline R tvd_slope(R a, R b) {
+  if constexpr (LIMC == LIM_VAN_LEER) {
+    return (absd(a) * b + a * absd(b)) / (absd(a + R(1e-16)) + absd(b));
+  } else {
+    if (!(a * b > R(0))) return R(0);
+    R m = (LIMC == LIM_MINMOD) ? fmin2(absd(a), absd(b))
+                               : fmax2(fmin2(R(2) * absd(b), absd(a)), fmin2(absd(b), R(2) * absd(a)));
+    return copysignr(m, a);
+  }
+}
+
 // muscl_tvd2.fypp:177-220
 template <typename R, int LIMC>
 __device__ inline void recon_tvd2(R qm, R q0, R q1, R q2, R &L, R &Rv) {
@@ -170,6 +186,11 @@ __device__ inline void recon_tvd2(R qm, R q0, R q1, R q2, R &L, R &Rv) {
   if (absd(dm) < Lim<R>::eps) dm = R(0);
   if (absd(dp) < Lim<R>::eps) dp = R(0);
   if (absd(d3) < Lim<R>::eps) d3 = R(0);
+  if constexpr (!std::is_same<R, double>::value) {
+    L = q0 + R(0.5) * tvd_slope<R, LIMC>(dm, dp);
+    Rv = q1 - R(0.5) * tvd_slope<R, LIMC>(d3, dp);
+    return;
+  }
   R r_L = dp / (dm + R(1e-16));
   R r_R = dp / (d3 + R(1e-16));
   R phi = tvd_phi<R>(LIMC, r_L);
