@@ -259,20 +259,31 @@ __device__ inline void flux_slau(R gamma, const EdgeState<R> &s, const PressNbr<
   const R eps = Lim<R>::eps;
   R rho_L = s.rL, u_L = s.uL, v_L = s.vL, p_L = s.pL;
   R rho_R = s.rR, u_R = s.uR, v_R = s.vR, p_R = s.pR;
+  // precision form: one reciprocal per shared denominator (rho_L, rho_R, c_bar)
+  // instead of a division per quotient; the FP64 form divides as the reference
+  constexpr bool kRcp = !std::is_same<R, double>::value;
+  R ir_L = R(0), ir_R = R(0), ic = R(0);
+  if constexpr (kRcp) {
+    ir_L = R(1) / rho_L;
+    ir_R = R(1) / rho_R;
+  }
+  auto over_rho = [&](R x, R rho, R ir) { return kRcp ? x * ir : x / rho; };
   R V_n_plus = u_L * n_x + v_L * n_y;
   R V_n_minus = u_R * n_x + v_R * n_y;
-  R c_plus = sqrtr(gamma * p_L / rho_L);
-  R c_minus = sqrtr(gamma * p_R / rho_R);
+  R c_plus = sqrtr(over_rho(gamma * p_L, rho_L, ir_L));
+  R c_minus = sqrtr(over_rho(gamma * p_R, rho_R, ir_R));
   R c_bar = R(0.5) * (c_plus + c_minus);
+  if constexpr (kRcp) ic = R(1) / c_bar;
+  auto over_c = [&](R x) { return kRcp ? x * ic : x / c_bar; };
   R V_n_bar = (rho_L * absd(V_n_plus) + rho_R * absd(V_n_minus)) / (rho_L + rho_R);
   R vel2 = sq(u_L) + sq(v_L) + sq(u_R) + sq(v_R);
-  R M_hat = fmin2(R(1), sqrtr(R(0.5) * vel2) / c_bar);
+  R M_hat = fmin2(R(1), over_c(sqrtr(R(0.5) * vel2)));
   R Chi = sq(R(1) - M_hat);
-  R M_bar = V_n_bar / c_bar;
+  R M_bar = over_c(V_n_bar);
   R p_bar = R(0.5) * (p_L + p_R);
   R rho_bar = R(0.5) * (rho_L + rho_R);
-  R M_plus = V_n_plus / c_bar;
-  R M_minus = V_n_minus / c_bar;
+  R M_plus = over_c(V_n_plus);
+  R M_minus = over_c(V_n_minus);
   R Beta_plus = (absd(M_plus) < R(1)) ? R(0.25) * (R(2) - M_plus) * sq(M_plus + R(1))
                                        : R(0.5) * (R(1) + copysignr(R(1), M_plus));
   R Beta_minus = (absd(M_minus) < R(1)) ? R(0.25) * (R(2) + M_minus) * sq(M_minus - R(1))
@@ -327,13 +338,13 @@ __device__ inline void flux_slau(R gamma, const EdgeState<R> &s, const PressNbr<
   R g = -(fmax2(fmin2(M_plus, R(0)), R(-1)) * fmin2(fmax2(M_minus, R(0)), R(1)));
   R base = R(0.5) * ((rho_L * V_n_plus) + (rho_R * V_n_minus) - absd(V_n_bar) * delta_rho) * (R(1) - g);
   R mass_flux;
-  if constexpr (SD) mass_flux = base - ((theta / (R(2) * c_bar)) * delta_p);
-  else mass_flux = base - ((Chi / (R(2) * c_bar)) * delta_p);
+  if constexpr (SD) mass_flux = base - ((kRcp ? theta * (R(0.5) * ic) : theta / (R(2) * c_bar)) * delta_p);
+  else mass_flux = base - ((kRcp ? Chi * (R(0.5) * ic) : Chi / (R(2) * c_bar)) * delta_p);
   R mfL = R(0.5) * (mass_flux + absd(mass_flux));
   R mfR = R(0.5) * (mass_flux - absd(mass_flux));
   R gg = gamma / (gamma - R(1));
-  R H_L = gg * (p_L / rho_L) + R(0.5) * (sq(u_L) + sq(v_L));
-  R H_R = gg * (p_R / rho_R) + R(0.5) * (sq(u_R) + sq(v_R));
+  R H_L = gg * over_rho(p_L, rho_L, ir_L) + R(0.5) * (sq(u_L) + sq(v_L));
+  R H_R = gg * over_rho(p_R, rho_R, ir_R) + R(0.5) * (sq(u_R) + sq(v_R));
   out[0] = mfL + mfR;
   out[1] = (mfL * u_L) + (mfR * u_R) + p_tilde * n_x;
   out[2] = (mfL * v_L) + (mfR * v_R) + p_tilde * n_y;
