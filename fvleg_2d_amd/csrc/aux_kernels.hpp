@@ -136,12 +136,21 @@ __device__ inline int err_code(int flags) {
   return 0;
 }
 
+// dt of the next step (main.f90:140-158): the startup period's dt, else a
+// constant dt or the given first dt, else the CFL minimum m.
+__device__ inline double next_dt(const LoopState *L, double m) {
+  if (L->startup) return (L->time <= L->startup_period) ? L->startup_dt : m;
+  if (L->constant_dt > 0.0) return L->constant_dt;
+  if (L->iter0 + L->iteration == 0 && L->initial_dt > 0.0) return L->initial_dt;
+  return m;
+}
+
 // Time-loop bookkeeping after a reduction of CFL minimum m and sanity flags e.
 // mode 1: initialise (first dt); mode 2: end of a step of main.f90:138-198
 // (master%time += dt; contour cadence; dt clip of the driver time).
 __device__ inline void loop_update(LoopState *L, double m, int e, int mode) {
   if (mode == 1) {
-    L->dt = (L->constant_dt > 0.0) ? L->constant_dt : m;
+    L->dt = next_dt(L, m);
     if (!(L->time < L->max_time && L->iteration < L->max_iterations)) L->done = 1;
   } else if (mode == 2) {
     const double EPS = Lim<double>::eps;
@@ -159,7 +168,7 @@ __device__ inline void loop_update(LoopState *L, double m, int e, int mode) {
       L->error = code;
       L->done = 1;
     }
-    L->dt = (L->constant_dt > 0.0) ? L->constant_dt : m;
+    L->dt = next_dt(L, m);
     if (!(L->time < L->max_time && L->iteration < L->max_iterations)) L->done = 1;
   }
 }

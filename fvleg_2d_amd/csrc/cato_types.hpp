@@ -63,6 +63,16 @@ struct LoopState {
   int done;
   int error;
   int pad[2];
+  // main.f90:140-158 dt choices besides constant_dt: a startup period of
+  // startup_dt while time <= startup_period (startup != 0), else initial_dt on
+  // the driver's iteration 0 (iter0 + iteration == 0; iter0 = the iteration
+  // count the run resumes from)
+  double initial_dt;
+  double startup_period;
+  double startup_dt;
+  long long iter0;
+  int startup;
+  int pad2[3];
 };
 
 // Convergence diagnostics of the step a finalize closes (calculate_convergence,

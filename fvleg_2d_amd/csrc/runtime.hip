@@ -473,6 +473,10 @@ struct cato_ctx {
   bool subcycle = false;
   double subcycle_tol = 1e-3;
   double last_dt = 0.0;
+  // cato_set_loop_dt: main.f90:140-158 dt choices of cato_run besides constant_dt
+  double loop_initial_dt = 0.0, loop_startup_period = 0.0, loop_startup_dt = 0.0;
+  int loop_startup = 0;
+  long long loop_iter0 = 0;
   bool hist_on = false;  // the caller asked for convergence rows (cato_set_diagnostics)
   std::vector<hipEvent_t> ev_pool;
   std::vector<std::pair<std::pair<hipEvent_t, hipEvent_t>, int>> pending;  // (start, stop), is_stage
@@ -1016,6 +1020,13 @@ static int run_impl(cato_ctx *c, double max_time, long long max_iter, double con
   L.constant_dt = constant_dt;
   L.iteration = 0;
   L.max_iterations = max_iter;
+  if (poll) {
+    L.initial_dt = c->loop_initial_dt;
+    L.startup = c->loop_startup;
+    L.startup_period = c->loop_startup_period;
+    L.startup_dt = c->loop_startup_dt;
+    L.iter0 = c->loop_iter0;
+  }
   HIPCK(hipMemcpyAsync(s.loop, &L, sizeof(L), hipMemcpyHostToDevice, c->stream));
   if ((rc = launch_cfl(c, 1))) return rc;
   long long launched = 0;
@@ -1128,6 +1139,20 @@ int cato_set_subcycle(cato_ctx *c, int enable, double tolerance) {
   c->subcycle = enable != 0;
   if (c->subcycle) c->subcycle_tol = tolerance;
   return set_conv_buffers(c, c->hist_on || c->subcycle);
+}
+
+int cato_set_loop_dt(cato_ctx *c, double initial_dt, int enable_startup, double startup_period, double startup_dt,
+                     long long start_iteration) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  if (enable_startup && !(startup_dt > 0.0)) return fail(CATO_ERR_INVALID, "startup_delta_t must be > 0");
+  if (start_iteration < 0) return fail(CATO_ERR_INVALID, "start_iteration must be >= 0");
+  c->loop_initial_dt = initial_dt > 0.0 ? initial_dt : 0.0;
+  c->loop_startup = enable_startup != 0;
+  c->loop_startup_period = startup_period;
+  c->loop_startup_dt = startup_dt;
+  c->loop_iter0 = start_iteration;
+  return CATO_OK;
 }
 
 int cato_last_dt(cato_ctx *c, double *dt) {

@@ -43,7 +43,8 @@ module cato_hip_api
             cato_get_conserved, cato_get_primitive, cato_set_energy_source, cato_set_diagnostics, &
             cato_get_convergence, cato_get_history, cato_compute_dt, cato_step, cato_run, cato_apply_bc, &
             cato_interpolate_edges, cato_edge_fluxes, cato_residual, cato_error_message, &
-            cato_set_pressure_input, cato_set_time, cato_set_subcycle, cato_last_dt
+            cato_set_pressure_input, cato_set_time, cato_set_subcycle, cato_last_dt, &
+            cato_set_loop_dt
 
   interface
     function cato_abi_version() bind(C, name="cato_abi_version") result(v)
@@ -135,6 +136,18 @@ module cato_hip_api
       type(c_ptr), value :: ctx
       integer(c_int), value :: enable
       real(c_double), value :: tolerance
+      integer(c_int) :: rc
+    end function
+
+    !> startup period / given first dt of cato_run (main.f90:140-158)
+    function cato_set_loop_dt(ctx, initial_dt, enable_startup, startup_period, startup_dt, start_iteration) &
+        bind(C, name="cato_set_loop_dt") result(rc)
+      import :: c_int, c_ptr, c_double, c_long_long
+      type(c_ptr), value :: ctx
+      real(c_double), value :: initial_dt
+      integer(c_int), value :: enable_startup
+      real(c_double), value :: startup_period, startup_dt
+      integer(c_long_long), value :: start_iteration
       integer(c_int) :: rc
     end function
 

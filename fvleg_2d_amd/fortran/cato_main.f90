@@ -172,19 +172,18 @@ program cato_hip_host
   contour_interval_dt = input%contour_interval_dt
   time = 0.0_rk
   iteration = 0
-  if (device_loop .and. (input%enable_startup_time .or. &
-                         (.not. input%use_constant_delta_t .and. input%initial_delta_t > 0.0_rk))) then
-    write(*, '(a)') 'Error: --device-loop does not support a given first time step (initial_delta_t) '// &
-      'or a startup period'
-    error stop 1
-  end if
   if (residual_hist) then
     call check_rc(cato_set_diagnostics(dev%ctx, 1_c_int), 'cato_set_diagnostics')
     open(newunit=hist_unit, file='residual_hist.csv', status='replace')
     write(hist_unit, '(a)') 'iteration,time,dt,rho,rho_u,rho_v,rho_E'
   end if
   if (device_loop) then
-    ! the device loop starts at the fluid time (the restart time) and counts its own iterations
+    ! the device loop starts at the fluid time (the restart time) and counts its own iterations;
+    ! main.f90:140-158: its startup period or given first dt, on the driver's iteration count
+    call check_rc(cato_set_loop_dt(dev%ctx, merge(input%initial_delta_t, 0.0_rk, &
+                                                  .not. input%use_constant_delta_t), &
+                                   merge(1_c_int, 0_c_int, input%enable_startup_time), input%startup_period, &
+                                   input%startup_delta_t, int(restart_iteration, c_long_long)), 'cato_set_loop_dt')
     call check_rc(cato_run(dev%ctx, input%max_time, int(input%max_iterations - restart_iteration, c_long_long), &
                            contour_interval_dt, merge(input%initial_delta_t, 0.0_rk, input%use_constant_delta_t), &
                            iters, time, e), 'cato_run')

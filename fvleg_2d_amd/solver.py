@@ -134,6 +134,13 @@ class Solver:
         norm exceeds the tolerance (SSP-RK2/3, host-driven steps)."""
         N.check(self.lib.cato_set_subcycle(self.h, int(on), float(tolerance)), "cato_set_subcycle")
 
+    def set_loop_dt(self, initial_dt: float = 0.0, startup: bool = False, startup_period: float = 0.0,
+                    startup_dt: float = 0.0, start_iteration: int = 0):
+        """main.f90:140-158 dt choices of run(): startup_dt while the driver time <= startup_period (startup),
+        else initial_dt on the driver's iteration 0 (start_iteration + the run's steps), else CFL/constant."""
+        N.check(self.lib.cato_set_loop_dt(self.h, float(initial_dt), int(bool(startup)), float(startup_period),
+                                          float(startup_dt), int(start_iteration)), "cato_set_loop_dt")
+
     def last_dt(self) -> float:
         """The dt the last step() used."""
         dt = C.c_double()

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define CATO_HIP_ABI_VERSION 3
+#define CATO_HIP_ABI_VERSION 4
 
 /* flux_solver names of the input deck (mod_fluid.f90:223-238) */
 enum {
@@ -153,6 +153,16 @@ int cato_step(cato_ctx *ctx, double dt, int *error_code);
  * few steps. */
 int cato_run(cato_ctx *ctx, double max_time, long long max_iterations, double contour_interval_dt, double constant_dt,
              long long *iterations, double *time, int *error_code);
+/* The other dt choices of the main.f90 time loop (main.f90:140-158) for the
+ * following cato_run calls: with enable_startup, dt = startup_dt while the driver
+ * time <= startup_period and the CFL dt after it ([time] enable_startup_time,
+ * startup_period, startup_delta_t); otherwise initial_dt > 0 is the dt of the
+ * driver's iteration 0 ([time] initial_delta_t without use_constant_delta_t),
+ * where the driver's iteration is start_iteration (a restart's iteration) plus
+ * the steps of this cato_run.  All in code units; 0 / 0 / 0 / 0 / 0 (the default)
+ * leaves CFL or constant_dt.  cato_advance always uses the CFL dt. */
+int cato_set_loop_dt(cato_ctx *ctx, double initial_dt, int enable_startup, double startup_period, double startup_dt,
+                     long long start_iteration);
 /* Energy source of fluid_t%time_derivative (mod_fluid.f90:500-509, with the field
  * that source_t%integrate returns for the step, ssp_rk_3_3 mod_fluid.f90:767-770):
  * q_dot (interior ni x nj, host layout ld/halo) is added to d(rho E)/dt in every

@@ -40,7 +40,7 @@ def test_abi_version_and_tiles_without_device():
     from fvleg_2d_amd import _native
 
     lib = _native.load()
-    assert lib.cato_abi_version() == 3
+    assert lib.cato_abi_version() == 4
     n = lib.cato_num_tiles()
     assert n >= 1
     tx, ty = ctypes.c_int(), ctypes.c_int()

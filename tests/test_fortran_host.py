@@ -192,14 +192,17 @@ def test_host_writes_contours_on_the_reference_cadence(tmp_path):
     assert all(b > a for a, b in zip(times, times[1:]))
 
 
-def oracle_main_loop(f, max_time, contour_dt, startup=None):
-    """main.f90:100-198 on an oracle Fluid: dt from the startup period (main.f90:141-145) or the CFL, the
-    contour-interval clip of the driver time; returns (iterations, driver time, residual rows)."""
+def oracle_main_loop(f, max_time, contour_dt, startup=None, initial=None):
+    """main.f90:100-198 on an oracle Fluid: dt from the startup period (main.f90:141-145), the given first dt
+    (main.f90:146-148) or the CFL, the contour-interval clip of the driver time; returns (iterations,
+    driver time, residual rows)."""
     eps = np.finfo(float).eps
     rows, master, time, it, next_out = [], 0.0, 0.0, 0, 0.0 + contour_dt
     while time < max_time:
-        if startup is not None and time <= startup[0]:
-            dt = startup[1]
+        if startup is not None:
+            dt = startup[1] if time <= startup[0] else f.get_timestep()
+        elif initial is not None and it == 0:
+            dt = initial
         else:
             dt = f.get_timestep()
         assert f.integrate(dt) == 0
@@ -216,17 +219,25 @@ def oracle_main_loop(f, max_time, contour_dt, startup=None):
 
 
 @pytest.mark.gpu
-def test_host_startup_period_uses_constant_dt(tmp_path, oracle):
-    """[time] enable_startup_time (main.f90:141-145): startup_delta_t while time <= startup_period, CFL after."""
+@pytest.mark.parametrize("device_loop", [False, True])
+@pytest.mark.parametrize("kind", ["startup", "initial"])
+def test_host_startup_period_uses_constant_dt(tmp_path, oracle, kind, device_loop):
+    """[time] enable_startup_time (main.f90:141-145): startup_delta_t while time <= startup_period, CFL after;
+    [time] initial_delta_t (main.f90:146-148): the given dt on iteration 0, CFL after.  Host-driven loop and
+    the device loop (cato_set_loop_dt) alike."""
     from fvleg_2d_amd.deck import read_state, write_deck
 
     pb = sod_problem(ni=100, nj=4, flux="AUSMPW+", limiter="van_leer", integrator="ssp_rk2", cfl=0.4, max_time=0.06)
     ini = write_deck(pb, str(tmp_path), "sod")
     period, dt0 = 0.01, 3e-4
     with open(ini, "a") as fh:
-        fh.write(f"\n[time]\nenable_startup_time = true\nstartup_period = {period!r}\nstartup_delta_t = {dt0!r}\n")
+        if kind == "startup":
+            fh.write(f"\n[time]\nenable_startup_time = true\nstartup_period = {period!r}\nstartup_delta_t = {dt0!r}\n")
+        else:
+            fh.write(f"\n[time]\ninitial_delta_t = {dt0!r}\n")
     out = os.path.join(str(tmp_path), "final.h5")
-    r = subprocess.run([host_or_fail(), ini, out], capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    cmd = [host_or_fail(), ini, out] + (["--device-loop"] if device_loop else [])
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr
     got = read_state(out)
     hist = read_residual_hist(os.path.join(str(tmp_path), "residual_hist.csv"))
@@ -237,9 +248,15 @@ def test_host_startup_period_uses_constant_dt(tmp_path, oracle):
                        gamma=cfg.gamma, cfl=cfg.cfl)
     og = oracle.Geometry(oc, pb.node_x, pb.node_y)
     f = oracle.Fluid(oc, og, *[np.pad(a, nh) for a in (pb.rho, pb.u, pb.v, pb.p)])
-    n, t, rows = oracle_main_loop(f, pb.max_time, pb.max_time, startup=(period, dt0))
+    if kind == "startup":
+        n, t, rows = oracle_main_loop(f, pb.max_time, pb.max_time, startup=(period, dt0))
+    else:
+        n, t, rows = oracle_main_loop(f, pb.max_time, pb.max_time, initial=dt0)
     dts = np.array(rows)[:, 2]
-    assert np.sum(dts == dt0) >= 30 and np.any(dts != dt0)  # both regimes ran
+    if kind == "startup":
+        assert np.sum(dts == dt0) >= 30 and np.any(dts != dt0)  # both regimes ran
+    else:
+        assert dts[0] == dt0 and np.all(dts[1:] != dt0)
     assert got["iteration"] == n and got["time"] == t
     assert np.array_equal(hist[:, 2], [float(f"{x:.6e}") for x in dts])
     for key, name in (("density", "rho"), ("rho_u", "rho_u"), ("rho_v", "rho_v"), ("rho_E", "rho_E")):
