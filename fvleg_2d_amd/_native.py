@@ -58,7 +58,7 @@ SIGNATURES = {
     "cato_set_time": (C.c_int, [_CTX, C.c_double]),
     "cato_set_subcycle": (C.c_int, [_CTX, C.c_int, C.c_double]),
     "cato_last_dt": (C.c_int, [_CTX, _DP]),
-    "cato_set_loop_dt": (C.c_int, [_CTX, C.c_double, C.c_int, C.c_double, C.c_double, C.c_longlong]),
+    "cato_set_loop_dt": (C.c_int, [_CTX, C.c_double, C.c_int, C.c_double, C.c_double, C.c_longlong, C.c_int]),
     "cato_get_primitive": (C.c_int, [_CTX, _DP, _DP, _DP, _DP, _DP, C.c_int, C.c_int]),
     "cato_compute_dt": (C.c_int, [_CTX, _DP]),
     "cato_step": (C.c_int, [_CTX, C.c_double, C.POINTER(C.c_int)]),

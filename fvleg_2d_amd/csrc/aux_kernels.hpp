@@ -138,11 +138,27 @@ __device__ inline int err_code(int flags) {
 
 // dt of the next step (main.f90:140-158): the startup period's dt, else a
 // constant dt or the given first dt, else the CFL minimum m.
-__device__ inline double next_dt(const LoopState *L, double m) {
+__device__ inline double chosen_dt(const LoopState *L, double m) {
   if (L->startup) return (L->time <= L->startup_period) ? L->startup_dt : m;
   if (L->constant_dt > 0.0) return L->constant_dt;
   if (L->iter0 + L->iteration == 0 && L->initial_dt > 0.0) return L->initial_dt;
   return m;
+}
+
+// With exact landing, no step carries the state's time past the next contour
+// time or max_time.  The loop moves next_output_time on one step after the
+// time reached it, so a time already on it aims at the one after.
+__device__ inline double next_dt(const LoopState *L, double m) {
+  double d = chosen_dt(L, m);
+  if (L->exact) {
+    double out = L->next_output_time;
+    if (!(out - L->master_time > 0.0)) out = out + L->contour_dt;
+    const double to_out = out - L->master_time;
+    const double to_end = L->max_time - L->master_time;
+    if (to_out > 0.0 && d > to_out) d = to_out;
+    if (to_end > 0.0 && d > to_end) d = to_end;
+  }
+  return d;
 }
 
 // Time-loop bookkeeping after a reduction of CFL minimum m and sanity flags e.

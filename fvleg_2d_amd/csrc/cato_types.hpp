@@ -66,13 +66,16 @@ struct LoopState {
   // main.f90:140-158 dt choices besides constant_dt: a startup period of
   // startup_dt while time <= startup_period (startup != 0), else initial_dt on
   // the driver's iteration 0 (iter0 + iteration == 0; iter0 = the iteration
-  // count the run resumes from)
+  // count the run resumes from).  exact != 0: the dt is also clipped so that
+  // the state's own time lands on the next contour time and on max_time (an
+  // option outside the reference, whose clip moves only the driver time)
   double initial_dt;
   double startup_period;
   double startup_dt;
   long long iter0;
   int startup;
-  int pad2[3];
+  int exact;
+  int pad2[2];
 };
 
 // Convergence diagnostics of the step a finalize closes (calculate_convergence,

@@ -475,7 +475,7 @@ struct cato_ctx {
   double last_dt = 0.0;
   // cato_set_loop_dt: main.f90:140-158 dt choices of cato_run besides constant_dt
   double loop_initial_dt = 0.0, loop_startup_period = 0.0, loop_startup_dt = 0.0;
-  int loop_startup = 0;
+  int loop_startup = 0, loop_exact = 0;
   long long loop_iter0 = 0;
   bool hist_on = false;  // the caller asked for convergence rows (cato_set_diagnostics)
   std::vector<hipEvent_t> ev_pool;
@@ -1026,6 +1026,7 @@ static int run_impl(cato_ctx *c, double max_time, long long max_iter, double con
     L.startup_period = c->loop_startup_period;
     L.startup_dt = c->loop_startup_dt;
     L.iter0 = c->loop_iter0;
+    L.exact = c->loop_exact;
   }
   HIPCK(hipMemcpyAsync(s.loop, &L, sizeof(L), hipMemcpyHostToDevice, c->stream));
   if ((rc = launch_cfl(c, 1))) return rc;
@@ -1142,7 +1143,7 @@ int cato_set_subcycle(cato_ctx *c, int enable, double tolerance) {
 }
 
 int cato_set_loop_dt(cato_ctx *c, double initial_dt, int enable_startup, double startup_period, double startup_dt,
-                     long long start_iteration) {
+                     long long start_iteration, int exact_landing) {
   int rc = check_ctx(c);
   if (rc) return rc;
   if (enable_startup && !(startup_dt > 0.0)) return fail(CATO_ERR_INVALID, "startup_delta_t must be > 0");
@@ -1152,6 +1153,7 @@ int cato_set_loop_dt(cato_ctx *c, double initial_dt, int enable_startup, double 
   c->loop_startup_period = startup_period;
   c->loop_startup_dt = startup_dt;
   c->loop_iter0 = start_iteration;
+  c->loop_exact = exact_landing != 0;
   return CATO_OK;
 }
 

@@ -140,7 +140,8 @@ module cato_hip_api
     end function
 
     !> startup period / given first dt of cato_run (main.f90:140-158)
-    function cato_set_loop_dt(ctx, initial_dt, enable_startup, startup_period, startup_dt, start_iteration) &
+    function cato_set_loop_dt(ctx, initial_dt, enable_startup, startup_period, startup_dt, start_iteration, &
+                              exact_landing) &
         bind(C, name="cato_set_loop_dt") result(rc)
       import :: c_int, c_ptr, c_double, c_long_long
       type(c_ptr), value :: ctx
@@ -148,6 +149,7 @@ module cato_hip_api
       integer(c_int), value :: enable_startup
       real(c_double), value :: startup_period, startup_dt
       integer(c_long_long), value :: start_iteration
+      integer(c_int), value :: exact_landing
       integer(c_int) :: rc
     end function
 

@@ -183,7 +183,8 @@ program cato_hip_host
     call check_rc(cato_set_loop_dt(dev%ctx, merge(input%initial_delta_t, 0.0_rk, &
                                                   .not. input%use_constant_delta_t), &
                                    merge(1_c_int, 0_c_int, input%enable_startup_time), input%startup_period, &
-                                   input%startup_delta_t, int(restart_iteration, c_long_long)), 'cato_set_loop_dt')
+                                   input%startup_delta_t, int(restart_iteration, c_long_long), 0_c_int), &
+                  'cato_set_loop_dt')
     call check_rc(cato_run(dev%ctx, input%max_time, int(input%max_iterations - restart_iteration, c_long_long), &
                            contour_interval_dt, merge(input%initial_delta_t, 0.0_rk, input%use_constant_delta_t), &
                            iters, time, e), 'cato_run')

@@ -135,11 +135,14 @@ class Solver:
         N.check(self.lib.cato_set_subcycle(self.h, int(on), float(tolerance)), "cato_set_subcycle")
 
     def set_loop_dt(self, initial_dt: float = 0.0, startup: bool = False, startup_period: float = 0.0,
-                    startup_dt: float = 0.0, start_iteration: int = 0):
+                    startup_dt: float = 0.0, start_iteration: int = 0, exact_landing: bool = False):
         """main.f90:140-158 dt choices of run(): startup_dt while the driver time <= startup_period (startup),
-        else initial_dt on the driver's iteration 0 (start_iteration + the run's steps), else CFL/constant."""
+        else initial_dt on the driver's iteration 0 (start_iteration + the run's steps), else CFL/constant.
+        exact_landing (not in the reference): steps are shortened so the state lands on the contour times and
+        max_time."""
         N.check(self.lib.cato_set_loop_dt(self.h, float(initial_dt), int(bool(startup)), float(startup_period),
-                                          float(startup_dt), int(start_iteration)), "cato_set_loop_dt")
+                                          float(startup_dt), int(start_iteration), int(bool(exact_landing))),
+                "cato_set_loop_dt")
 
     def last_dt(self) -> float:
         """The dt the last step() used."""

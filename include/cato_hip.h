@@ -160,9 +160,13 @@ int cato_run(cato_ctx *ctx, double max_time, long long max_iterations, double co
  * driver's iteration 0 ([time] initial_delta_t without use_constant_delta_t),
  * where the driver's iteration is start_iteration (a restart's iteration) plus
  * the steps of this cato_run.  All in code units; 0 / 0 / 0 / 0 / 0 (the default)
- * leaves CFL or constant_dt.  cato_advance always uses the CFL dt. */
+ * leaves CFL or constant_dt.  exact_landing != 0 (an option outside the reference,
+ * for runs that need the state at a given time) also shortens a step that would
+ * carry the fluid time past the next contour time or max_time, so the state lands
+ * on them; the reference (exact_landing = 0) clips only the driver time
+ * (main.f90:178-194).  cato_advance always uses the CFL dt. */
 int cato_set_loop_dt(cato_ctx *ctx, double initial_dt, int enable_startup, double startup_period, double startup_dt,
-                     long long start_iteration);
+                     long long start_iteration, int exact_landing);
 /* Energy source of fluid_t%time_derivative (mod_fluid.f90:500-509, with the field
  * that source_t%integrate returns for the step, ssp_rk_3_3 mod_fluid.f90:767-770):
  * q_dot (interior ni x nj, host layout ld/halo) is added to d(rho E)/dt in every

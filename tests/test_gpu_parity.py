@@ -444,6 +444,60 @@ def test_sod_config1_full_run(oracle):
         assert np.array_equal(g[k], o[k])
 
 
+def test_exact_landing_option(oracle):
+    """Outside the reference (which clips only the driver time, main.f90:178-194): with exact_landing the
+    device loop shortens the steps that would carry the state past the next contour time or max_time, so
+    the state lands on them.  The oracle steps with the same clipped dt; states, iterations and the final
+    time must agree bit for bit, and every contour time must be a step's end time."""
+    pb = sod()
+    nh = pb.cfg.n_ghost
+    T = pb.max_time
+    c = T / 4
+    oc = oracle_cfg(oracle, pb.cfg, pb.ni, pb.nj)
+    og = oracle.Geometry(oc, pb.node_x, pb.node_y)
+    f = oracle.Fluid(oc, og, *[pad(a, nh) for a in (pb.rho, pb.u, pb.v, pb.p)])
+    eps = np.finfo(float).eps
+    master = time = 0.0
+    nxt = 0.0 + c
+    it, ends, targets = 0, [], [nxt]
+
+    def landing(m):
+        out = nxt if nxt - master > 0.0 else nxt + c
+        if out - master > 0.0 and m > out - master:
+            m = out - master
+        if T - master > 0.0 and m > T - master:
+            m = T - master
+        return m
+
+    while time < T:
+        d = landing(f.get_timestep())
+        assert f.integrate(d) == 0
+        master = master + d
+        new = master
+        if abs(time - nxt) < eps:
+            nxt = nxt + c
+            targets.append(nxt)
+        if new > nxt:
+            new = time + abs(nxt - time)
+        time = new
+        it += 1
+        ends.append(master)
+    assert time == master == T
+    assert all(t in ends for t in targets if t <= T)
+    with Solver(pb.cfg, pb.node_x, pb.node_y) as s:
+        s.set_primitive(pb.rho, pb.u, pb.v, pb.p)
+        s.set_diagnostics(True)
+        s.set_loop_dt(exact_landing=True)
+        n_g, t_g, e_g = s.run(T, 0, c)
+        hist = s.history()
+        g = s.conserved()
+    say("exact landing", n_g, it, t_g)
+    assert e_g == 0 and n_g == it and t_g == T
+    assert np.array_equal(hist[:, 1], ends)
+    for k, name in enumerate(("rho", "rho_u", "rho_v", "rho_E")):
+        assert np.array_equal(g[k], f.interior(name)), name
+
+
 def test_riemann2d_reduced_full_run(oracle):
     """Four-quadrant Riemann problem (HLLC, MLP5, SSP-RK3, CFL 0.4) to t = 0.25 at 96^2."""
     pb = riemann2d(n=96, seed=1)
