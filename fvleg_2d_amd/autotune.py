@@ -1,7 +1,7 @@
 """Tile autotuner of the fused stage kernel.
 
 The stage kernel is compiled for every tile shape of ``cato_tile_shape``
-(32x8, 64x4, 16x16, 64x8 cells per workgroup).  The best shape depends on the
+(32x8, 32x16, 16x16, 64x8 cells per workgroup).  The best shape depends on the
 scheme (LDS footprint and register pressure of the flux/limiter variant), the
 precision form and the grid size (tile count against the 256 CUs), so it is
 measured rather than predicted: each candidate advances a copy of the caller's

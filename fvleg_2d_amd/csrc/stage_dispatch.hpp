@@ -13,7 +13,7 @@ struct TileShape {
   int tx, ty;
 };
 constexpr int kNumTiles = 4;
-constexpr TileShape kTiles[kNumTiles] = {{32, 8}, {64, 4}, {16, 16}, {64, 8}};
+constexpr TileShape kTiles[kNumTiles] = {{32, 8}, {32, 16}, {16, 16}, {64, 8}};
 
 using StageLauncher = void (*)(dim3 grid, hipStream_t stream, const StageArgs &args);
 
