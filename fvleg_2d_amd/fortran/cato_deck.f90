@@ -19,7 +19,7 @@ module cato_deck
   type :: cato_input_t
     character(len=256) :: title = ''
     ! [time]
-    real(rk) :: max_time = 0.0_rk
+    real(rk) :: max_time = 1.0_rk
     real(rk) :: initial_delta_t = 0.0_rk
     logical :: use_constant_delta_t = .false.
     real(rk) :: cfl = 0.1_rk
@@ -40,7 +40,7 @@ module cato_deck
     ! [physics]
     real(rk) :: polytropic_index = 5.0_rk / 3.0_rk
     ! [scheme]
-    character(len=32) :: flux_solver = ''
+    character(len=32) :: flux_solver = 'AUSMPW+'
     character(len=32) :: spatial_reconstruction = 'MUSCL'
     character(len=32) :: limiter = 'minmod'
     logical :: apply_low_mach_fix = .true.
@@ -57,7 +57,7 @@ module cato_deck
     character(len=256) :: bc_pressure_input_file = ''
     real(rk) :: bc_pressure_scale_factor = 1.0_rk
     ! [io]
-    real(rk) :: contour_interval_dt = huge(1.0_rk)
+    real(rk) :: contour_interval_dt = 0.1_rk
     logical :: plot_64bit = .true., plot_volume = .true.   ! mod_input.f90:432,443
     character(len=8) :: contour_format = 'xdmf'            ! [io] format (mod_input.f90:429-430)
     ! [general] units: the contour writer's unit system (units.f90:94-143)

@@ -106,6 +106,7 @@ program cato_hip_host
     write(*, '(a, 8(1x, i0))') 'config: flux limiter integrator bcs(+x -x +y -y) n_ghost =', cfg%flux, cfg%limiter, &
       cfg%integrator, cfg%bc_plus_x, cfg%bc_minus_x, cfg%bc_plus_y, cfg%bc_minus_y, cfg%n_ghost_layers
     write(*, '(a, 3(1x, es23.16))') 'gamma cfl max_time =', cfg%gamma, cfg%cfl, input%max_time
+    write(*, '(a, 1x, es23.16)') 'contour_interval_dt =', input%contour_interval_dt
     write(*, '(a, 2(1x, es23.16))') 'bc_pressure bc_density =', cfg%bc_pressure, cfg%bc_density
     write(*, '(a, 2(1x, es23.16))') 'x(1-nh,1-nh) y(ni+nh+1,nj+nh+1) =', grid%node_x(1 - nh, 1 - nh), &
       grid%node_y(grid%ni + nh + 1, grid%nj + nh + 1)

@@ -50,6 +50,25 @@ def test_host_reads_deck_without_device(tmp_path):
     assert float(cells[0]) == pb.rho[0, 0] and float(cells[1]) == pb.p[-1, -1]
 
 
+def test_host_deck_defaults_follow_the_reference(tmp_path):
+    """Keys a deck may omit take mod_input.f90's defaults: flux_solver 'AUSMPW+' (the component default,
+    mod_input.f90:132, read without one at :264; the reference's noh, rayleigh_taylor and richtmyer_meshkov
+    decks omit it) and contour_interval_dt 0.1 (mod_input.f90:436), which drives the driver-time clip."""
+    from fvleg_2d_amd.deck import write_deck
+
+    pb = sod_problem(ni=40, nj=4)
+    ini = write_deck(pb, str(tmp_path), "sod")
+    lines = [l for l in open(ini).read().splitlines()
+             if not l.startswith("flux_solver") and not l.startswith("contour_interval_dt")]
+    open(ini, "w").write("\n".join(lines) + "\n")
+    r = subprocess.run([host_or_fail(), ini, "--dry-run"], capture_output=True, text=True, timeout=60,
+                       cwd=str(tmp_path))
+    assert r.returncode == 0, r.stdout + r.stderr
+    cfg_line = [l for l in r.stdout.splitlines() if l.startswith("config:")][0]
+    assert cfg_line.split("=")[1].split()[0] == "0"  # AUSMPW+
+    assert float([l for l in r.stdout.splitlines() if l.startswith("contour_interval_dt")][0].split("=")[1]) == 0.1
+
+
 def test_host_rejects_unsupported_scheme(tmp_path):
     from fvleg_2d_amd.deck import write_deck
 
