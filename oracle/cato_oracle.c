@@ -930,7 +930,9 @@ static void flux_ausm_plus(double gamma, const edge_state *s, double n_x, double
  * 2009, sec. 10.4-10.6): PVRS pressure estimate, pressure-based wave speeds
  * (eq. 10.59-10.60), contact speed eq. 10.37, star states eq. 10.39.  The
  * reference hllc_solver.fypp:345-353 sketches exactly this (p_pvrs, p_star,
- * S_L = u_L - cs_L*q_L, ...) but is truncated, so parity is pinned by invariants. */
+ * S_L = u_L - cs_L*q_L, ...) but is truncated, so parity is pinned by what Toro's scheme fixes:
+ * supersonic upwinding, exact isolated contacts and shear waves, and Toro's alternative
+ * star-flux form (eq. 10.41-10.42) on general states. */
 static void flux_hllc(double gamma, const edge_state *s, double n_x, double n_y, double out[4]) {
   double rL = s->rL, uL = s->uL, vL = s->vL, pL = s->pL;
   double rR = s->rR, uR = s->uR, vR = s->vR, pR = s->pR;

@@ -9,10 +9,11 @@
  * (src/lib/field/field_2d.f90:477-483).  Edge states are (L/R, i, j) arrays,
  * edge fluxes are (4, i, j) arrays (src/lib/flux_solvers/slau_solver.fypp:209-210).
  *
- * Parity status per function is stated in oracle/README.md.  Parity unpinned (no check
- * independent of this restatement exists): the HLLC flux (the reference's body is incomplete;
- * textbook Toro), and the boundary time each stage of or_integrate evaluates a time-dependent
- * pressure input at (stage_bc_time, read from the reference's solver clocks).
+ * Parity status per function is stated in oracle/README.md.  The HLLC flux follows textbook Toro
+ * (the reference's body is incomplete) and is pinned against what that scheme fixes.  Parity
+ * unpinned (no check independent of this restatement exists): the boundary time each stage of
+ * or_integrate evaluates a time-dependent pressure input at (stage_bc_time, read from the
+ * reference's solver clocks).
  */
 #ifndef CATO_ORACLE_H
 #define CATO_ORACLE_H

@@ -10,6 +10,8 @@ import numpy as np
 import pytest
 
 from fvleg_2d_amd import SchemeConfig, Solver
+from error_cases import ERROR_CASES, error_case_fields, error_case_oracle
+from error_cases import NI as EC_NI, NJ as EC_NJ
 from fvleg_2d_amd.problems import kelvin_helmholtz, riemann2d, sod, uniform_nodes
 
 pytestmark = pytest.mark.gpu
@@ -555,24 +557,30 @@ def test_maximum_size_grid_conserves_totals():
         assert abs(tot1 - tot0) <= 1e-12 * mag
 
 
-def test_error_codes_nan_and_negative_density():
-    say("error codes")
-    ni, nj = 20, 10
-    cfg = SchemeConfig(flux="SLAU", limiter="minmod", integrator="ssp_rk2", bcs=("zero_gradient",) * 4)
-    nx, ny = uniform_nodes(ni, nj, 2)
-    rho = np.ones((ni, nj), order="F")
-    u = np.zeros((ni, nj), order="F")
-    p = np.ones((ni, nj), order="F")
+@pytest.mark.parametrize("name", sorted(ERROR_CASES))
+@pytest.mark.parametrize("loop", ["step", "run"])
+def test_error_codes_match_the_oracle(oracle, name, loop):
+    """fluid_t%integrate's error code (error.f90:6-9) for a NaN planted in the input, a NaN produced by
+    a stage, negative density, negative pressure and two at once, from cato_step and from the device
+    loop (cato_run, one step of constant dt): the exact code of the oracle's or_integrate, and for the
+    cases without NaN the same (finite, partly negative) state bit for bit."""
+    c = ERROR_CASES[name]
+    say("error code", name, loop)
+    cfg = SchemeConfig(flux="SLAU", limiter="minmod", integrator=c["integrator"], bcs=("zero_gradient",) * 4)
+    nx, ny = uniform_nodes(EC_NI, EC_NJ, cfg.n_ghost)
+    code_o, f = error_case_oracle(oracle, name)
     with Solver(cfg, nx, ny) as s:
-        r = rho.copy(order="F")
-        r[4, 4] = np.nan
-        s.set_primitive(r, u, u.copy(order="F"), p)
-        assert s.step(1e-3) == 3
-        r = rho.copy(order="F")
-        p2 = p.copy(order="F")
-        p2[7, 3] = 1e6  # violent blast with a huge dt drives density negative
-        s.set_primitive(r, u, u.copy(order="F"), p2)
-        assert s.step(0.5) in (1, 2, 3)
+        s.set_primitive(*error_case_fields(name))
+        if loop == "step":
+            code = s.step(c["dt"])
+        else:
+            it, _, code = s.run(1e300, 1, 1e300, c["dt"])
+            assert it == 1
+        got = s.conserved()
+    assert code == code_o == c["code"], (code, code_o, c["code"])
+    if code != 3:
+        for k, n in enumerate(("rho", "rho_u", "rho_v", "rho_E")):
+            assert np.array_equal(got[k], f.interior(n)), n
 
 
 def test_invalid_configuration_is_rejected():
