@@ -61,6 +61,8 @@ SIGNATURES = {
     "cato_set_loop_dt": (C.c_int, [_CTX, C.c_double, C.c_int, C.c_double, C.c_double, C.c_longlong, C.c_int]),
     "cato_get_primitive": (C.c_int, [_CTX, _DP, _DP, _DP, _DP, _DP, C.c_int, C.c_int]),
     "cato_compute_dt": (C.c_int, [_CTX, _DP]),
+    "cato_compute_dt_noalloc": (C.c_int, [_CTX, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                          C.c_void_p]),
     "cato_step": (C.c_int, [_CTX, C.c_double, C.POINTER(C.c_int)]),
     "cato_run": (C.c_int, [_CTX, C.c_double, C.c_longlong, C.c_double, C.c_double, C.POINTER(C.c_longlong),
                            _DP, C.POINTER(C.c_int)]),

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -33,6 +34,43 @@ int hip_check(hipError_t e, const char *what) {
     int _rc = hip_check((expr), #expr);                 \
     if (_rc != CATO_OK) return _rc;                     \
   } while (0)
+
+// ---------------------------------------------------------------------------
+// Device memory of the allocation-free entry point (cato_compute_dt_noalloc):
+// the per-workgroup CFL partial minima and sanity flags and the reduced
+// result live in this fixed array instead of an allocation.  Every GPU that
+// loads the library holds its own copy (about 48 KiB) until the process ends;
+// host code reaches it only through the per-device address that
+// hipGetSymbolAddress returns (noalloc_scratch), never through the symbol.
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocBlocks = 4096;  // partials per call: larger grids stream through 4096 workgroups
+constexpr size_t kNoAllocMinOff = 0;
+constexpr size_t kNoAllocErrOff = kNoAllocMinOff + kNoAllocBlocks * sizeof(double);
+constexpr size_t kNoAllocResOff = kNoAllocErrOff + kNoAllocBlocks * sizeof(int);
+constexpr size_t kNoAllocBytes = kNoAllocResOff + 2 * sizeof(double);
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+// Once per device, with that device current: the device address of `scratch`.
+static int noalloc_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(CATO_ERR_INVALID, "device id out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return CATO_OK;
+  void *ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) return hip_check(e != hipSuccess ? e : hipErrorInvalidSymbol, "scratch address");
+  g_scratch_ptr[device] = static_cast<unsigned char *>(ptr);
+  return CATO_OK;
+}
+
+static unsigned char *noalloc_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
 
 // ---------------------------------------------------------------------------
 // Geometry: quad_cell.f90:69-229 per cell; the HIP path accepts rectilinear
@@ -723,6 +761,10 @@ int cato_ctx_create(const cato_config *cfg, const cato_geom *geom, cato_ctx **ou
     return rc;
   }
   (void)hipGetDevice(&c->device);
+  if ((rc = noalloc_scratch_init(c->device))) {
+    delete c;
+    return rc;
+  }
   if ((rc = hip_check(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "stream"))) {
     delete c;
     return rc;
@@ -904,6 +946,37 @@ int cato_compute_dt(cato_ctx *c, double *dt) {
   HIPCK(hipStreamSynchronize(c->stream));
   if (dt) *dt = r[0];
   return CATO_OK;
+}
+
+int cato_compute_dt_noalloc(cato_ctx *c, const double *rho, const double *u, const double *v, const double *p,
+                            int ld, int halo, double *dt_device) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  const Solve &s = c->s;
+  if (!rho || !u || !v || !p || !dt_device) return fail(CATO_ERR_INVALID, "null device pointer");
+  if (halo < 0 || ld < s.ni + 2 * halo) return fail(CATO_ERR_INVALID, "bad leading dimension / halo");
+  unsigned char *base = noalloc_scratch(c->device);
+  if (!base) return fail(CATO_ERR_INVALID, "allocation-free scratch is not set up on this context's device");
+  const size_t n = (size_t)s.ni * s.nj;
+  const size_t want = (n + kThreads - 1) / kThreads;
+  // k_cfl_partial strides over the cells, so capping the workgroups at the
+  // scratch's kNoAllocBlocks partials changes who reduces which cell, not the
+  // minimum (min is exact in any order)
+  const int nb = (int)std::min(want, (size_t)kNoAllocBlocks);
+  double *bmin = reinterpret_cast<double *>(base + kNoAllocMinOff);
+  int *berr = reinterpret_cast<int *>(base + kNoAllocErrOff);
+  double *res = reinterpret_cast<double *>(base + kNoAllocResOff);
+  FieldGeom g = s.fgeom();
+  g.pitch = ld;  // the caller's layout: interior (1, 1) at halo + halo * ld
+  const size_t off = (size_t)halo + (size_t)halo * (size_t)ld;
+  hipLaunchKernelGGL(k_cfl_partial, dim3((unsigned)nb), dim3(kThreads), 0, c->stream, g, 1, rho + off, u + off,
+                     v + off, p + off, bmin, berr);
+  HIPCK(hipGetLastError());
+  hipLaunchKernelGGL(k_reduce_finalize, dim3(1), dim3(1024), 0, c->stream, nb, bmin, berr, res, (LoopState *)nullptr,
+                     0, ConvOut{});
+  HIPCK(hipGetLastError());
+  return hip_check(hipMemcpyAsync(dt_device, res, sizeof(double), hipMemcpyDeviceToDevice, c->stream),
+                   "compute_dt_noalloc");
 }
 
 int cato_step(cato_ctx *c, double dt, int *error_code) {

@@ -143,6 +143,18 @@ int cato_get_primitive(cato_ctx *ctx, double *rho, double *u, double *v, double 
 
 /* fluid_t%get_timestep */
 int cato_compute_dt(cato_ctx *ctx, double *dt);
+/* Allocation-free variant of cato_compute_dt, with no counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  The CFL time step (fluid_t%get_timestep, as cato_compute_dt computes it) of
+ * a state the caller holds in device memory: rho, u, v, p are device arrays in the host API's layout
+ * (ld, halo) of the context's grid; dt is written to the device double *dt_device on the context's
+ * stream, with no host synchronisation, no allocation and no memset.  The per-workgroup partial
+ * minima live in one fixed library array per GPU (4096 partials, about 48 KiB, held until the
+ * process ends); a grid with more than 4096 x 256 cells streams through 4096 workgroups, so no grid
+ * is too large.  Calls that overlap on one GPU (other contexts or streams) share that array and
+ * race: serialise them. */
+int cato_compute_dt_noalloc(cato_ctx *ctx, const double *rho, const double *u, const double *v, const double *p,
+                            int ld, int halo, double *dt_device);
 /* fluid_t%integrate with a host-given dt (one full SSP/RK step). */
 int cato_step(cato_ctx *ctx, double dt, int *error_code);
 /* main.f90 time loop on the device: while time < max_time and iteration <
