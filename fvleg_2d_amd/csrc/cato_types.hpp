@@ -100,6 +100,7 @@ struct StageArgs {
   int check_input;           // 1: flag NaN in the input primitives (the caller's, first stage)
   int last_stage;            // 1: write CFL partials and sanity flags for the new state
   int low_mach_fix;
+  int fast_ops;              // 1: FastOps edge arithmetic allowed on in-range tiles (gamma in (1, 4])
   int bc[4];                 // +x, -x, +y, -y
   int tiles_x, tiles_y;      // tile grid
   double gamma, cfl;

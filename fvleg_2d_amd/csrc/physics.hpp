@@ -154,13 +154,79 @@ __device__ inline Prim<double> cons_to_prim_fast(double r, double ru, double rv,
 }
 
 // ---------------------------------------------------------------------------
+// FP64 division and square root of the edge computations.
+//
+// IeeeOps: the correctly rounded operators.  hipcc lowers a / b to a range
+// scaling of both operands (v_div_scale), the reciprocal with two Newton steps,
+// the quotient with one correction (v_div_fmas) and a special-case fix-up
+// (v_div_fixup); sqrt(x) to an input scaling below 2^-767, v_rsq, Goldschmidt
+// steps, the inverse scaling and a select for 0 / inf.  While the scalings are
+// identities and no special case arises, dropping those steps leaves the same
+// operations on the same values, hence the same bits.  FastOps is that
+// remainder, with the reciprocal of a shared denominator formed once.  The
+// stage kernel uses it only on tiles whose operands keep every division in
+// range (numerator 0 or |a| >= 2^-969, 2^-1021 <= |b| < 2^1021, normal
+// quotient, exponents less than 768 apart; square roots of 2^-767 <= x < inf)
+// and only in formulas where a numerator -0 over a positive denominator, which
+// FastOps returns as +0, cannot change a result.
+// ---------------------------------------------------------------------------
+struct IeeeOps {
+  template <typename T> struct Rcp { T b; };
+  template <typename T> static __device__ Rcp<T> rcp(T b) { return Rcp<T>{b}; }
+  template <typename T> static __device__ T div(T a, Rcp<T> r) { return a / r.b; }
+  template <typename T> static __device__ T div(T a, T b) { return a / b; }
+  template <typename T> static __device__ T sqrt_pos(T x) { return sqrtr(x); }
+};
+struct FastOps {
+  template <typename T> struct Rcp { double b, y; };
+  static __device__ Rcp<double> rcp(double b) {
+    double y = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    return Rcp<double>{b, y};
+  }
+  static __device__ double div(double a, Rcp<double> r) {
+    const double q = a * r.y;
+    const double e = __builtin_fma(-r.b, q, a);
+    return __builtin_fma(e, r.y, q);
+  }
+  static __device__ double div(double a, double b) { return div(a, rcp(b)); }
+  static __device__ double sqrt_pos(double x) {
+    const double r = __builtin_amdgcn_rsq(x);
+    double g = x * r, h = r * 0.5;
+    const double e = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, e, g);
+    double d = __builtin_fma(-g, g, x);
+    h = __builtin_fma(h, e, h);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    return __builtin_fma(d, h, g);
+  }
+};
+
+// Operands for which the FastOps edge computations (TVD2 reconstruction and the
+// SLAU family) stay in range, by cell: rho and p in [2^-60, 2^60], u and v zero
+// or 2^-150 <= |u| <= 2^60.  Then every reconstructed value is 0 or within
+// [2^-218, 2^62] (rho, p within [2^-61, 2^61]), differences are 0 or at least
+// epsilon after their flush, and every quotient of those formulas lies within
+// [2^-405, 2^248] with operand exponents at most 300 apart (gamma in (1, 4]).
+__device__ inline bool fast_ops_cell(const Prim<double> &q) {
+  const double LO = 0x1p-60, HI = 0x1p60, VLO = 0x1p-150;
+  const double au = absd(q.u), av = absd(q.v);
+  return q.rho >= LO && q.rho <= HI && q.p >= LO && q.p <= HI && au <= HI && av <= HI &&
+         (au == 0.0 || au >= VLO) && (av == 0.0 || av >= VLO);
+}
+
+// ---------------------------------------------------------------------------
 // Reconstruction (edge states L/R at the interface between cells 0 and 1).
 // Stencil values along the edge normal: qm3=q(-2), qm=q(-1), q0, q1, q2, q3=q(+3).
 // ---------------------------------------------------------------------------
-template <typename R> __device__ inline R tvd_phi(int lim, R r) {
+template <typename R, class D = IeeeOps> __device__ inline R tvd_phi(int lim, R r) {
   if (lim == LIM_MINMOD) return fmax2(R(0), fmin2(r, R(1)));  // muscl_tvd2.fypp:196
   if (lim == LIM_SUPERBEE) return fmax2(fmax2(R(0), fmin2(R(2) * r, R(1))), fmin2(r, R(2)));  // :198
-  return (r + absd(r)) / (R(1) + absd(r));  // :200
+  return D::div(r + absd(r), R(1) + absd(r));  // :200 (numerator +0 or positive)
 }
 
 // Precision form: phi(b / (a + 1e-16)) * a, the limited slope, written without
@@ -179,8 +245,9 @@ template <typename R, int LIMC> __device__ inline R tvd_slope(R a, R b) {
   }
 }
 
-// muscl_tvd2.fypp:177-220
-template <typename R, int LIMC>
+// muscl_tvd2.fypp:177-220.  The flushes leave every difference +0 or at least
+// epsilon in magnitude, so no numerator below is -0 and |dm + 1e-16| >= 1.2e-16.
+template <typename R, int LIMC, class D = IeeeOps>
 __device__ inline void recon_tvd2(R qm, R q0, R q1, R q2, R &L, R &Rv) {
   R dm = q0 - qm, dp = q1 - q0, d3 = q2 - q1;
   if (absd(dm) < Lim<R>::eps) dm = R(0);
@@ -189,14 +256,14 @@ __device__ inline void recon_tvd2(R qm, R q0, R q1, R q2, R &L, R &Rv) {
   if constexpr (!std::is_same<R, double>::value) {
     L = q0 + R(0.5) * tvd_slope<R, LIMC>(dm, dp);
     Rv = q1 - R(0.5) * tvd_slope<R, LIMC>(d3, dp);
-    return;
+  } else {
+    R r_L = D::div(dp, dm + R(1e-16));
+    R r_R = D::div(dp, d3 + R(1e-16));
+    R phi = tvd_phi<R, D>(LIMC, r_L);
+    L = q0 + R(0.5) * phi * dm;
+    phi = tvd_phi<R, D>(LIMC, r_R);
+    Rv = q1 - R(0.5) * phi * d3;
   }
-  R r_L = dp / (dm + R(1e-16));
-  R r_R = dp / (d3 + R(1e-16));
-  R phi = tvd_phi<R>(LIMC, r_L);
-  L = q0 + R(0.5) * phi * dm;
-  phi = tvd_phi<R>(LIMC, r_R);
-  Rv = q1 - R(0.5) * phi * d3;
 }
 
 // muscl_mlp.fypp:196-279.  t_* are the transverse neighbours for tan(theta):
@@ -251,8 +318,10 @@ template <typename R> struct PressNbr {
   R p0p, p0m, p1p, p1m;    // (0,+1) (0,-1) (1,+1) (1,-1)
 };
 
-// SLAU family: slau_solver.fypp:355-490
-template <typename R, int VARIANT, int DIR>
+// SLAU family: slau_solver.fypp:355-490.  With D = FastOps: every numerator is
+// +0 or positive except V_n_plus / V_n_minus, whose quotients M_plus / M_minus
+// enter only Beta (even in a zero M) and g (whose sign of zero vanishes in 1 - g).
+template <typename R, int VARIANT, int DIR, class D = IeeeOps>
 __device__ inline void flux_slau(R gamma, const EdgeState<R> &s, const PressNbr<R> &pn, R out[4]) {
   const R n_x = DIR == 0 ? R(1) : R(0), n_y = DIR == 0 ? R(0) : R(1);
   const R THRESHOLD = R(5e-3), C_SD1 = R(0.01);
@@ -267,15 +336,25 @@ __device__ inline void flux_slau(R gamma, const EdgeState<R> &s, const PressNbr<
     ir_L = R(1) / rho_L;
     ir_R = R(1) / rho_R;
   }
-  auto over_rho = [&](R x, R rho, R ir) { return kRcp ? x * ir : x / rho; };
+  // FP64: IEEE quotients (D = IeeeOps) or the same quotients through one
+  // reciprocal per shared denominator rho_L, rho_R, c_bar (D = FastOps)
+  const auto dr_L = D::rcp(rho_L), dr_R = D::rcp(rho_R);
+  auto over_rho = [&](R x, R ir, decltype(dr_L) dr) {
+    if constexpr (kRcp) return x * ir;
+    else return D::div(x, dr);
+  };
   R V_n_plus = u_L * n_x + v_L * n_y;
   R V_n_minus = u_R * n_x + v_R * n_y;
-  R c_plus = sqrtr(over_rho(gamma * p_L, rho_L, ir_L));
-  R c_minus = sqrtr(over_rho(gamma * p_R, rho_R, ir_R));
+  R c_plus = D::sqrt_pos(over_rho(gamma * p_L, ir_L, dr_L));
+  R c_minus = D::sqrt_pos(over_rho(gamma * p_R, ir_R, dr_R));
   R c_bar = R(0.5) * (c_plus + c_minus);
   if constexpr (kRcp) ic = R(1) / c_bar;
-  auto over_c = [&](R x) { return kRcp ? x * ic : x / c_bar; };
-  R V_n_bar = (rho_L * absd(V_n_plus) + rho_R * absd(V_n_minus)) / (rho_L + rho_R);
+  const auto dc = D::rcp(c_bar);
+  auto over_c = [&](R x) {
+    if constexpr (kRcp) return x * ic;
+    else return D::div(x, dc);
+  };
+  R V_n_bar = D::div(rho_L * absd(V_n_plus) + rho_R * absd(V_n_minus), rho_L + rho_R);
   R vel2 = sq(u_L) + sq(v_L) + sq(u_R) + sq(v_R);
   R M_hat = fmin2(R(1), over_c(sqrtr(R(0.5) * vel2)));
   R Chi = sq(R(1) - M_hat);
@@ -324,7 +403,9 @@ __device__ inline void flux_slau(R gamma, const EdgeState<R> &s, const PressNbr<
 #pragma unroll
     for (int k = 1; k < 6; ++k) mx6 = fmax2(mx6, d[k]);
     R delta_p_max = fmax2(absd(delta_p), mx6);
-    theta = fmin2(R(1), sq(((C_SD1 * absd(delta_p) / p_bar) + C_SD1) / ((delta_p_max / p_bar) + C_SD1))) *
+    const auto dp_bar = D::rcp(p_bar);
+    theta = fmin2(R(1), sq(D::div((D::div(C_SD1 * absd(delta_p), dp_bar) + C_SD1),
+                                  (D::div(delta_p_max, dp_bar) + C_SD1)))) *
             fmax2(R(0), R(1) - absd(M_bar));
   } else {
     (void)pn;
@@ -338,13 +419,13 @@ __device__ inline void flux_slau(R gamma, const EdgeState<R> &s, const PressNbr<
   R g = -(fmax2(fmin2(M_plus, R(0)), R(-1)) * fmin2(fmax2(M_minus, R(0)), R(1)));
   R base = R(0.5) * ((rho_L * V_n_plus) + (rho_R * V_n_minus) - absd(V_n_bar) * delta_rho) * (R(1) - g);
   R mass_flux;
-  if constexpr (SD) mass_flux = base - ((kRcp ? theta * (R(0.5) * ic) : theta / (R(2) * c_bar)) * delta_p);
-  else mass_flux = base - ((kRcp ? Chi * (R(0.5) * ic) : Chi / (R(2) * c_bar)) * delta_p);
+  if constexpr (SD) mass_flux = base - ((kRcp ? theta * (R(0.5) * ic) : D::div(theta, R(2) * c_bar)) * delta_p);
+  else mass_flux = base - ((kRcp ? Chi * (R(0.5) * ic) : D::div(Chi, R(2) * c_bar)) * delta_p);
   R mfL = R(0.5) * (mass_flux + absd(mass_flux));
   R mfR = R(0.5) * (mass_flux - absd(mass_flux));
   R gg = gamma / (gamma - R(1));
-  R H_L = gg * over_rho(p_L, rho_L, ir_L) + R(0.5) * (sq(u_L) + sq(v_L));
-  R H_R = gg * over_rho(p_R, rho_R, ir_R) + R(0.5) * (sq(u_R) + sq(v_R));
+  R H_L = gg * over_rho(p_L, ir_L, dr_L) + R(0.5) * (sq(u_L) + sq(v_L));
+  R H_R = gg * over_rho(p_R, ir_R, dr_R) + R(0.5) * (sq(u_R) + sq(v_R));
   out[0] = mfL + mfR;
   out[1] = (mfL * u_L) + (mfR * u_R) + p_tilde * n_x;
   out[2] = (mfL * v_L) + (mfR * v_R) + p_tilde * n_y;
@@ -720,16 +801,22 @@ __device__ inline void flux_m_ausmpw(R gamma, int low_mach_fix, const EdgeState<
 
 template <int FLUX> constexpr bool needs_superbee_states() { return FLUX == FLUX_M_AUSMPW_PLUS; }
 
-template <typename R, int FLUX, int DIR>
+template <typename R, int FLUX, int DIR, class D = IeeeOps>
 __device__ inline void edge_flux(R gamma, int low_mach_fix, const EdgeState<R> &s, const PressNbr<R> &pn, R out[4],
                                  const EdgeState<R> *sb = nullptr) {
   if constexpr (FLUX == FLUX_AUSMPW_PLUS) flux_ausmpw<R, DIR>(gamma, low_mach_fix, s, pn, out);
   else if constexpr (FLUX == FLUX_M_AUSMPW_PLUS) flux_m_ausmpw<R, DIR>(gamma, low_mach_fix, s, *sb, pn, out);
   else if constexpr (FLUX == FLUX_SLAU || FLUX == FLUX_SLAU2 || FLUX == FLUX_SD_SLAU || FLUX == FLUX_SD_SLAU2)
-    flux_slau<R, FLUX, DIR>(gamma, s, pn, out);
+    flux_slau<R, FLUX, DIR, D>(gamma, s, pn, out);
   else if constexpr (FLUX == FLUX_ROE) flux_roe<R, DIR>(gamma, s, pn, out);
   else if constexpr (FLUX == FLUX_AUSM_PLUS) flux_ausm_plus<R, DIR>(gamma, s, out);
   else flux_hllc<R, DIR>(gamma, s, out);
+}
+
+// Schemes whose FP64 edge computations have a FastOps path: TVD2 reconstruction with the SLAU family.
+template <int FLUX, int LIMC> constexpr bool fast_ops_scheme() {
+  return (LIMC == LIM_MINMOD || LIMC == LIM_SUPERBEE || LIMC == LIM_VAN_LEER) &&
+         (FLUX == FLUX_SLAU || FLUX == FLUX_SLAU2 || FLUX == FLUX_SD_SLAU || FLUX == FLUX_SD_SLAU2);
 }
 
 template <int FLUX> constexpr bool needs_pressure_nbrs() {

@@ -272,6 +272,7 @@ StageArgs Solve::stage_args(int mode, double a, double b, double c, int in_buf, 
   s.check_input = check_input ? 1 : 0;
   s.last_stage = last ? 1 : 0;
   s.low_mach_fix = cfg.apply_low_mach_fix;
+  s.fast_ops = (cfg.gamma > 1.0 && cfg.gamma <= 4.0) ? 1 : 0;  // the range bounds of fast_ops_cell assume it
   s.bc[0] = cfg.bc_plus_x;
   s.bc[1] = cfg.bc_minus_x;
   s.bc[2] = cfg.bc_plus_y;

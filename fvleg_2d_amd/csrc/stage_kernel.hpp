@@ -173,7 +173,7 @@ template <int LIMC> constexpr int halo_of() { return LIMC == LIM_MLP5 ? 3 : 2; }
 // Reconstruct one variable at an edge from LDS row/column access.
 // s(k) returns q at normal offset k (k=-2..3) on the edge's row; t(k, dt) the
 // transverse neighbours.
-template <typename R, int LIMC>
+template <typename R, int LIMC, class D = IeeeOps>
 __device__ inline void recon_var(const R *base, int sn, int st, R &L, R &Rv) {
   // base points at cell 0 (left of the edge); sn = stride along the normal, st = transverse stride
   if constexpr (LIMC == LIM_MLP3 || LIMC == LIM_MLP5) {
@@ -182,7 +182,7 @@ __device__ inline void recon_var(const R *base, int sn, int st, R &L, R &Rv) {
     recon_mlp<R, (LIMC == LIM_MLP5 ? 5 : 3)>(qm3, base[-sn], base[0], base[sn], base[2 * sn], q3, base[st], base[-st],
                                                base[sn + st], base[sn - st], L, Rv);
   } else {
-    recon_tvd2<R, LIMC>(base[-sn], base[0], base[sn], base[2 * sn], L, Rv);
+    recon_tvd2<R, LIMC, D>(base[-sn], base[0], base[sn], base[2 * sn], L, Rv);
   }
 }
 
@@ -195,62 +195,29 @@ template <int LIMC, typename R, int TX, int TY> struct TileSmem {
   R sfj[4][TY + 1][TX];
 };
 
-// Block-wide scratch for the CFL / sanity / convergence reductions.
+// Block-wide scratch for the CFL / sanity / convergence reductions and the
+// per-wave votes on the FastOps operand ranges.
 struct RedSmem {
   double m[kThreads / 64];
   int e[kThreads / 64];
+  int fast[kThreads / 64];
   double s[kThreads / 64][8];
 };
 
-// One tile of one stage.  Writes the block's CFL partial and sanity flags to
-// a.block_min[tile] / a.block_err[tile] when requested.  OUTFLOW_COL: the tile
-// also writes the +x outflow ghost column of the new state (the batched
-// kernel); the single-solve launch leaves that to k_outflow_column, since the
-// extra epilogue slows every tile of the fused kernel by ~5%.
-template <int FLUX, int LIMC, typename R, int TX, int TY, bool OUTFLOW_COL = false>
-__device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileSmem<LIMC, R, TX, TY> &sm,
-                                  RedSmem &red) {
+// Phase 2 of a stage tile: every i- and j-edge of the tile once, reconstructed
+// from the LDS primitives and its flux written to LDS.  D: the division and
+// square root of the FP64 arithmetic (IeeeOps, or FastOps on in-range tiles).
+template <int FLUX, int LIMC, typename R, int TX, int TY, class D>
+__device__ inline void edge_pass(const StageArgs &a, TileSmem<LIMC, R, TX, TY> &sm, int i0, int j0) {
   constexpr int H = halo_of<LIMC>();
-  constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
+  constexpr int RX = TX + 2 * H;
   constexpr int NIE = (TX + 1) * TY, NJE = TX * (TY + 1);
-  // FP64 form: the reference's arithmetic bit for bit; precision form (f32x):
-  // a lighter FP64 epilogue (reciprocal-based primitive recovery and flux balance)
-  constexpr bool kExact = !std::is_same<R, f32x>::value;
   auto &sp = sm.sp;
   auto &sfi = sm.sfi;
   auto &sfj = sm.sfj;
-  auto &red_min = red.m;
-  auto &red_err = red.e;
-  const int tid = threadIdx.x;
-  const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
-  const int i0 = tx * TX + 1, j0 = ty * TY + 1;
-  const int ni = a.ni, nj = a.nj, nh = a.nh;
-  int err = 0;
-
-  // ---- 1. primitives of the tile + halo ---------------------------------
-  // (precision form outside MODE_RESIDUAL: recovered from U_k, and checked
-  // for NaN here, since no earlier stage stored and checked them)
-  const bool recover = !kExact && a.mode != MODE_RESIDUAL;
-  for (int idx = tid; idx < RX * RY; idx += kThreads) {
-    int ly = idx / RX, lx = idx - ly * RX;
-    int gi = i0 - H + lx, gj = j0 - H + ly;
-    Prim<double> pr{0.0, 0.0, 0.0, 0.0};
-    if (gi >= 1 - nh && gi <= ni + nh && gj >= 1 - nh && gj <= nj + nh) {
-      pr = recover ? load_recovered(a, gi, gj) : load_prim(a, gi, gj);
-      if ((recover || a.check_input) && gi >= 1 && gi <= ni && gj >= 1 && gj <= nj &&
-          (pr.rho != pr.rho || pr.u != pr.u || pr.v != pr.v || pr.p != pr.p))
-        err |= 1;
-    }
-    sp[0][ly][lx] = (R)pr.rho;
-    sp[1][ly][lx] = (R)pr.u;
-    sp[2][ly][lx] = (R)pr.v;
-    sp[3][ly][lx] = (R)pr.p;
-  }
-  __syncthreads();
-
-  // ---- 2. edge states and fluxes, each edge once --------------------------
+  const int ni = a.ni, nj = a.nj;
   const R gamma = (R)a.gamma;
-  for (int it = tid; it < NIE + NJE; it += kThreads) {
+  for (int it = threadIdx.x; it < NIE + NJE; it += kThreads) {
     const bool idir = it < NIE;
     int ex, ey;
     if (idir) {
@@ -264,14 +231,14 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
     // edge between cell (ci, cj) and its +normal neighbour
     int ci = idir ? i0 - 1 + ex : i0 + ex;
     int cj = idir ? j0 + ey : j0 - 1 + ey;
-    if (idir ? (ci > ni || cj > nj) : (ci > ni || cj > nj)) continue;
+    if (ci > ni || cj > nj) continue;
     int lx = ci - (i0 - H), ly = cj - (j0 - H);
     const int sn = idir ? 1 : RX, st = idir ? RX : 1;
     EdgeState<R> s;
-    recon_var<R, LIMC>(&sp[0][ly][lx], sn, st, s.rL, s.rR);
-    recon_var<R, LIMC>(&sp[1][ly][lx], sn, st, s.uL, s.uR);
-    recon_var<R, LIMC>(&sp[2][ly][lx], sn, st, s.vL, s.vR);
-    recon_var<R, LIMC>(&sp[3][ly][lx], sn, st, s.pL, s.pR);
+    recon_var<R, LIMC, D>(&sp[0][ly][lx], sn, st, s.rL, s.rR);
+    recon_var<R, LIMC, D>(&sp[1][ly][lx], sn, st, s.uL, s.uR);
+    recon_var<R, LIMC, D>(&sp[2][ly][lx], sn, st, s.vL, s.vR);
+    recon_var<R, LIMC, D>(&sp[3][ly][lx], sn, st, s.pL, s.pR);
     EdgeState<R> sb;  // superbee reconstruction of the same edge (M-AUSMPW+ only)
     if constexpr (needs_superbee_states<FLUX>()) {
       recon_var<R, LIM_SUPERBEE>(&sp[0][ly][lx], sn, st, sb.rL, sb.rR);
@@ -293,14 +260,82 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
     }
     R f[4];
     if (idir) {
-      edge_flux<R, FLUX, 0>(gamma, a.low_mach_fix, s, pn, f, &sb);
+      edge_flux<R, FLUX, 0, D>(gamma, a.low_mach_fix, s, pn, f, &sb);
 #pragma unroll
       for (int v = 0; v < 4; ++v) sfi[v][ey][ex] = f[v];
     } else {
-      edge_flux<R, FLUX, 1>(gamma, a.low_mach_fix, s, pn, f, &sb);
+      edge_flux<R, FLUX, 1, D>(gamma, a.low_mach_fix, s, pn, f, &sb);
 #pragma unroll
       for (int v = 0; v < 4; ++v) sfj[v][ey][ex] = f[v];
     }
+  }
+}
+
+// One tile of one stage.  Writes the block's CFL partial and sanity flags to
+// a.block_min[tile] / a.block_err[tile] when requested.  OUTFLOW_COL: the tile
+// also writes the +x outflow ghost column of the new state (the batched
+// kernel); the single-solve launch leaves that to k_outflow_column, since the
+// extra epilogue slows every tile of the fused kernel by ~5%.
+template <int FLUX, int LIMC, typename R, int TX, int TY, bool OUTFLOW_COL = false>
+__device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileSmem<LIMC, R, TX, TY> &sm,
+                                  RedSmem &red) {
+  constexpr int H = halo_of<LIMC>();
+  constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
+  // FP64 form: the reference's arithmetic bit for bit; precision form (f32x):
+  // a lighter FP64 epilogue (reciprocal-based primitive recovery and flux balance)
+  constexpr bool kExact = !std::is_same<R, f32x>::value;
+  auto &sp = sm.sp;
+  auto &sfi = sm.sfi;
+  auto &sfj = sm.sfj;
+  auto &red_min = red.m;
+  auto &red_err = red.e;
+  const int tid = threadIdx.x;
+  const int tx = tile % a.tiles_x, ty = tile / a.tiles_x;
+  const int i0 = tx * TX + 1, j0 = ty * TY + 1;
+  const int ni = a.ni, nj = a.nj, nh = a.nh;
+  int err = 0;
+
+  // ---- 1. primitives of the tile + halo ---------------------------------
+  // (precision form outside MODE_RESIDUAL: recovered from U_k, and checked
+  // for NaN here, since no earlier stage stored and checked them)
+  const bool recover = !kExact && a.mode != MODE_RESIDUAL;
+  // FP64 schemes with a FastOps path: whether every cell the edges read keeps
+  // the FastOps operations in range (fast_ops_cell)
+  constexpr bool kFastScheme = kExact && fast_ops_scheme<FLUX, LIMC>();
+  bool in_range = true;
+  for (int idx = tid; idx < RX * RY; idx += kThreads) {
+    int ly = idx / RX, lx = idx - ly * RX;
+    int gi = i0 - H + lx, gj = j0 - H + ly;
+    Prim<double> pr{0.0, 0.0, 0.0, 0.0};
+    if (gi >= 1 - nh && gi <= ni + nh && gj >= 1 - nh && gj <= nj + nh) {
+      pr = recover ? load_recovered(a, gi, gj) : load_prim(a, gi, gj);
+      if ((recover || a.check_input) && gi >= 1 && gi <= ni && gj >= 1 && gj <= nj &&
+          (pr.rho != pr.rho || pr.u != pr.u || pr.v != pr.v || pr.p != pr.p))
+        err |= 1;
+      if constexpr (kFastScheme) in_range = in_range && fast_ops_cell(pr);
+    }
+    sp[0][ly][lx] = (R)pr.rho;
+    sp[1][ly][lx] = (R)pr.u;
+    sp[2][ly][lx] = (R)pr.v;
+    sp[3][ly][lx] = (R)pr.p;
+  }
+  if constexpr (kFastScheme) {
+    const int vote = __all(in_range ? 1 : 0);
+    if ((tid & 63) == 0) red.fast[tid >> 6] = vote;
+  }
+  __syncthreads();
+
+  // ---- 2. edge states and fluxes, each edge once --------------------------
+  if constexpr (kFastScheme) {
+    bool fast = a.fast_ops != 0;
+#pragma unroll
+    for (int w = 0; w < kThreads / 64; ++w) fast = fast && red.fast[w] != 0;
+    if (fast)
+      edge_pass<FLUX, LIMC, R, TX, TY, FastOps>(a, sm, i0, j0);
+    else
+      edge_pass<FLUX, LIMC, R, TX, TY, IeeeOps>(a, sm, i0, j0);
+  } else {
+    edge_pass<FLUX, LIMC, R, TX, TY, IeeeOps>(a, sm, i0, j0);
   }
   __syncthreads();
 

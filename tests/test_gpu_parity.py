@@ -192,6 +192,42 @@ def test_step_parity_ssp_rk3(oracle, flux, lim):
         assert np.array_equal(gp[k], op[k]), ("prim", k)
 
 
+def fast_ops_edge_states(ni, nj, seed):
+    """States around the operand ranges of the FP64 FastOps edge arithmetic (fast_ops_cell: rho and p in
+    [2^-60, 2^60], u and v zero or 2^-150 <= |u| <= 2^60).  'scaled': density and pressure of every cell
+    times 1e20, so no tile qualifies; 'mixed': a patch of velocities 1e-300 (whose quotients need the
+    IEEE range scaling) in some tiles and negative-zero velocities (allowed) in others, next to tiles that
+    qualify."""
+    rho, u, v, p = shock_state(ni, nj, seed)
+    scaled = [rho * 1e20, u.copy(order="F"), v.copy(order="F"), p * 1e20]
+    u2, v2 = u.copy(order="F"), v.copy(order="F")
+    u2[4:12, 3:9] = 1e-300
+    v2[4:12, 3:9] = -1e-300
+    u2[40:56, 20:30] = -0.0
+    v2[40:56, 20:30] = -0.0
+    mixed = [rho.copy(order="F"), u2, v2, p.copy(order="F")]
+    return {"scaled": scaled, "mixed": mixed}
+
+
+@pytest.mark.parametrize("flux", ["SLAU", "SLAU2", "SD-SLAU", "SD-SLAU2"])
+@pytest.mark.parametrize("lim", ["minmod", "van_leer"])
+@pytest.mark.parametrize("case", ["scaled", "mixed"])
+def test_fast_ops_tiles_and_their_fallback(oracle, flux, lim, case):
+    """The stage kernel takes its FastOps edge arithmetic only on tiles whose cells keep it in range and
+    the IEEE sequences elsewhere: both bit-identical to the oracle, with symmetry ghosts (mirrored
+    velocities, negative zeros) at two sides."""
+    say("fast ops", flux, lim, case)
+    ni, nj = 64, 48
+    cfg = SchemeConfig(flux=flux, limiter=lim, integrator="ssp_rk3",
+                       bcs=("symmetry", "zero_gradient", "periodic", "periodic"), cfl=0.4)
+    nx, ny = uniform_nodes(ni, nj, cfg.n_ghost)
+    g, o, gp, op = run_steps(oracle, cfg, nx, ny, fast_ops_edge_states(ni, nj, 8)[case], 4)
+    for k in range(4):
+        assert np.array_equal(g[k], o[k]), (k, rel_linf(g[k], o[k]))
+    for k in range(5):
+        assert np.array_equal(gp[k], op[k]), ("prim", k)
+
+
 @pytest.mark.parametrize("flux, lim", [("AUSMPW+", "minmod"), ("SLAU", "MLP5"), ("M-AUSMPW+", "van_leer")])
 def test_step_parity_outflow(oracle, flux, lim):
     """outlet_bc.f90 at +x (subsonic extrapolation to the ambient pressure) with symmetry in y, 20 steps."""
