@@ -45,6 +45,7 @@ HBM_PEAK_GBS = 8000.0
 # (64 B); stages 2-3 read U^n and U^k and write U^k+1 (96 B).  SSP-RK3 step: 256 B.
 STEP_BYTES_PER_CELL = {"ssp_rk3": (64 + 96 + 96), "ssp_rk2": (64 + 96)}
 STAGES = {"ssp_rk3": 3, "ssp_rk2": 2}
+WARM_S = 1.0  # seconds of untimed stepping of the timed solve before its window opens
 
 
 def parse():
@@ -67,29 +68,19 @@ def say(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(pb, steps):
-    """The C oracle (reference operation order) on the same workload, its rows in parallel over the
-    host cores (OMP_NUM_THREADS, else every core); returns (rate, seconds, threads)."""
-    import numpy as np
+def cpu_leg(workload, n, steps, threads):
+    """The C oracle (reference operation order, OpenMP over rows) on a bounded sample of a workload, in a
+    process of its own started with OMP_NUM_THREADS = threads (oracle/cpu_rate.py)."""
+    import subprocess
 
-    threads = int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
-    os.environ["OMP_NUM_THREADS"] = str(threads)
-    os.environ["CATO_ORACLE_OMP"] = "1"
-    from oracle import oracle as O
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), CATO_ORACLE_OMP="1")
+    out = subprocess.run([sys.executable, "-m", "oracle.cpu_rate", "--workload", workload, "--n", str(n),
+                          "--steps", str(steps)], cwd=ROOT, env=env, check=True, capture_output=True, text=True)
+    return json.loads(out.stdout.strip().splitlines()[-1])
 
-    O.build()
-    cfg = pb.cfg
-    oc = O.Config(pb.ni, pb.nj, flux=cfg.flux, limiter=cfg.limiter, integrator=cfg.integrator, bcs=cfg.bcs,
-                  gamma=cfg.gamma, cfl=cfg.cfl)
-    og = O.Geometry(oc, pb.node_x, pb.node_y)
-    nh = cfg.n_ghost
-    f = O.Fluid(oc, og, *[np.pad(a, nh) for a in (pb.rho, pb.u, pb.v, pb.p)])
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        dt = f.get_timestep()
-        f.integrate(dt)
-    el = time.perf_counter() - t0
-    return pb.ni * pb.nj * steps / el, el, threads
+
+def host_threads():
+    return int(os.environ.get("OMP_NUM_THREADS") or os.cpu_count() or 1)
 
 
 def rel_errors(ref, got):
@@ -122,6 +113,15 @@ def timed_kh(n, precision, steps, warmup, tile, profile_only=False, nj=None):
         s.set_primitive(pb.rho, pb.u, pb.v, pb.p)
         s.advance(warmup)
         s.synchronize()
+        # after the W warm-up steps, keep stepping (untimed) until WARM_S seconds have passed since the
+        # solve started, so that the timed window does not open on clocks that idled during the
+        # host-side set-up
+        extra, t_w = 0, time.perf_counter()
+        while not profile_only and time.perf_counter() - t_w < WARM_S:
+            s.advance(8)
+            s.synchronize()
+            extra += 8
+        pb.meta["extra_warmup_steps"] = extra
         # per-launch time of every fused stage in the timed region: HIP events on the
         # solve's stream (markers only; the host enqueues far ahead of the GPU)
         s.set_timing(not profile_only)
@@ -129,10 +129,15 @@ def timed_kh(n, precision, steps, warmup, tile, profile_only=False, nj=None):
         s.advance(steps)
         s.synchronize()
         wall = time.perf_counter() - t0
+        # the timed region must have taken all its steps: a step after an error (or past the
+        # loop's end) returns at once and would inflate the rate
+        it, _, err = s.loop_status()
+        assert (it, err) == (steps, 0), f"timed region ran {it} of {steps} steps, error code {err}"
         if profile_only:
             return pb, wall, None, 0, None
         tm = s.timing()
         s.set_timing(False)
+        assert tm["stage_launches"] == STAGES[pb.cfg.integrator] * steps, tm
         state = s.conserved()
     return pb, wall, tm["stage_ms"] / max(tm["stage_launches"], 1), tm["stage_launches"], state
 
@@ -190,7 +195,7 @@ def device_graph_cost(pb_kh, kh_steps=200):
     return out
 
 
-def roofline(pb, stage_avg_ms, launches, precision, n):
+def roofline(pb, stage_avg_ms, launches, precision, n, measured_gbs=None):
     integ = pb.cfg.integrator
     cells = pb.ni * pb.nj
     bytes_per_launch = STEP_BYTES_PER_CELL[integ] / STAGES[integ] * cells
@@ -199,11 +204,9 @@ def roofline(pb, stage_avg_ms, launches, precision, n):
     tpath = os.path.join(ROOT, "profiles", "stage_kernel_traffic.json")
     if os.path.exists(tpath):
         try:
-            tj = json.load(open(tpath))
-            key = f"{n}_{precision}"
-            if key in tj:
-                traffic = tj[key].get("bytes_per_launch")
-        except Exception:
+            # calibrated HBM bytes per launch of the same workload, keyed by precision and tile
+            traffic = json.load(open(tpath))[f"{n}_{precision}"][str(pb.cfg.tile)]["bytes_per_launch"]
+        except (OSError, ValueError, KeyError):
             traffic = None
     return {
         "bound": "hbm",
@@ -216,6 +219,9 @@ def roofline(pb, stage_avg_ms, launches, precision, n):
         "algorithmic_bytes_per_launch": bytes_per_launch,
         "kernel_avg_ms": stage_avg_ms,
         "launches_timed": launches,
+        "measured_ceiling_gbs": measured_gbs,
+        "frac_of_measured_ceiling": (achieved / measured_gbs) if measured_gbs else None,
+        "measured_ceiling_kernel": "cato_hbm_probe triad a = b + s c, 3 x 1 GiB FP64 arrays (median of 20)",
     }
 
 
@@ -259,17 +265,24 @@ def riemann_forms():
     return res
 
 
-def copy_calibration():
-    """A plain device copy of known size: FETCH_SIZE / WRITE_SIZE of this kernel calibrate the counters."""
-    import torch
+def hbm_probe(kind, gib=1, reps=20):
+    """The library's streaming copy (kind 0) or triad (kind 1) over arrays of `gib` GiB: (GB/s, ms/launch)."""
+    import ctypes as C
 
-    nbytes = 1 << 30
-    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
-    b = torch.empty_like(a)
-    for _ in range(5):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 5}))
+    from fvleg_2d_amd import _native as N
+
+    gbs, ms = C.c_double(), C.c_double()
+    N.check(N.load().cato_hbm_probe(int(kind), int(gib) << 30, int(reps), C.byref(gbs), C.byref(ms)),
+            "cato_hbm_probe")
+    return gbs.value, ms.value
+
+
+def copy_calibration():
+    """A plain device copy of known size (the library's streaming copy kernel, 1 GiB read and 1 GiB
+    written per launch): FETCH_SIZE / WRITE_SIZE of this kernel calibrate the counters."""
+    gbs, ms = hbm_probe(0, gib=1, reps=5)
+    print(json.dumps({"copy_bytes_read": 1 << 30, "copy_bytes_written": 1 << 30, "launches": 5 + 3,
+                      "copy_gb_per_s": gbs, "ms_per_launch": ms}))
 
 
 def main():
@@ -290,6 +303,8 @@ def main():
         return
     cells = pb.ni * pb.nj
     value = cells * args.steps / wall
+    say("[bench] measured HBM ceiling (streaming triad, 3 x 1 GiB)")
+    triad_gbs, _ = hbm_probe(1, gib=1, reps=20)
     out = {
         "metric": METRIC,
         "value": value,
@@ -310,9 +325,10 @@ def main():
             "tile": pb.cfg.tile,
             "autotune": pb.meta.get("autotune"),
         },
-        "roofline": roofline(pb, stage_ms, launches, args.precision, args.n),
+        "roofline": roofline(pb, stage_ms, launches, args.precision, args.n, triad_gbs),
         "valu": valu_figure(stage_ms, args.precision, args.n, pb.cfg.tile),
     }
+    out["config"]["extra_warmup_steps"] = pb.meta.get("extra_warmup_steps")
     if args.forms == "all" and args.precision == "fp64":
         say("[bench] precision form (FP32 flux, FP64 state) on the same workload")
         pb32, wall32, ms32, l32, state32 = timed_kh(args.n, "fp32_flux", args.steps, args.warmup, args.tile)
@@ -322,7 +338,7 @@ def main():
             "ms_per_step": wall32 / args.steps * 1e3,
             "tile": pb32.cfg.tile,
             "autotune": pb32.meta.get("autotune"),
-            "roofline": roofline(pb32, ms32, l32, "fp32_flux", args.n),
+            "roofline": roofline(pb32, ms32, l32, "fp32_flux", args.n, triad_gbs),
             "valu": valu_figure(ms32, "fp32_flux", args.n, pb32.cfg.tile),
             "difference_from_fp64_state": rel_errors(state, state32),
         }
@@ -352,17 +368,27 @@ def main():
         say("[bench] served: Poisson arrivals vs a plain FP64 queue")
         out["served"] = measure_served(n=16)
     if not args.no_cpu and os.environ.get("CATO_BENCH_NO_CPU") != "1":
-        say("[bench] CPU baseline (C oracle, OpenMP over rows)")
-        rate, el, threads = cpu_baseline(pb, args.cpu_steps)
+        threads = host_threads()
+        say(f"[bench] CPU baseline (C oracle, OpenMP over rows, {threads} threads)")
+        leg = cpu_leg("kh", args.n, args.cpu_steps, threads)
         out["cpu_baseline"] = {
-            "value": rate,
+            "value": leg["cell_updates_per_s"],
             "unit": "cell-updates/s",
             "cores": threads,
             "kind": "port",
             "sample": f"{args.cpu_steps} SSP-RK3 steps of the same {pb.ni}x{pb.nj} workload with the C oracle "
                       f"(reference operation order, -O3 -ffp-contract=off, OpenMP over rows on {threads} threads), "
-                      f"{el:.1f} s",
+                      f"{leg['seconds']:.1f} s",
         }
+        if args.forms == "all":
+            say("[bench] CPU legs: one core, Sod 400x4 full run, Riemann 1024^2 sample")
+            legs = {
+                f"kh_{args.n}_1core_1step": cpu_leg("kh", args.n, 1, 1),
+                "sod_400x4_full_run_all_cores": cpu_leg("sod", 400, 0, threads),
+                "sod_400x4_full_run_1core": cpu_leg("sod", 400, 0, 1),
+                "riemann_1024_hllc_mlp5_2steps_all_cores": cpu_leg("riemann", 1024, 2, threads),
+            }
+            out["cpu_legs"] = legs
     print(json.dumps(out))
 
 

@@ -67,6 +67,7 @@ SIGNATURES = {
     "cato_run": (C.c_int, [_CTX, C.c_double, C.c_longlong, C.c_double, C.c_double, C.POINTER(C.c_longlong),
                            _DP, C.POINTER(C.c_int)]),
     "cato_advance": (C.c_int, [_CTX, C.c_longlong]),
+    "cato_loop_status": (C.c_int, [_CTX, C.POINTER(C.c_longlong), _DP, C.POINTER(C.c_int)]),
     "cato_set_diagnostics": (C.c_int, [_CTX, C.c_int]),
     "cato_get_convergence": (C.c_int, [_CTX, _DP]),
     "cato_get_history": (C.c_int, [_CTX, _DP, C.c_longlong, C.POINTER(C.c_longlong)]),
@@ -93,6 +94,7 @@ SIGNATURES = {
     "cato_server_poll": (C.c_int, [_CTX, C.c_int, C.POINTER(C.c_int), _DP, C.POINTER(C.c_longlong),
                                    C.POINTER(C.c_int)]),
     "cato_server_clock": (C.c_double, [_CTX]),
+    "cato_hbm_probe": (C.c_int, [C.c_int, C.c_ulonglong, C.c_int, _DP, _DP]),
 }
 
 

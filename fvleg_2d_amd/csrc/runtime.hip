@@ -1162,6 +1162,19 @@ int cato_advance(cato_ctx *c, long long n) {
   return run_impl(c, 1e300, n, 1e300, 0.0, false, n, nullptr, nullptr, nullptr);
 }
 
+int cato_loop_status(cato_ctx *c, long long *iterations, double *time, int *error_code) {
+  int rc = check_ctx(c);
+  if (rc) return rc;
+  LoopState H{};
+  HIPCK(hipMemcpyAsync(&H, c->s.loop, sizeof(H), hipMemcpyDeviceToHost, c->stream));
+  HIPCK(hipStreamSynchronize(c->stream));
+  if (c->timing) collect_timing(c);
+  if (iterations) *iterations = H.iteration;
+  if (time) *time = H.time;
+  if (error_code) *error_code = H.error;
+  return CATO_OK;
+}
+
 int cato_synchronize(cato_ctx *c) {
   int rc = check_ctx(c);
   if (rc) return rc;

@@ -185,6 +185,12 @@ class Solver:
     def advance(self, n: int):
         N.check(self.lib.cato_advance(self.h, int(n)), "cato_advance")
 
+    def loop_status(self):
+        """(steps taken, driver time, error code) of the last run / advance, after waiting for it."""
+        it, t, err = C.c_longlong(), C.c_double(), C.c_int()
+        N.check(self.lib.cato_loop_status(self.h, C.byref(it), C.byref(t), C.byref(err)), "cato_loop_status")
+        return it.value, t.value, err.value
+
     def synchronize(self):
         N.check(self.lib.cato_synchronize(self.h), "cato_synchronize")
 

@@ -200,6 +200,11 @@ int cato_set_pressure_input(cato_ctx *ctx, const double *time, const double *pre
 int cato_set_time(cato_ctx *ctx, double time);
 /* Enqueue exactly n steps with CFL dt and no host synchronisation (benchmarks). */
 int cato_advance(cato_ctx *ctx, long long n);
+/* The device-side loop of the last cato_run / cato_advance, after waiting for the context's stream:
+ * steps taken, driver time and the first error code (0 ALL_OK, 1, 2, 3 as error.f90:6-9).  A step
+ * after an error, or past the loop's end, returns at once without work, so a caller of cato_advance
+ * checks here that all n steps ran.  No counterpart in the reference (its loop is on the host). */
+int cato_loop_status(cato_ctx *ctx, long long *iterations, double *time, int *error_code);
 /* Device-side loop of cato_run / cato_advance as a captured hipGraph of 32 steps
  * (enable != 0, the default) or as individual kernel launches.  Both give
  * identical results; timing (cato_set_timing) always launches individually.
@@ -305,6 +310,12 @@ int cato_server_poll(cato_server *s, int id, int *done, double *completion_time,
                      int *error_code);
 /* Seconds since cato_server_create. */
 double cato_server_clock(cato_server *s);
+
+/* ---- measurement utility (no counterpart in the reference) ----
+ * The device's measured HBM ceiling: kind 0 streams a copy b -> a, kind 1 a triad a = b + s c, over
+ * FP64 arrays of bytes_per_array each (allocated and freed inside), reps timed launches; returns the
+ * median rate in GB/s of compulsory traffic (2 or 3 arrays) and its launch time. */
+int cato_hbm_probe(int kind, unsigned long long bytes_per_array, int reps, double *gb_per_s, double *ms_per_launch);
 
 #ifdef __cplusplus
 }
