@@ -186,9 +186,17 @@ program cato_hip_host
                                    merge(1_c_int, 0_c_int, input%enable_startup_time), input%startup_period, &
                                    input%startup_delta_t, int(restart_iteration, c_long_long), 0_c_int), &
                   'cato_set_loop_dt')
-    call check_rc(cato_run(dev%ctx, input%max_time, int(input%max_iterations - restart_iteration, c_long_long), &
-                           contour_interval_dt, merge(input%initial_delta_t, 0.0_rk, input%use_constant_delta_t), &
-                           iters, time, e), 'cato_run')
+    if (input%max_iterations - restart_iteration > 0) then
+      call check_rc(cato_run(dev%ctx, input%max_time, int(input%max_iterations - restart_iteration, c_long_long), &
+                             contour_interval_dt, merge(input%initial_delta_t, 0.0_rk, input%use_constant_delta_t), &
+                             iters, time, e), 'cato_run')
+    else
+      ! main.f90:138: a restart already at max_iterations takes no step (cato_run reads a count <= 0 as
+      ! no limit, so it is not called)
+      iters = 0
+      time = restart_time
+      e = 0
+    end if
     iteration = restart_iteration + int(iters)
     error_code = e
     if (residual_hist) call drain_history()
@@ -221,7 +229,12 @@ program cato_hip_host
       master_time = master_time + delta_t  ! master_puppeteer%time
       last_dt = delta_t
       new_time = master_time
-      if (error_code /= ALL_OK) exit
+      if (error_code /= ALL_OK) then
+        ! main.f90:170-175: the failing state is written to a contour before stopping
+        if (len_trim(contour_dir) > 0) call contour(iteration, master_time, last_dt)
+        last_io_iteration = iteration
+        exit
+      end if
       if (abs(time - next_output_time) < epsilon(1.0_rk)) then
         next_output_time = next_output_time + contour_interval_dt
         if (len_trim(contour_dir) > 0) call contour(iteration, master_time, last_dt)

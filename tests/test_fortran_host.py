@@ -547,3 +547,55 @@ def test_host_contour_units_and_precision(tmp_path, units, plot_64bit):
             fh.write(f"\n[restart]\nrestart_from_file = true\nrestart_file = '{os.path.join(d, 'c', path)}'\n")
         out2 = run_host([rest, os.path.join(d, "r.h5")], d)
         assert "Restart from" in out2
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("device_loop", [False, True])
+def test_host_restart_at_max_iterations_takes_no_step(tmp_path, device_loop):
+    """main.f90:138: a restart whose iteration is already max_iterations leaves the loop at once; the
+    output repeats the restart state, time and iteration (the device loop is not entered either)."""
+    from fvleg_2d_amd.deck import read_state, write_deck
+
+    pb = sod_problem(ni=100, nj=4, flux="SLAU", limiter="minmod", integrator="ssp_rk3", cfl=0.4, max_time=0.1)
+    d = str(tmp_path)
+    part = write_deck(pb, d, "part", max_iterations=25)
+    run_host([part, os.path.join(d, "b1.h5")], d)
+    rest = write_deck(pb, d, "rest", max_iterations=25)
+    with open(rest, "a") as fh:
+        fh.write(f"\n[restart]\nrestart_from_file = true\nrestart_file = '{os.path.join(d, 'b1.h5')}'\n")
+    run_host([rest, os.path.join(d, "b2.h5")] + (["--device-loop"] if device_loop else []), d)
+    b1, b2 = read_state(os.path.join(d, "b1.h5")), read_state(os.path.join(d, "b2.h5"))
+    assert b1["iteration"] == b2["iteration"] == 25
+    assert b2["time"] == b1["time"]
+    for key in ("density", "rho_u", "rho_v", "rho_E"):
+        assert np.array_equal(b2[key], b1[key]), key
+
+
+@pytest.mark.gpu
+def test_host_writes_the_failing_state_to_a_contour(tmp_path):
+    """main.f90:170-175: when integrate reports an error the state is written to a contour of the current
+    iteration before the program stops with an error, also when that iteration already had one (here the
+    first step: step_0000000.h5 then holds the failed state, not the initial one)."""
+    import dataclasses
+
+    from error_cases import ERROR_CASES, NI, NJ, error_case_fields
+    from fvleg_2d_amd.deck import read_contour, write_deck
+    from fvleg_2d_amd.problems import riemann2d
+
+    case = ERROR_CASES["negative_density_and_pressure"]
+    rho, u, v, p = error_case_fields("negative_density_and_pressure")
+    base = riemann2d(n=NI, nj=NJ, flux="SLAU", limiter="minmod", integrator=case["integrator"])
+    pb = dataclasses.replace(base, rho=rho, u=u, v=v, p=p, max_time=1.0, contour_interval_dt=1.0)
+    ini = write_deck(pb, str(tmp_path), "fail")
+    with open(ini, "a") as fh:
+        fh.write(f"\n[time]\nuse_constant_delta_t = true\ninitial_delta_t = {case['dt']!r}\n")
+    cdir = os.path.join(str(tmp_path), "results")
+    r = subprocess.run([host_or_fail(), ini, os.path.join(str(tmp_path), "out.h5"), "--contours", cdir],
+                       capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    print(r.stdout[-1500:], flush=True)
+    assert r.returncode != 0 and "error code 1" in r.stdout
+    c = read_contour(os.path.join(cdir, "step_0000000.h5"))
+    # contour scalars and fields are stored in single precision unless plot_64bit
+    assert c["iteration"] == 0 and abs(c["time"] - case["dt"]) < 1e-7 and abs(c["delta_t"] - case["dt"]) < 1e-7
+    nh = 2
+    assert np.min(c["density"][nh:-nh, nh:-nh]) < 0.0
