@@ -95,6 +95,14 @@ SIGNATURES = {
                                    C.POINTER(C.c_int)]),
     "cato_server_clock": (C.c_double, [_CTX]),
     "cato_hbm_probe": (C.c_int, [C.c_int, C.c_ulonglong, C.c_int, _DP, _DP]),
+    "cato_split_create": (C.c_int, [C.POINTER(CatoConfig), C.POINTER(CatoGeom), C.c_int, C.c_int,
+                                    C.POINTER(_CTX)]),
+    "cato_split_destroy": (C.c_int, [_CTX]),
+    "cato_split_set_primitive": (C.c_int, [_CTX, _DP, _DP, _DP, _DP, C.c_int, C.c_int]),
+    "cato_split_get_conserved": (C.c_int, [_CTX, _DP, _DP, _DP, _DP, C.c_int, C.c_int]),
+    "cato_split_compute_dt": (C.c_int, [_CTX, _DP]),
+    "cato_split_step": (C.c_int, [_CTX, C.c_double, C.POINTER(C.c_int)]),
+    "cato_split_exchange_time": (C.c_int, [_CTX, _DP, C.POINTER(C.c_longlong)]),
 }
 
 

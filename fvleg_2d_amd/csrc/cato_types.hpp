@@ -120,6 +120,27 @@ struct StageArgs {
   int *block_err;            // per-block sanity flags
   const double *conv_first[4];  // last stage: the state calculate_convergence compares against (null: off)
   double *block_conv;        // per-block sums for the convergence norms, 8 arrays of tiles_x*tiles_y
+  // A sub-domain of a split solve: sides whose bit is set (1 +x, 2 -x, 4 +y, 8 -y) take their ghost
+  // cells from the halo store (neighbour primitives exchanged after every stage) instead of a boundary
+  // condition; layout in halo_store_index
+  int halo_mask;
+  const double *halo;
 };
+
+// Halo store of a sub-domain: the +x and -x strips [var][layer][row] (4 nh nj each), the +y and -y strips
+// [var][layer][col] (4 nh ni each) and the corners +x+y, -x+y, +x-y, -x-y [var][layer_y][layer_x]
+// (4 nh^2 each).  Layer k of a +side is ghost n + 1 + k, of a -side ghost k - nh + 1.
+__host__ __device__ inline size_t halo_store_size(int ni, int nj, int nh) {
+  return (size_t)8 * nh * (ni + nj) + (size_t)16 * nh * nh;
+}
+__host__ __device__ inline size_t halo_region_offset(int region, int ni, int nj, int nh) {
+  const size_t xs = (size_t)4 * nh * nj, ys = (size_t)4 * nh * ni, cs = (size_t)4 * nh * nh;
+  if (region < 2) return region * xs;
+  if (region < 4) return 2 * xs + (region - 2) * ys;
+  return 2 * xs + 2 * ys + (region - 4) * cs;
+}
+__host__ __device__ inline size_t halo_region_size(int region, int ni, int nj, int nh) {
+  return region < 2 ? (size_t)4 * nh * nj : region < 4 ? (size_t)4 * nh * ni : (size_t)4 * nh * nh;
+}
 
 }  // namespace cato

@@ -301,6 +301,8 @@ StageArgs Solve::stage_args(int mode, double a, double b, double c, int in_buf, 
   s.dt_value = dt_value;
   s.block_min = block_min;
   s.block_err = block_err;
+  s.halo_mask = halo_mask;
+  s.halo = halo;
   return s;
 }
 

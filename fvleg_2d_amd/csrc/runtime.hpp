@@ -46,6 +46,9 @@ struct Solve {
   // sums, the 4 norms of the last step, and a ring of history rows
   double *conv_sums = nullptr, *conv_norms = nullptr, *conv_hist = nullptr;
   static constexpr int kHistCap = 64;
+  // sub-domain of a split solve: sides that take their ghosts from the halo store (owned by the split)
+  int halo_mask = 0;
+  double *halo = nullptr;
   int cur = 0;             // which U[] holds U^n
   bool prims_valid = false; // P[0] holds the caller's primitives (not yet checked by a stage)
   bool prims_stale = false; // precision form: stages left P[0] behind U[cur] (they store no primitives)

@@ -99,11 +99,55 @@ __device__ inline int wave_or(int v) {
   return v;
 }
 
+// Sub-domain of a split solve: halo'd cell (gi, gj) when a side takes its
+// ghosts from the halo store.  The physical sides map first (as the reference
+// fills boundary ghosts after the exchange, over the full transverse extent),
+// then the cell is interior, in an exchanged strip or in an exchanged corner.
+__device__ inline Prim<double> load_prim_split(const StageArgs &a, int gi, int gj) {
+  const int ni = a.ni, nj = a.nj, nh = a.nh;
+  const bool xh = (gi > ni && (a.halo_mask & 1)) || (gi < 1 && (a.halo_mask & 2));
+  const bool yh = (gj > nj && (a.halo_mask & 4)) || (gj < 1 && (a.halo_mask & 8));
+  bool fu = false, fv = false;
+  const int si = xh ? gi : map_axis(gi, ni, a.bc[0], a.bc[1], fu);
+  const int sj = yh ? gj : map_axis(gj, nj, a.bc[2], a.bc[3], fv);
+  const bool xo = si < 1 || si > ni, yo = sj < 1 || sj > nj;
+  Prim<double> pr;
+  if (!xo && !yo) {
+    const size_t off = (size_t)(sj - 1) * a.pitch + (size_t)(si - 1);
+    pr = Prim<double>{a.prim[0][off], a.prim[1][off], a.prim[2][off], a.prim[3][off]};
+  } else {
+    const int kx = si > ni ? si - ni - 1 : si + nh - 1, ky = sj > nj ? sj - nj - 1 : sj + nh - 1;
+    int region;
+    size_t stride, pos;  // element (v, layer, pos) at v * stride + layer * len + pos
+    size_t len;
+    if (xo && !yo) {
+      region = si > ni ? 0 : 1;
+      len = (size_t)nj;
+      pos = (size_t)kx * len + (size_t)(sj - 1);
+    } else if (!xo) {
+      region = sj > nj ? 2 : 3;
+      len = (size_t)ni;
+      pos = (size_t)ky * len + (size_t)(si - 1);
+    } else {
+      region = 4 + (si > ni ? 0 : 1) + (sj > nj ? 0 : 2);
+      len = (size_t)nh;
+      pos = (size_t)ky * len + (size_t)kx;
+    }
+    stride = (size_t)nh * len;
+    const double *h = a.halo + halo_region_offset(region, ni, nj, nh) + pos;
+    pr = Prim<double>{h[0], h[stride], h[2 * stride], h[3 * stride]};
+  }
+  if (fu) pr.u = -pr.u;
+  if (fv) pr.v = -pr.v;
+  return pr;
+}
+
 // Primitives of halo'd cell (gi, gj), with the BC mapping.  The stored
 // primitives of a state are those fluid_t%calculate_derived_quantities gives
 // for it (written by the stage that produced the state), or the caller's on
 // the first stage after cato_set_primitive.
 __device__ inline Prim<double> load_prim(const StageArgs &a, int gi, int gj) {
+  if (a.halo_mask) return load_prim_split(a, gi, gj);
   bool fu, fv;
   int si = map_axis(gi, a.ni, a.bc[0], a.bc[1], fu);
   // +x outflow / pressure input: every ghost layer reads column ni+1 of the

@@ -257,3 +257,60 @@ class Solver:
         N.check(self.lib.cato_residual(self.h, _p(rho), _p(u), _p(v), _p(p), *[_p(x) for x in d],
                                        self.halo_shape()[0]), "cato_residual")
         return d
+
+
+class SplitSolver:
+    """One solve split into px x py sub-domains on one GPU with halos exchanged after every RK stage
+    (``cato_split_*``): the decomposition a run over several GPUs uses, bit-identical to ``Solver``."""
+
+    def __init__(self, cfg: SchemeConfig, node_x: np.ndarray, node_y: np.ndarray, px: int, py: int):
+        self.lib = N.load()
+        self.cfg = cfg
+        self.node_x, self.node_y = _f64f(node_x), _f64f(node_y)
+        nh = cfg.n_ghost
+        self.ni = self.node_x.shape[0] - 1 - 2 * nh
+        self.nj = self.node_x.shape[1] - 1 - 2 * nh
+        self._ccfg = cfg.to_c()
+        g = N.CatoGeom(self.ni, self.nj, _p(self.node_x), _p(self.node_y))
+        h = C.c_void_p()
+        N.check(self.lib.cato_split_create(C.byref(self._ccfg), C.byref(g), int(px), int(py), C.byref(h)),
+                "cato_split_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.cato_split_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_primitive(self, rho, u, v, p):
+        arrs = [_f64f(a) for a in (rho, u, v, p)]
+        N.check(self.lib.cato_split_set_primitive(self.h, *[_p(a) for a in arrs], self.ni, 0),
+                "cato_split_set_primitive")
+
+    def conserved(self):
+        out = [np.zeros((self.ni, self.nj), order="F") for _ in range(4)]
+        N.check(self.lib.cato_split_get_conserved(self.h, *[_p(a) for a in out], self.ni, 0),
+                "cato_split_get_conserved")
+        return out
+
+    def compute_dt(self) -> float:
+        dt = C.c_double()
+        N.check(self.lib.cato_split_compute_dt(self.h, C.byref(dt)), "cato_split_compute_dt")
+        return dt.value
+
+    def step(self, dt: float) -> int:
+        err = C.c_int()
+        N.check(self.lib.cato_split_step(self.h, float(dt), C.byref(err)), "cato_split_step")
+        return err.value
+
+    def exchange_time(self):
+        """(GPU ms of all halo exchanges so far, number of exchanges)."""
+        ms, n = C.c_double(), C.c_longlong()
+        N.check(self.lib.cato_split_exchange_time(self.h, C.byref(ms), C.byref(n)), "cato_split_exchange_time")
+        return ms.value, n.value

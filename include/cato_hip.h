@@ -311,6 +311,26 @@ int cato_server_poll(cato_server *s, int id, int *done, double *completion_time,
 /* Seconds since cato_server_create. */
 double cato_server_clock(cato_server *s);
 
+/* ---- one solve split into px x py sub-domains on one GPU ----
+ * The shape a run of one large solve over several GPUs takes: the reference's near-square blocks per
+ * coarray image (mod_parallel.f90:41-124) with nh-deep edge and corner halos of the primitives exchanged
+ * after every RK stage (field_2d.f90:551-654) and the CFL dt as the minimum over the blocks
+ * (mod_fluid.f90:560).  Here every block is a sub-domain on the same GPU and the exchange is a pack kernel
+ * (each block gathers what its neighbours need into a send buffer) and an unpack kernel (each block pulls
+ * its neighbours' send regions into its halo store).  Results are bit-identical to the unsplit solve.
+ * FP64 form; periodic, zero-gradient and symmetry sides.  Host arrays as for cato_ctx (whole grid). */
+typedef struct cato_split cato_split;
+int cato_split_create(const cato_config *cfg, const cato_geom *geom, int px, int py, cato_split **out);
+int cato_split_destroy(cato_split *split);
+int cato_split_set_primitive(cato_split *split, const double *rho, const double *u, const double *v,
+                             const double *p, int ld, int halo);
+int cato_split_get_conserved(cato_split *split, double *rho, double *rho_u, double *rho_v, double *rho_E, int ld,
+                             int halo);
+int cato_split_compute_dt(cato_split *split, double *dt);
+int cato_split_step(cato_split *split, double dt, int *error_code);
+/* GPU time of the halo exchanges so far (pack + unpack of every block, HIP events) and their number. */
+int cato_split_exchange_time(cato_split *split, double *ms_total, long long *exchanges);
+
 /* ---- measurement utility (no counterpart in the reference) ----
  * The device's measured HBM ceiling: kind 0 streams a copy b -> a, kind 1 a triad a = b + s c, over
  * FP64 arrays of bytes_per_array each (allocated and freed inside), reps timed launches; returns the
