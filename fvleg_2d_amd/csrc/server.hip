@@ -108,6 +108,17 @@ struct cato_server {
   int grid = 0;
   int completed = 0;
   int integrator = -1;
+  // the resident set of the last table upload; while it is unchanged (no admission, no retirement) the
+  // tables stay on the device and the tick's steps replay one captured graph
+  bool dirty = true;
+  std::vector<int> ids;
+  int nitems = 0;
+  // resident solves large enough to fill the GPU alone take their own stage-kernel launch per stage (the
+  // single-solve kernel, with the registers and LDS of their variant) instead of the merged batch kernel;
+  // `tab` is the host copy of the stage arguments those launches pass
+  std::vector<int> big;
+  std::vector<StageArgs> tab;
+  hipGraphExec_t tick_exec = nullptr;
   std::chrono::steady_clock::time_point t0;
   double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };
@@ -141,9 +152,14 @@ int server_tables(cato_server *S) {
 }
 
 // Rebuild the work list and per-slot stage arguments of the resident set.
+// A resident solve of at least this many tiles runs its own stage kernel (one launch fills the GPU).
+constexpr int kBigTiles = 2048;
+
 int server_upload(cato_server *S, int &nitems, std::vector<int> &active_ids) {
   const int n = S->max_slots;
-  std::vector<StageArgs> tab((size_t)(S->nstages + 1) * n);
+  std::vector<StageArgs> &tab = S->tab;
+  tab.assign((size_t)(S->nstages + 1) * n, StageArgs{});
+  S->big.clear();
   std::vector<int> variant(n, 0);
   std::vector<BatchSolveRef> refs;
   std::vector<BatchItem> &items = S->items;
@@ -177,6 +193,10 @@ int server_upload(cato_server *S, int &nitems, std::vector<int> &active_ids) {
   for (int k : order) {
     const Solve &sv = *S->slots[k].s;
     int nt = sv.tiles_x * sv.tiles_y;
+    if (nt >= kBigTiles) {
+      S->big.push_back(k);
+      continue;
+    }
     for (int t = 0; t < nt; ++t) items.push_back({k, t});
   }
   nitems = (int)items.size();
@@ -201,7 +221,7 @@ int server_upload(cato_server *S, int &nitems, std::vector<int> &active_ids) {
   if ((rc = hip_check(hipMemcpyAsync(S->table_mem + S->count_off, &S->item_count, sizeof(int),
                                      hipMemcpyHostToDevice, st), "count")))
     return rc;
-  // pageable host vectors: wait before they go out of scope
+  // pageable host vectors: wait before they go out of scope (tab is kept for the big solves' launches)
   return hip_check(hipStreamSynchronize(st), "server upload");
 }
 
@@ -277,7 +297,8 @@ int admit_into(cato_server *S, Request &R, Solve &solve, int k, size_t off) {
   return hip_check(hipStreamSynchronize(st), "admit");  // L lives on this stack frame
 }
 
-int retire(cato_server *S) {
+int retire(cato_server *S, int &retired) {
+  retired = 0;
   std::vector<LoopState> H(S->max_slots);
   int rc = hip_check(hipMemcpyAsync(H.data(), S->table_mem + S->loops_off, S->max_slots * sizeof(LoopState),
                                     hipMemcpyDeviceToHost, S->stream),
@@ -302,8 +323,50 @@ int retire(cato_server *S) {
     sl.active = false;
     sl.s.reset();
     S->completed++;
+    retired++;
   }
   return CATO_OK;
+}
+
+void drop_tick_graph(cato_server *S) {
+  if (S->tick_exec) (void)hipGraphExecDestroy(S->tick_exec);
+  S->tick_exec = nullptr;
+}
+
+// steps_per_tick steps of the resident set: stage launches (the merged batch kernel over the small solves'
+// tiles, one stage-kernel launch per big solve) and the per-solve finalize of every step.  first_fresh:
+// stage 0 of the first step takes the table's first-stage slot (fresh solves read their primitives).
+int enqueue_tick(cato_server *S, bool first_fresh, int grid, int resident) {
+  const int n = S->max_slots;
+  const StageArgs *args = reinterpret_cast<const StageArgs *>(S->table_mem + S->args_off);
+  for (int step = 0; step < S->steps_per_tick; ++step) {
+    for (int s = 0; s < S->nstages; ++s) {
+      const int slot = (step == 0 && s == 0 && first_fresh) ? S->nstages : s;
+      if (S->nitems > 0) {
+        BatchStageArgs b;
+        b.args = args + (size_t)slot * n;
+        b.variant = reinterpret_cast<const int *>(S->table_mem + S->variant_off);
+        b.items = reinterpret_cast<const BatchItem *>(S->table_mem + S->items_off);
+        b.count = reinterpret_cast<const int *>(S->table_mem + S->count_off);
+        b.cursor = reinterpret_cast<int *>(S->table_mem + S->cursor_off) + (size_t)s * kNumXcd;
+        S->launcher(dim3((unsigned)grid), S->stream, b);
+      }
+      for (int k : S->big) {
+        const Solve &sv = *S->slots[k].s;
+        const StageArgs &a = S->tab[(size_t)slot * n + k];
+        sv.launcher(dim3((unsigned)(sv.tiles_x * sv.tiles_y)), S->stream, a);
+        // the merged kernel writes a +x outflow ghost column in its epilogue; the stage kernel leaves it to this
+        if (S->precision == CATO_PRECISION_FP64) {
+          int rc = fill_outflow_column(S->stream, sv, a.prim_out);
+          if (rc) return rc;
+        }
+      }
+    }
+    hipLaunchKernelGGL(batch_finalize_kernel, dim3((unsigned)resident), dim3(kThreads), 0, S->stream,
+                       reinterpret_cast<const BatchSolveRef *>(S->table_mem + S->refs_off), 2,
+                       reinterpret_cast<int *>(S->table_mem + S->cursor_off), S->nstages * kNumXcd);
+  }
+  return hip_check(hipGetLastError(), "server step");
 }
 
 }  // namespace
@@ -354,6 +417,7 @@ int cato_server_create(double hbm_budget_bytes, int max_slots, int precision, in
 int cato_server_destroy(cato_server *S) {
   if (!S) return CATO_OK;
   if (S->stream) (void)hipStreamSynchronize(S->stream);
+  drop_tick_graph(S);
   S->slots.clear();
   if (S->arena_mem) (void)hipFree(S->arena_mem);
   if (S->table_mem) (void)hipFree(S->table_mem);
@@ -408,8 +472,9 @@ int cato_server_submit(cato_server *S, const cato_request *req, double arrival_t
 
 int cato_server_tick(cato_server *S, int *n_resident, int *n_queued, int *n_completed_total) {
   if (!S) return fail(CATO_ERR_INVALID, "null server");
-  int rc;
-  if (S->table_mem && (rc = retire(S))) return rc;
+  int rc, retired = 0;
+  if (S->table_mem && (rc = retire(S, retired))) return rc;
+  if (retired) S->dirty = true;
   // admission in arrival order while slots and memory allow
   while (!S->queue.empty()) {
     int rid = S->queue.front();
@@ -424,42 +489,45 @@ int cato_server_tick(cato_server *S, int *n_resident, int *n_queued, int *n_comp
       continue;
     }
     S->queue.pop_front();
+    S->dirty = true;
   }
-  int nitems = 0, resident = 0;
-  std::vector<int> ids;
-  if (S->table_mem) {
-    if ((rc = server_upload(S, nitems, ids))) return rc;
-    resident = (int)ids.size();
-  }
-  if (resident > 0) {
-    const int n = S->max_slots;
+  if (S->table_mem && S->dirty) {
+    drop_tick_graph(S);
+    if ((rc = server_upload(S, S->nitems, S->ids))) return rc;
     // work cursors start at zero (the finalize kernel resets them after every step)
     if ((rc = hip_check(hipMemsetAsync(S->table_mem + S->cursor_off, 0, (size_t)S->nstages * kNumXcd * sizeof(int),
                                        S->stream), "cursors")))
       return rc;
-    const StageArgs *args = reinterpret_cast<const StageArgs *>(S->table_mem + S->args_off);
+    S->dirty = false;
+  }
+  const int resident = S->table_mem ? (int)S->ids.size() : 0;
+  if (resident > 0) {
     // slot `nstages` of the table holds stage 0 as the first step of this tick
     // runs it: fresh solves read their uploaded primitives, the others their state
     bool any_fresh = false;
-    for (int k : ids) any_fresh = any_fresh || S->slots[k].fresh;
-    const StageArgs *first0 = any_fresh ? args + (size_t)S->nstages * n : args;
-    int grid = std::max(kNumXcd, (std::min(nitems, S->grid) / kNumXcd) * kNumXcd);
-    for (int step = 0; step < S->steps_per_tick; ++step) {
-      for (int s = 0; s < S->nstages; ++s) {
-        BatchStageArgs b;
-        b.args = (step == 0 && s == 0) ? first0 : args + (size_t)s * n;
-        b.variant = reinterpret_cast<const int *>(S->table_mem + S->variant_off);
-        b.items = reinterpret_cast<const BatchItem *>(S->table_mem + S->items_off);
-        b.count = reinterpret_cast<const int *>(S->table_mem + S->count_off);
-        b.cursor = reinterpret_cast<int *>(S->table_mem + S->cursor_off) + (size_t)s * kNumXcd;
-        S->launcher(dim3((unsigned)grid), S->stream, b);
+    for (int k : S->ids) any_fresh = any_fresh || S->slots[k].fresh;
+    const int grid = std::max(kNumXcd, (std::min(S->nitems, S->grid) / kNumXcd) * kNumXcd);
+    if (any_fresh) {
+      if ((rc = enqueue_tick(S, true, grid, resident))) return rc;
+    } else {
+      // the same launches every tick until the resident set changes: one graph
+      if (!S->tick_exec) {
+        if ((rc = hip_check(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal), "capture tick")))
+          return rc;
+        rc = enqueue_tick(S, false, grid, resident);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(S->stream, &g);
+        if (!rc && e == hipSuccess) e = hipGraphInstantiate(&S->tick_exec, g, nullptr, nullptr, 0);
+        if (g) (void)hipGraphDestroy(g);
+        if (rc) return rc;
+        if ((rc = hip_check(e, "tick graph"))) {
+          S->tick_exec = nullptr;
+          return rc;
+        }
       }
-      hipLaunchKernelGGL(batch_finalize_kernel, dim3((unsigned)resident), dim3(kThreads), 0, S->stream,
-                         reinterpret_cast<const BatchSolveRef *>(S->table_mem + S->refs_off), 2,
-                         reinterpret_cast<int *>(S->table_mem + S->cursor_off), S->nstages * kNumXcd);
-      if ((rc = hip_check(hipGetLastError(), "server step"))) return rc;
+      if ((rc = hip_check(hipGraphLaunch(S->tick_exec, S->stream), "tick graph launch"))) return rc;
     }
-    for (int k : ids) {
+    for (int k : S->ids) {
       S->slots[k].fresh = false;
       S->slots[k].s->prims_valid = false;
     }
