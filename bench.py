@@ -61,6 +61,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--profile-only", action="store_true", help="run the timed loop only (for rocprofv3)")
+    ap.add_argument("--served-requests", type=int, default=48, help="requests of the serving workload")
     return ap.parse_args()
 
 
@@ -365,8 +366,8 @@ def main():
 
         say("[bench] batched: 256 Riemann solves vs one at a time")
         out["batched"] = measure_batched(n_solves=256)
-        say("[bench] served: Poisson arrivals vs a plain FP64 queue")
-        out["served"] = measure_served(n=16)
+        say(f"[bench] served: {args.served_requests} Poisson arrivals vs a plain FP64 queue")
+        out["served"] = measure_served(n=args.served_requests)
     if not args.no_cpu and os.environ.get("CATO_BENCH_NO_CPU") != "1":
         threads = host_threads()
         say(f"[bench] CPU baseline (C oracle, OpenMP over rows, {threads} threads)")

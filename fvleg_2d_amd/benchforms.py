@@ -14,8 +14,10 @@ from .serve import Server
 from .solver import Solver
 
 
-def plain_one_at_a_time(pbs):
-    """Each solve through the plain single-solve API, strictly one after another."""
+def plain_one_at_a_time(pbs, parts=None):
+    """Each solve through the plain single-solve API, strictly one after another.  parts (a dict), when
+    given, collects the summed seconds of each solve's set-up (context, geometry, upload), run (the device
+    loop: kernels, host polls, graph capture) and download (the result and the context's release)."""
     t0 = time.perf_counter()
     results = []
     lat = []
@@ -23,9 +25,16 @@ def plain_one_at_a_time(pbs):
         ts = time.perf_counter()
         with Solver(pb.cfg, pb.node_x, pb.node_y) as s:
             s.set_primitive(pb.rho, pb.u, pb.v, pb.p)
+            tr = time.perf_counter()
             n, t, e = s.run(pb.max_time, 0, pb.max_time)
+            td = time.perf_counter()
             results.append(s.conserved())
-        lat.append(time.perf_counter() - ts)
+        te = time.perf_counter()
+        if parts is not None:
+            parts["setup_s"] = parts.get("setup_s", 0.0) + (tr - ts)
+            parts["run_s"] = parts.get("run_s", 0.0) + (td - tr)
+            parts["download_s"] = parts.get("download_s", 0.0) + (te - td)
+        lat.append(te - ts)
     return time.perf_counter() - t0, results, lat
 
 
@@ -58,7 +67,8 @@ def measure_batched(n_solves=256, seed=2024, side_range=(64, 512), max_time=0.2,
     batched(warm, tile=tile)
     plain_one_at_a_time(warm[:1])
     b = batched(pbs, tile=tile)
-    plain_s, plain_res, _ = plain_one_at_a_time(pbs)
+    parts = {}
+    plain_s, plain_res, _ = plain_one_at_a_time(pbs, parts)
     identical = None
     if check:
         identical = all(np.array_equal(a, r) for br, pr in zip(b["results"], plain_res) for a, r in zip(br, pr))
@@ -77,6 +87,11 @@ def measure_batched(n_solves=256, seed=2024, side_range=(64, 512), max_time=0.2,
         "errors": int(np.count_nonzero(b["errors"])),
         "tile": tile,
         "autotune": tuned,
+        # where the plain path's time goes: per-solve set-up and download against the device loops; a
+        # batch removes the first and most of the last, and runs the loops together
+        "plain_breakdown_s": parts,
+        "plain_fixed_cost_share": 1.0 - parts["run_s"] / plain_s,
+        "speedup_ceiling_without_fixed_costs": plain_s / parts["run_s"],
     }
 
 
@@ -166,8 +181,8 @@ def rel_l1(ref, got):
     return max(float(np.mean(np.abs(g - r)) / max(np.max(np.abs(r)), 1e-300)) for r, g in zip(ref, got))
 
 
-def measure_served(n=16, seed=4242, side_range=(128, 2048), load=0.9, budget_bytes=64e9, max_slots=64,
-                   steps_per_tick=8):
+def measure_served(n=48, seed=4242, side_range=(128, 2048), load=0.9, budget_bytes=64e9, max_slots=64,
+                   steps_per_tick=16):
     """Serving workload: Poisson arrivals at rate = load / (mean plain FP64 service time).
 
     Reports solves/s and p50/p99 time-to-solution of the server (FP32-flux/FP64-state) beside the
@@ -185,7 +200,8 @@ def measure_served(n=16, seed=4242, side_range=(128, 2048), load=0.9, budget_byt
     plain_tts, plain_end = fifo_tts(arrivals, service)
     sv = served_run(reqs, arrivals, budget_bytes, max_slots=max_slots, steps_per_tick=steps_per_tick)
     sat = served_run(reqs, np.zeros(n), budget_bytes, max_slots=max_slots, steps_per_tick=steps_per_tick)
-    l1 = max(rel_l1(p, s) for p, s in zip(plain_res, sv["results"]))
+    l1s = [rel_l1(p, s) for p, s in zip(plain_res, sv["results"])]
+    l1 = max(l1s)
     pct = lambda a, q: float(np.percentile(a, q))
     return {
         "requests": n,
@@ -203,6 +219,9 @@ def measure_served(n=16, seed=4242, side_range=(128, 2048), load=0.9, budget_byt
         "saturated_speedup": float(service.sum()) / sat["makespan"],
         "p99_tts_ratio_plain_over_served": pct(plain_tts, 99) / pct(sv["tts"], 99),
         "max_rel_l1_vs_plain_fp64": l1,
+        "rel_l1_vs_plain_fp64_per_request": [float(f"{x:.3g}") for x in l1s],
+        "steps_per_tick": steps_per_tick,
+        "percentile_note": f"p50 / p99 by linear interpolation over {n} requests",
         "max_resident": sv["max_resident"],
         "errors": int(sum(1 for e in sv["errors"] if e != 0)),
     }
