@@ -206,14 +206,28 @@ struct FastOps {
   }
 };
 
-// Operands for which the FastOps edge computations (TVD2 reconstruction and the
-// SLAU family) stay in range, by cell: rho and p in [2^-60, 2^60], u and v zero
-// or 2^-150 <= |u| <= 2^60.  Then every reconstructed value is 0 or within
-// [2^-218, 2^62] (rho, p within [2^-61, 2^61]), differences are 0 or at least
-// epsilon after their flush, and every quotient of those formulas lies within
-// [2^-405, 2^248] with operand exponents at most 300 apart (gamma in (1, 4]).
-__device__ inline bool fast_ops_cell(const Prim<double> &q) {
-  const double LO = 0x1p-60, HI = 0x1p60, VLO = 0x1p-150;
+// Operands for which the FastOps edge computations stay in range, by cell: rho
+// and p in [2^-B, 2^B], u and v zero or 2^-150 <= |u| <= 2^B (gamma in (1, 4]).
+//
+// B = 60, TVD2 reconstruction with the SLAU family: every reconstructed value is
+// 0 or within [2^-218, 2^62] (rho, p within [2^-61, 2^61]), differences are 0 or
+// at least epsilon after their flush, and every quotient of those formulas lies
+// within [2^-405, 2^248] with operand exponents at most 300 apart.
+//
+// B = 20, every other scheme: the reconstruction (TVD2 within the bound above)
+// and, where it has a FastOps path, the flux (HLLC).  MLP: nonzero flushed
+// differences lie in [2^-52, 2^21], so the slope ratios are 0 or within
+// [2^-73, 2^121], r + 1e-30 stays at least 2^-100 from zero, and the quotients
+// of tan(theta) and alpha lie within [2^-390, 2^250]; the reconstructed states
+// leave [min, max] of their two cells by at most 5e-15 |dp| plus rounding, so
+// rho and p stay within [2^-21, 2^21].  HLLC on such states: c in [2^-21, 2^22],
+// |q| / c <= 2^42 < 2^52, so S_L - q_L and S_R - q_R keep at least half of c w
+// and m_L, m_R are nonzero within [2^-43, 2^88]; |S_L| >= 2^-75 where it
+// divides; S*, the star factors and the energy quotients lie within
+// [2^-400, 2^170].  No numerator of either is -0.
+template <int LOG2B> __device__ inline bool fast_ops_cell(const Prim<double> &q) {
+  static_assert(LOG2B >= 1 && LOG2B <= 60, "fast_ops_cell bound");
+  constexpr double HI = (double)(1ull << LOG2B), LO = 1.0 / HI, VLO = 0x1p-150;
   const double au = absd(q.u), av = absd(q.v);
   return q.rho >= LO && q.rho <= HI && q.p >= LO && q.p <= HI && au <= HI && av <= HI &&
          (au == 0.0 || au >= VLO) && (av == 0.0 || av >= VLO);
@@ -268,36 +282,39 @@ __device__ inline void recon_tvd2(R qm, R q0, R q1, R q2, R &L, R &Rv) {
 
 // muscl_mlp.fypp:196-279.  t_* are the transverse neighbours for tan(theta):
 // a0p = q(0,+1), a0m = q(0,-1), a1p = q(1,+1), a1m = q(1,-1) in (normal, transverse).
-template <typename R, int ORDER>
+template <typename R, int ORDER, class D = IeeeOps>
 __device__ inline void recon_mlp(R qm3, R qm, R q0, R q1, R q2, R q3, R a0p, R a0m, R a1p, R a1m, R &L, R &Rv) {
   const R SMALL = R(1e-30);
   R dm = q0 - qm, dp = q1 - q0, d3 = q2 - q1;
   if (absd(dm) < Lim<R>::eps) dm = R(0);
   if (absd(dp) < Lim<R>::eps) dp = R(0);
   if (absd(d3) < Lim<R>::eps) d3 = R(0);
-  R r_R_p1 = dp / (d3 + SMALL);
-  R r_L = dp / (dm + SMALL);
+  R r_R_p1 = D::div(dp, d3 + SMALL);
+  R r_L = D::div(dp, dm + SMALL);
   R beta_L, beta_R;
   if constexpr (ORDER == 5) {
     R dm3 = qm - qm3, d5 = q3 - q2;
     if (absd(dm3) < Lim<R>::eps) dm3 = R(0);
     if (absd(d5) < Lim<R>::eps) d5 = R(0);
-    R r_L_m1 = dm / (dm3 + SMALL);
-    R r_L_p1 = d3 / (dp + SMALL);
-    R r_R_p2 = d3 / (d5 + SMALL);
-    R r_R = dm / (dp + SMALL);
-    beta_L = ((-(R(2) / (r_L_m1 + SMALL)) + R(11)) + R(24) * r_L - R(3) * r_L * r_L_p1) / R(30);
-    beta_R = ((-(R(2) / (r_R_p2 + SMALL)) + R(11)) + R(24) * r_R_p1 - R(3) * r_R_p1 * r_R) / R(30);
+    const auto r_dp = D::rcp(dp + SMALL);
+    R r_L_m1 = D::div(dm, dm3 + SMALL);
+    R r_L_p1 = D::div(d3, r_dp);
+    R r_R_p2 = D::div(d3, d5 + SMALL);
+    R r_R = D::div(dm, r_dp);
+    const auto r30 = D::rcp(R(30));
+    beta_L = D::div(((-D::div(R(2), r_L_m1 + SMALL)) + R(11)) + R(24) * r_L - R(3) * r_L * r_L_p1, r30);
+    beta_R = D::div(((-D::div(R(2), r_R_p2 + SMALL)) + R(11)) + R(24) * r_R_p1 - R(3) * r_R_p1 * r_R, r30);
   } else {
     (void)qm3;
     (void)q3;
-    beta_L = (R(1) + R(2) * r_L) / R(3);
-    beta_R = (R(1) + R(2) * r_R_p1) / R(3);
+    const auto r3 = D::rcp(R(3));
+    beta_L = D::div(R(1) + R(2) * r_L, r3);
+    beta_R = D::div(R(1) + R(2) * r_R_p1, r3);
   }
-  R tt = absd(a0p - a0m) / (absd(q1 - qm) + SMALL);
-  R tt1 = absd(a1p - a1m) / (absd(q2 - q0) + SMALL);
-  R aLt = (R(2) * fmax2(R(1), r_L) * (R(1) + fmax2(R(0), (tt1 / (r_R_p1 + SMALL))))) / (R(1) + tt);
-  R aRt = (R(2) * fmax2(R(1), r_R_p1) * (R(1) + fmax2(R(0), (tt / (r_L + SMALL))))) / (R(1) + tt1);
+  R tt = D::div(absd(a0p - a0m), absd(q1 - qm) + SMALL);
+  R tt1 = D::div(absd(a1p - a1m), absd(q2 - q0) + SMALL);
+  R aLt = D::div(R(2) * fmax2(R(1), r_L) * (R(1) + fmax2(R(0), D::div(tt1, r_R_p1 + SMALL))), R(1) + tt);
+  R aRt = D::div(R(2) * fmax2(R(1), r_R_p1) * (R(1) + fmax2(R(0), D::div(tt, r_L + SMALL))), R(1) + tt1);
   R aL = fmax2(R(1), fmin2(R(2), aLt));
   R aR = fmax2(R(1), fmin2(R(2), aRt));
   L = q0 + R(0.5) * fmax2(R(0), fmin2(fmin2(aL * r_L, aL), beta_L)) * dm;
@@ -637,8 +654,9 @@ __device__ inline void flux_ausm_plus(R gamma, const EdgeState<R> &s, R out[4]) 
 }
 
 // HLLC (Toro 2009, sec. 10.4-10.6): PVRS pressure, pressure-based wave speeds,
-// contact speed eq. 10.37, star states eq. 10.39.
-template <typename R, int DIR>
+// contact speed eq. 10.37, star states eq. 10.39.  D = FastOps shares the
+// reciprocals of rho_L, rho_R and gamma - 1.
+template <typename R, int DIR, class D = IeeeOps>
 __device__ inline void flux_hllc(R gamma, const EdgeState<R> &s, R out[4]) {
   const R n_x = DIR == 0 ? R(1) : R(0), n_y = DIR == 0 ? R(0) : R(1);
   R rL = s.rL, uL = s.uL, vL = s.vL, pL = s.pL;
@@ -646,22 +664,24 @@ __device__ inline void flux_hllc(R gamma, const EdgeState<R> &s, R out[4]) {
   R gm1 = gamma - R(1);
   R qL = uL * n_x + vL * n_y;
   R qR = uR * n_x + vR * n_y;
-  R cL = sqrtr(gamma * pL / rL);
-  R cR = sqrtr(gamma * pR / rR);
+  const auto d_rL = D::rcp(rL), d_rR = D::rcp(rR);
+  R cL = D::sqrt_pos(D::div(gamma * pL, d_rL));
+  R cR = D::sqrt_pos(D::div(gamma * pR, d_rR));
   R rho_bar = R(0.5) * (rL + rR);
   R cs_bar = R(0.5) * (cL + cR);
   R p_pvrs = R(0.5) * (pL + pR) - R(0.5) * (qR - qL) * rho_bar * cs_bar;
   R p_star = fmax2(R(0), p_pvrs);
-  R gfac = (gamma + R(1)) / (R(2) * gamma);
-  R wL = (p_star <= pL) ? R(1) : sqrtr(R(1) + gfac * (p_star / pL - R(1)));
-  R wR = (p_star <= pR) ? R(1) : sqrtr(R(1) + gfac * (p_star / pR - R(1)));
+  R gfac = D::div(gamma + R(1), R(2) * gamma);
+  R wL = (p_star <= pL) ? R(1) : D::sqrt_pos(R(1) + gfac * (D::div(p_star, pL) - R(1)));
+  R wR = (p_star <= pR) ? R(1) : D::sqrt_pos(R(1) + gfac * (D::div(p_star, pR) - R(1)));
   R SL = qL - cL * wL;
   R SR = qR + cR * wR;
   R mL = rL * (SL - qL);
   R mR = rR * (SR - qR);
-  R S_star = (pR - pL + qL * mL - qR * mR) / (mL - mR);
-  R EL = pL / gm1 + R(0.5) * rL * (uL * uL + vL * vL);
-  R ER = pR / gm1 + R(0.5) * rR * (uR * uR + vR * vR);
+  R S_star = D::div(pR - pL + qL * mL - qR * mR, mL - mR);
+  const auto d_gm1 = D::rcp(gm1);
+  R EL = D::div(pL, d_gm1) + R(0.5) * rL * (uL * uL + vL * vL);
+  R ER = D::div(pR, d_gm1) + R(0.5) * rR * (uR * uR + vR * vR);
   if (R(0) <= SL) {
     out[0] = rL * qL;
     out[1] = rL * uL * qL + pL * n_x;
@@ -673,19 +693,19 @@ __device__ inline void flux_hllc(R gamma, const EdgeState<R> &s, R out[4]) {
     out[2] = rR * vR * qR + pR * n_y;
     out[3] = qR * (ER + pR);
   } else if (S_star >= R(0)) {
-    R fac = mL / (SL - S_star);
+    R fac = D::div(mL, SL - S_star);
     R Us1 = fac * (uL + (S_star - qL) * n_x);
     R Us2 = fac * (vL + (S_star - qL) * n_y);
-    R Us3 = fac * (EL / rL + (S_star - qL) * (S_star + pL / mL));
+    R Us3 = fac * (D::div(EL, d_rL) + (S_star - qL) * (S_star + D::div(pL, mL)));
     out[0] = rL * qL + SL * (fac - rL);
     out[1] = (rL * uL * qL + pL * n_x) + SL * (Us1 - rL * uL);
     out[2] = (rL * vL * qL + pL * n_y) + SL * (Us2 - rL * vL);
     out[3] = qL * (EL + pL) + SL * (Us3 - EL);
   } else {
-    R fac = mR / (SR - S_star);
+    R fac = D::div(mR, SR - S_star);
     R Us1 = fac * (uR + (S_star - qR) * n_x);
     R Us2 = fac * (vR + (S_star - qR) * n_y);
-    R Us3 = fac * (ER / rR + (S_star - qR) * (S_star + pR / mR));
+    R Us3 = fac * (D::div(ER, d_rR) + (S_star - qR) * (S_star + D::div(pR, mR)));
     out[0] = rR * qR + SR * (fac - rR);
     out[1] = (rR * uR * qR + pR * n_x) + SR * (Us1 - rR * uR);
     out[2] = (rR * vR * qR + pR * n_y) + SR * (Us2 - rR * vR);
@@ -810,13 +830,18 @@ __device__ inline void edge_flux(R gamma, int low_mach_fix, const EdgeState<R> &
     flux_slau<R, FLUX, DIR, D>(gamma, s, pn, out);
   else if constexpr (FLUX == FLUX_ROE) flux_roe<R, DIR>(gamma, s, pn, out);
   else if constexpr (FLUX == FLUX_AUSM_PLUS) flux_ausm_plus<R, DIR>(gamma, s, out);
-  else flux_hllc<R, DIR>(gamma, s, out);
+  else flux_hllc<R, DIR, D>(gamma, s, out);
 }
 
-// Schemes whose FP64 edge computations have a FastOps path: TVD2 reconstruction with the SLAU family.
-template <int FLUX, int LIMC> constexpr bool fast_ops_scheme() {
-  return (LIMC == LIM_MINMOD || LIMC == LIM_SUPERBEE || LIMC == LIM_VAN_LEER) &&
-         (FLUX == FLUX_SLAU || FLUX == FLUX_SLAU2 || FLUX == FLUX_SD_SLAU || FLUX == FLUX_SD_SLAU2);
+// FP64 fluxes with a FastOps path (the SLAU family and HLLC; every limiter has
+// one), and the bound B of fast_ops_cell that keeps a scheme's operations in
+// range: 60 for TVD2 with the SLAU family, 20 otherwise.
+template <int FLUX> constexpr bool fast_ops_flux() {
+  return FLUX == FLUX_SLAU || FLUX == FLUX_SLAU2 || FLUX == FLUX_SD_SLAU || FLUX == FLUX_SD_SLAU2 ||
+         FLUX == FLUX_HLLC;
+}
+template <int FLUX, int LIMC> constexpr int fast_ops_bound() {
+  return (FLUX != FLUX_HLLC && fast_ops_flux<FLUX>() && LIMC != LIM_MLP3 && LIMC != LIM_MLP5) ? 60 : 20;
 }
 
 template <int FLUX> constexpr bool needs_pressure_nbrs() {

@@ -223,7 +223,7 @@ __device__ inline void recon_var(const R *base, int sn, int st, R &L, R &Rv) {
   if constexpr (LIMC == LIM_MLP3 || LIMC == LIM_MLP5) {
     R qm3 = (LIMC == LIM_MLP5) ? base[-2 * sn] : R(0);
     R q3 = (LIMC == LIM_MLP5) ? base[3 * sn] : R(0);
-    recon_mlp<R, (LIMC == LIM_MLP5 ? 5 : 3)>(qm3, base[-sn], base[0], base[sn], base[2 * sn], q3, base[st], base[-st],
+    recon_mlp<R, (LIMC == LIM_MLP5 ? 5 : 3), D>(qm3, base[-sn], base[0], base[sn], base[2 * sn], q3, base[st], base[-st],
                                                base[sn + st], base[sn - st], L, Rv);
   } else {
     recon_tvd2<R, LIMC, D>(base[-sn], base[0], base[sn], base[2 * sn], L, Rv);
@@ -249,9 +249,10 @@ struct RedSmem {
 };
 
 // Phase 2 of a stage tile: every i- and j-edge of the tile once, reconstructed
-// from the LDS primitives and its flux written to LDS.  D: the division and
-// square root of the FP64 arithmetic (IeeeOps, or FastOps on in-range tiles).
-template <int FLUX, int LIMC, typename R, int TX, int TY, class D>
+// from the LDS primitives and its flux written to LDS.  D / DF: the division
+// and square root of the FP64 reconstruction / flux (IeeeOps, or FastOps on
+// in-range tiles).
+template <int FLUX, int LIMC, typename R, int TX, int TY, class D, class DF>
 __device__ inline void edge_pass(const StageArgs &a, TileSmem<LIMC, R, TX, TY> &sm, int i0, int j0) {
   constexpr int H = halo_of<LIMC>();
   constexpr int RX = TX + 2 * H;
@@ -285,10 +286,10 @@ __device__ inline void edge_pass(const StageArgs &a, TileSmem<LIMC, R, TX, TY> &
     recon_var<R, LIMC, D>(&sp[3][ly][lx], sn, st, s.pL, s.pR);
     EdgeState<R> sb;  // superbee reconstruction of the same edge (M-AUSMPW+ only)
     if constexpr (needs_superbee_states<FLUX>()) {
-      recon_var<R, LIM_SUPERBEE>(&sp[0][ly][lx], sn, st, sb.rL, sb.rR);
-      recon_var<R, LIM_SUPERBEE>(&sp[1][ly][lx], sn, st, sb.uL, sb.uR);
-      recon_var<R, LIM_SUPERBEE>(&sp[2][ly][lx], sn, st, sb.vL, sb.vR);
-      recon_var<R, LIM_SUPERBEE>(&sp[3][ly][lx], sn, st, sb.pL, sb.pR);
+      recon_var<R, LIM_SUPERBEE, D>(&sp[0][ly][lx], sn, st, sb.rL, sb.rR);
+      recon_var<R, LIM_SUPERBEE, D>(&sp[1][ly][lx], sn, st, sb.uL, sb.uR);
+      recon_var<R, LIM_SUPERBEE, D>(&sp[2][ly][lx], sn, st, sb.vL, sb.vR);
+      recon_var<R, LIM_SUPERBEE, D>(&sp[3][ly][lx], sn, st, sb.pL, sb.pR);
     }
     PressNbr<R> pn;
     if constexpr (needs_pressure_nbrs<FLUX>()) {
@@ -304,11 +305,11 @@ __device__ inline void edge_pass(const StageArgs &a, TileSmem<LIMC, R, TX, TY> &
     }
     R f[4];
     if (idir) {
-      edge_flux<R, FLUX, 0, D>(gamma, a.low_mach_fix, s, pn, f, &sb);
+      edge_flux<R, FLUX, 0, DF>(gamma, a.low_mach_fix, s, pn, f, &sb);
 #pragma unroll
       for (int v = 0; v < 4; ++v) sfi[v][ey][ex] = f[v];
     } else {
-      edge_flux<R, FLUX, 1, D>(gamma, a.low_mach_fix, s, pn, f, &sb);
+      edge_flux<R, FLUX, 1, DF>(gamma, a.low_mach_fix, s, pn, f, &sb);
 #pragma unroll
       for (int v = 0; v < 4; ++v) sfj[v][ey][ex] = f[v];
     }
@@ -343,9 +344,12 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
   // (precision form outside MODE_RESIDUAL: recovered from U_k, and checked
   // for NaN here, since no earlier stage stored and checked them)
   const bool recover = !kExact && a.mode != MODE_RESIDUAL;
-  // FP64 schemes with a FastOps path: whether every cell the edges read keeps
-  // the FastOps operations in range (fast_ops_cell)
-  constexpr bool kFastScheme = kExact && fast_ops_scheme<FLUX, LIMC>();
+  // FP64: whether every cell the edges read keeps the FastOps operations of the
+  // reconstruction and of the flux (where it has a FastOps path) in range
+  // (fast_ops_cell).  TVD2 under a flux without one divides too little for the
+  // range vote to pay.
+  constexpr bool kFastScheme =
+      kExact && (fast_ops_flux<FLUX>() || LIMC == LIM_MLP3 || LIMC == LIM_MLP5);
   bool in_range = true;
   for (int idx = tid; idx < RX * RY; idx += kThreads) {
     int ly = idx / RX, lx = idx - ly * RX;
@@ -356,7 +360,7 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
       if ((recover || a.check_input) && gi >= 1 && gi <= ni && gj >= 1 && gj <= nj &&
           (pr.rho != pr.rho || pr.u != pr.u || pr.v != pr.v || pr.p != pr.p))
         err |= 1;
-      if constexpr (kFastScheme) in_range = in_range && fast_ops_cell(pr);
+      if constexpr (kFastScheme) in_range = in_range && fast_ops_cell<fast_ops_bound<FLUX, LIMC>()>(pr);
     }
     sp[0][ly][lx] = (R)pr.rho;
     sp[1][ly][lx] = (R)pr.u;
@@ -374,12 +378,13 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
     bool fast = a.fast_ops != 0;
 #pragma unroll
     for (int w = 0; w < kThreads / 64; ++w) fast = fast && red.fast[w] != 0;
+    using DF = typename std::conditional<fast_ops_flux<FLUX>(), FastOps, IeeeOps>::type;
     if (fast)
-      edge_pass<FLUX, LIMC, R, TX, TY, FastOps>(a, sm, i0, j0);
+      edge_pass<FLUX, LIMC, R, TX, TY, FastOps, DF>(a, sm, i0, j0);
     else
-      edge_pass<FLUX, LIMC, R, TX, TY, IeeeOps>(a, sm, i0, j0);
+      edge_pass<FLUX, LIMC, R, TX, TY, IeeeOps, IeeeOps>(a, sm, i0, j0);
   } else {
-    edge_pass<FLUX, LIMC, R, TX, TY, IeeeOps>(a, sm, i0, j0);
+    edge_pass<FLUX, LIMC, R, TX, TY, IeeeOps, IeeeOps>(a, sm, i0, j0);
   }
   __syncthreads();
 

@@ -194,28 +194,37 @@ def test_step_parity_ssp_rk3(oracle, flux, lim):
 
 def fast_ops_edge_states(ni, nj, seed):
     """States around the operand ranges of the FP64 FastOps edge arithmetic (fast_ops_cell: rho and p in
-    [2^-60, 2^60], u and v zero or 2^-150 <= |u| <= 2^60).  'scaled': density and pressure of every cell
-    times 1e20, so no tile qualifies; 'mixed': a patch of velocities 1e-300 (whose quotients need the
+    [2^-B, 2^B], u and v zero or 2^-150 <= |u| <= 2^B; B = 60 for TVD2 with the SLAU family, 20 with MLP or
+    HLLC).  'scaled': density and pressure of every cell times 1e20, so no tile qualifies; 'scaled_1e7':
+    times 1e7, beyond 2^20 but within 2^60; 'mixed': a patch of velocities 1e-300 (whose quotients need the
     IEEE range scaling) in some tiles and negative-zero velocities (allowed) in others, next to tiles that
-    qualify."""
+    qualify; 'wide': density over [2^-19, 2^19] at pressure 1 (sound speeds over 2^19), every tile within
+    the B = 20 bound."""
     rho, u, v, p = shock_state(ni, nj, seed)
     scaled = [rho * 1e20, u.copy(order="F"), v.copy(order="F"), p * 1e20]
+    scaled7 = [rho * 1e7, u.copy(order="F"), v.copy(order="F"), p * 1e7]
     u2, v2 = u.copy(order="F"), v.copy(order="F")
     u2[4:12, 3:9] = 1e-300
     v2[4:12, 3:9] = -1e-300
     u2[40:56, 20:30] = -0.0
     v2[40:56, 20:30] = -0.0
     mixed = [rho.copy(order="F"), u2, v2, p.copy(order="F")]
-    return {"scaled": scaled, "mixed": mixed}
+    X, Y = np.meshgrid((np.arange(ni) + 0.5) / ni, (np.arange(nj) + 0.5) / nj, indexing="ij")
+    rw = np.exp2(19.0 * np.sin(2 * np.pi * X) * np.cos(2 * np.pi * Y))
+    wide = [np.asfortranarray(rw), np.asfortranarray(0.05 * u), np.asfortranarray(0.05 * v),
+            np.asfortranarray(np.ones_like(rw))]
+    return {"scaled": scaled, "scaled_1e7": scaled7, "mixed": mixed, "wide": wide}
 
 
-@pytest.mark.parametrize("flux", ["SLAU", "SLAU2", "SD-SLAU", "SD-SLAU2"])
-@pytest.mark.parametrize("lim", ["minmod", "van_leer"])
-@pytest.mark.parametrize("case", ["scaled", "mixed"])
+@pytest.mark.parametrize("flux", ["SLAU", "SLAU2", "SD-SLAU", "SD-SLAU2", "HLLC", "AUSM+", "AUSMPW+", "M-AUSMPW+",
+                                  "Roe"])
+@pytest.mark.parametrize("lim", ["minmod", "van_leer", "MLP3", "MLP5"])
+@pytest.mark.parametrize("case", ["scaled", "scaled_1e7", "mixed", "wide"])
 def test_fast_ops_tiles_and_their_fallback(oracle, flux, lim, case):
-    """The stage kernel takes its FastOps edge arithmetic only on tiles whose cells keep it in range and
-    the IEEE sequences elsewhere: both bit-identical to the oracle, with symmetry ghosts (mirrored
-    velocities, negative zeros) at two sides."""
+    """The stage kernel takes its FastOps edge arithmetic (the reconstruction of every scheme, the flux of
+    the SLAU family and HLLC) only on tiles whose cells keep it in range and the IEEE sequences elsewhere:
+    both bit-identical to the oracle, with symmetry ghosts (mirrored velocities, negative zeros) at two
+    sides."""
     say("fast ops", flux, lim, case)
     ni, nj = 64, 48
     cfg = SchemeConfig(flux=flux, limiter=lim, integrator="ssp_rk3",
