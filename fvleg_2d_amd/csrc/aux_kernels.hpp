@@ -16,6 +16,9 @@ struct FieldGeom {
   double gamma, cfl;
   double bc_pressure;
   const double *dxi, *dyj;
+  const double *quad;  // per-cell quad_cell geometry (QuadSlot arrays), null on rectilinear grids
+  int quad_ncx;
+  long long quad_n;
 };
 
 // Strided 2D copy on the device: dst(i, j) = src(i, j) for i < ni, j < nj with
@@ -105,7 +108,15 @@ static __global__ __launch_bounds__(kThreads) void k_cfl_partial(FieldGeom g, in
     if (rho != rho || u != u || v != v || p != p) err |= 1;
     if (rho < 0.0) err |= 2;
     if (p < 0.0) err |= 4;
-    double dx = absd(g.dxi[i + g.nh]), dy = absd(g.dyj[j + g.nh]);
+    double dx, dy;
+    if (g.quad) {  // min_dx / min_dy of the cell (non-negative)
+      const size_t gc = (size_t)(i + g.nh) + (size_t)(j + g.nh) * (size_t)g.quad_ncx;
+      dx = g.quad[Q_MDX * (size_t)g.quad_n + gc];
+      dy = g.quad[Q_MDY * (size_t)g.quad_n + gc];
+    } else {
+      dx = absd(g.dxi[i + g.nh]);
+      dy = absd(g.dyj[j + g.nh]);
+    }
     double val = g.cfl / (((absd(u) + cs) / dx) + ((absd(v) + cs) / dy));
     m = (val < m) ? val : m;
   }
@@ -598,15 +609,22 @@ static __global__ void k_edge_flux(FieldGeom g, int ld, const double *__restrict
   pn.p00 = P(0, 0); pn.p10 = P(1, 0); pn.p20 = P(2, 0); pn.pm10 = P(-1, 0);
   pn.p0p = P(0, 1); pn.p0m = P(0, -1); pn.p1p = P(1, 1); pn.p1m = P(1, -1);
   R f[4];
-  if (!dir) {
+  if (g.quad) {  // normal of the right (i-edge) or top (j-edge) edge of cell (i, j)
+    const size_t gc = (size_t)(i + g.nh - 1) + (size_t)(j + g.nh - 1) * (size_t)g.quad_ncx;
+    const size_t n = (size_t)g.quad_n;
+    s.nx = (R)g.quad[(dir ? Q_NTX : Q_NRX) * n + gc];
+    s.ny = (R)g.quad[(dir ? Q_NTY : Q_NRY) * n + gc];
+    if (!dir)
+      edge_flux<R, FLUX, 2>((R)g.gamma, low_mach_fix, s, pn, f, &sb);
+    else
+      edge_flux<R, FLUX, 3>((R)g.gamma, low_mach_fix, s, pn, f, &sb);
+  } else if (!dir) {
     edge_flux<R, FLUX, 0>((R)g.gamma, low_mach_fix, s, pn, f, &sb);
-    double *d = iflux + 4 * ((size_t)i + (size_t)(g.ni + 1) * (j - 1));
-    for (int v = 0; v < 4; ++v) d[v] = (double)f[v];
   } else {
     edge_flux<R, FLUX, 1>((R)g.gamma, low_mach_fix, s, pn, f, &sb);
-    double *d = jflux + 4 * ((size_t)(i - 1) + (size_t)g.ni * j);
-    for (int v = 0; v < 4; ++v) d[v] = (double)f[v];
   }
+  double *d = dir ? jflux + 4 * ((size_t)(i - 1) + (size_t)g.ni * j) : iflux + 4 * ((size_t)i + (size_t)(g.ni + 1) * (j - 1));
+  for (int v = 0; v < 4; ++v) d[v] = (double)f[v];
 }
 
 }  // namespace cato

@@ -125,6 +125,19 @@ struct StageArgs {
   // condition; layout in halo_store_index
   int halo_mask;
   const double *halo;
+  // General quadrilateral grid (null: rectilinear, dxi..lyj): G_COUNT per-cell arrays of quad_cell
+  // geometry over the halo'd cells, quad_ncx = ni + 2 nh per row, quad_n = quad_ncx (nj + 2 nh) per array
+  const double *quad;
+  int quad_ncx;
+  long long quad_n;
+};
+
+// Per-cell geometry of a general quadrilateral grid (quad_cell.f90:69-229), one
+// array each over the halo'd cells, column-major: the edge lengths of the
+// bottom, right, top and left edges, the volume, min_dx / min_dy, and the unit
+// normals of the right edge (the i-flux) and the top edge (the j-flux).
+enum QuadSlot {
+  Q_LEN0 = 0, Q_LEN1, Q_LEN2, Q_LEN3, Q_VOL, Q_MDX, Q_MDY, Q_NRX, Q_NRY, Q_NTX, Q_NTY, Q_COUNT
 };
 
 // Halo store of a sub-domain: the +x and -x strips [var][layer][row] (4 nh nj each), the +y and -y strips

@@ -326,7 +326,24 @@ __device__ inline void recon_mlp(R qm3, R qm, R q0, R q1, R q2, R q3, R a0p, R a
 // DIR 1: j-edge (normal (0,1), TOP edge).  Rectilinear grids give exactly these
 // normals from quad_cell.f90:193-229.
 // ---------------------------------------------------------------------------
-template <typename R> struct EdgeState { R rL, uL, vL, pL, rR, uR, vR, pR; };
+template <typename R> struct EdgeState {
+  R rL, uL, vL, pL, rR, uR, vR, pR;
+  R nx, ny;  // unit normal of the edge (direction codes 2 / 3 only)
+};
+
+// Edge direction codes of the flux templates: 0 / 1, the i- / j-edge of a
+// rectilinear grid (normal (1,0) / (0,1), folded at compile time); 2 / 3, the
+// same edges of a general quadrilateral grid, normal (s.nx, s.ny).  The
+// direction-dependent choices of the reference (AUSMPW+'s normal velocity and
+// get_f cells, the sensor stencils) follow the index direction DIR & 1.
+template <typename R, int DIR> __device__ inline R edge_nx(const EdgeState<R> &s) {
+  if constexpr (DIR >= 2) return s.nx;
+  else return DIR == 0 ? R(1) : R(0);
+}
+template <typename R, int DIR> __device__ inline R edge_ny(const EdgeState<R> &s) {
+  if constexpr (DIR >= 2) return s.ny;
+  else return DIR == 0 ? R(0) : R(1);
+}
 
 // Neighbour cell pressures needed by the shock sensors, in (normal, transverse)
 // coordinates relative to the edge's left cell 0: p(a,b).
@@ -340,7 +357,7 @@ template <typename R> struct PressNbr {
 // enter only Beta (even in a zero M) and g (whose sign of zero vanishes in 1 - g).
 template <typename R, int VARIANT, int DIR, class D = IeeeOps>
 __device__ inline void flux_slau(R gamma, const EdgeState<R> &s, const PressNbr<R> &pn, R out[4]) {
-  const R n_x = DIR == 0 ? R(1) : R(0), n_y = DIR == 0 ? R(0) : R(1);
+  const R n_x = edge_nx<R, DIR>(s), n_y = edge_ny<R, DIR>(s);
   const R THRESHOLD = R(5e-3), C_SD1 = R(0.01);
   const R eps = Lim<R>::eps;
   R rho_L = s.rL, u_L = s.uL, v_L = s.vL, p_L = s.pL;
@@ -397,7 +414,7 @@ __device__ inline void flux_slau(R gamma, const EdgeState<R> &s, const PressNbr<
   if constexpr (SD) {
     // slau_solver.fypp:410-448, cell pressures in the edge's (normal, transverse) frame
     R d[6];
-    if constexpr (DIR == 0) {
+    if constexpr ((DIR & 1) == 0) {
       d[0] = absd(pn.p0p - pn.p00);   // p(i,j+1)   - p(i,j)
       d[1] = absd(pn.pm10 - pn.p00);  // p(i-1,j)   - p(i,j)
       d[2] = absd(pn.p0m - pn.p00);   // p(i,j-1)   - p(i,j)
@@ -472,12 +489,12 @@ template <typename R> __device__ inline R pw_M_minus(R M) {
 template <typename R, int DIR>
 __device__ inline void flux_ausmpw(R gamma, int low_mach_fix, const EdgeState<R> &s, const PressNbr<R> &pn,
                                    R out[4]) {
-  const R n_x = DIR == 0 ? R(1) : R(0), n_y = DIR == 0 ? R(0) : R(1);
+  const R n_x = edge_nx<R, DIR>(s), n_y = edge_ny<R, DIR>(s);
   const R LOW_MACH = R(0.1);  // globals.f90:32
   R rho_L = s.rL, u_LHS = s.uL, v_LHS = s.vL, p_L = s.pL;
   R rho_R = s.rR, u_RHS = s.uR, v_RHS = s.vR, p_R = s.pR;
-  R U_L = DIR == 0 ? u_LHS : v_LHS, U_R = DIR == 0 ? u_RHS : v_RHS;
-  R V_L = DIR == 0 ? v_LHS : u_LHS, V_R = DIR == 0 ? v_RHS : u_RHS;
+  R U_L = (DIR & 1) == 0 ? u_LHS : v_LHS, U_R = (DIR & 1) == 0 ? u_RHS : v_RHS;
+  R V_L = (DIR & 1) == 0 ? v_LHS : u_LHS, V_R = (DIR & 1) == 0 ? v_RHS : u_RHS;
   R cs_L = sqrtr(gamma * p_L / rho_L);
   R cs_R = sqrtr(gamma * p_R / rho_R);
   R M_LHS = sqrtr(sq(u_LHS) + sq(v_LHS)) / cs_L;
@@ -507,8 +524,8 @@ __device__ inline void flux_ausmpw(R gamma, int low_mach_fix, const EdgeState<R>
   R w = R(1) - mt * mt * mt;
   // get_f neighbours (ausmpw_plus_solver.fypp:433-441): i-dir p1=(0,-1),(1,-1) p2=(0,+1),(1,+1);
   // j-dir p1=(i+1,j),(i+1,j+1) p2=(i-1,j),(i-1,j+1), i.e. transverse +1 then -1.
-  R p1L = DIR == 0 ? pn.p0m : pn.p0p, p1R = DIR == 0 ? pn.p1m : pn.p1p;
-  R p2L = DIR == 0 ? pn.p0p : pn.p0m, p2R = DIR == 0 ? pn.p1p : pn.p1m;
+  R p1L = (DIR & 1) == 0 ? pn.p0m : pn.p0p, p1R = (DIR & 1) == 0 ? pn.p1m : pn.p1p;
+  R p2L = (DIR & 1) == 0 ? pn.p0p : pn.p0m, p2R = (DIR & 1) == 0 ? pn.p1p : pn.p1m;
   R f_L, f_R;
   if (absd(p_s) < Lim<R>::tiny) {
     f_L = R(0);
@@ -544,7 +561,7 @@ template <typename R> __device__ inline R roe_f_h(R pL, R pR) {
 
 template <typename R, int DIR>
 __device__ inline void flux_roe(R gamma, const EdgeState<R> &s, const PressNbr<R> &pn, R out[4]) {
-  const R nx = DIR == 0 ? R(1) : R(0), ny = DIR == 0 ? R(0) : R(1);
+  const R nx = edge_nx<R, DIR>(s), ny = edge_ny<R, DIR>(s);
   R rl = absd(s.rL), ul = s.uL, vl = s.vL, pl = absd(s.pL);
   R gg = gamma / (gamma - R(1));
   R hl = gg * (pl / rl) + R(0.5) * (sq(ul) + sq(vl));
@@ -587,7 +604,7 @@ __device__ inline void flux_roe(R gamma, const EdgeState<R> &s, const PressNbr<R
   out[3] = R(0.5) * (FR3 + FL3 - rm3);
   // f_p (roe_solver.fypp:496-507); f_h is symmetric in its two pressures.
   R fp;
-  if constexpr (DIR == 0) {
+  if constexpr ((DIR & 1) == 0) {
     fp = roe_f_h(pn.p00, pn.p10);            // (i,j)   (i+1,j)
     fp = fmax2(fp, roe_f_h(pn.p00, pn.p0p)); // (i,j)   (i,j+1)
     fp = fmax2(fp, roe_f_h(pn.p00, pn.p0m)); // (i,j)   (i,j-1)
@@ -608,7 +625,7 @@ __device__ inline void flux_roe(R gamma, const EdgeState<R> &s, const PressNbr<R
 // (M_p = p_u = 0), beta = 1/8, alpha = 3/16, upwinded as in the commented solve (312-325).
 template <typename R, int DIR>
 __device__ inline void flux_ausm_plus(R gamma, const EdgeState<R> &s, R out[4]) {
-  const R n_x = DIR == 0 ? R(1) : R(0), n_y = DIR == 0 ? R(0) : R(1);
+  const R n_x = edge_nx<R, DIR>(s), n_y = edge_ny<R, DIR>(s);
   const R beta = R(1) / R(8), alpha = R(3) / R(16);
   R rho_L = s.rL, u_L = s.uL, v_L = s.vL, p_L = s.pL;
   R rho_R = s.rR, u_R = s.uR, v_R = s.vR, p_R = s.pR;
@@ -658,7 +675,7 @@ __device__ inline void flux_ausm_plus(R gamma, const EdgeState<R> &s, R out[4]) 
 // reciprocals of rho_L, rho_R and gamma - 1.
 template <typename R, int DIR, class D = IeeeOps>
 __device__ inline void flux_hllc(R gamma, const EdgeState<R> &s, R out[4]) {
-  const R n_x = DIR == 0 ? R(1) : R(0), n_y = DIR == 0 ? R(0) : R(1);
+  const R n_x = edge_nx<R, DIR>(s), n_y = edge_ny<R, DIR>(s);
   R rL = s.rL, uL = s.uL, vL = s.vL, pL = s.pL;
   R rR = s.rR, uR = s.uR, vR = s.vR, pR = s.pR;
   R gm1 = gamma - R(1);
@@ -731,13 +748,13 @@ template <typename R> __device__ inline R m_phi_R_half(R phi_L, R phi_R, R phi_R
 template <typename R, int DIR>
 __device__ inline void flux_m_ausmpw(R gamma, int low_mach_fix, const EdgeState<R> &s, const EdgeState<R> &sb,
                                      const PressNbr<R> &pn, R out[4]) {
-  const R n_x = DIR == 0 ? R(1) : R(0), n_y = DIR == 0 ? R(0) : R(1);
+  const R n_x = edge_nx<R, DIR>(s), n_y = edge_ny<R, DIR>(s);
   const R LOW_MACH = R(0.1);
   const R FEPS = R(Lim<double>::eps * 1e3);
   R rho_L = s.rL, u_LHS = s.uL, v_LHS = s.vL, p_L = s.pL;
   R rho_R = s.rR, u_RHS = s.uR, v_RHS = s.vR, p_R = s.pR;
-  R U_L = DIR == 0 ? u_LHS : v_LHS, U_R = DIR == 0 ? u_RHS : v_RHS;
-  R V_L = DIR == 0 ? v_LHS : u_LHS, V_R = DIR == 0 ? v_RHS : u_RHS;
+  R U_L = (DIR & 1) == 0 ? u_LHS : v_LHS, U_R = (DIR & 1) == 0 ? u_RHS : v_RHS;
+  R V_L = (DIR & 1) == 0 ? v_LHS : u_LHS, V_R = (DIR & 1) == 0 ? v_RHS : u_RHS;
   R cs_L = sqrtr(gamma * p_L / rho_L);
   R cs_R = sqrtr(gamma * p_R / rho_R);
   R M_LHS = sqrtr(sq(u_LHS) + sq(v_LHS)) / cs_L;
@@ -765,8 +782,8 @@ __device__ inline void flux_m_ausmpw(R gamma, int low_mach_fix, const EdgeState<
   R mt = fmin2((p_L / p_R), (p_R / p_L));
   R w = R(1) - (mt * mt * mt);
   // get_f neighbours as AUSMPW+: i-dir p1 = (0,-1),(1,-1), p2 = (0,+1),(1,+1); j-dir the transverse +1 then -1
-  R p1L = DIR == 0 ? pn.p0m : pn.p0p, p1R = DIR == 0 ? pn.p1m : pn.p1p;
-  R p2L = DIR == 0 ? pn.p0p : pn.p0m, p2R = DIR == 0 ? pn.p1p : pn.p1m;
+  R p1L = (DIR & 1) == 0 ? pn.p0m : pn.p0p, p1R = (DIR & 1) == 0 ? pn.p1m : pn.p1p;
+  R p2L = (DIR & 1) == 0 ? pn.p0p : pn.p0m, p2R = (DIR & 1) == 0 ? pn.p1p : pn.p1m;
   R f_L, f_R;
   if (absd(p_s) < Lim<R>::tiny) {
     f_L = R(0);

@@ -73,22 +73,80 @@ static unsigned char *noalloc_scratch(int device) {
 }
 
 // ---------------------------------------------------------------------------
-// Geometry: quad_cell.f90:69-229 per cell; the HIP path accepts rectilinear
-// blocks (node_x depends on i only, node_y on j only), for which the per-cell
-// geometry is exactly reproduced from four 1D arrays.
+// Geometry: quad_cell.f90:69-229 per cell.  A rectilinear block (node_x
+// depends on i only, node_y on j only, axis-aligned normals) keeps four 1D
+// arrays from which the fused kernel reproduces every cell's geometry exactly;
+// any other grid keeps the full per-cell geometry (Geometry1D::q).
 // ---------------------------------------------------------------------------
 static inline double hmax2(double a, double b) { return (b > a) ? b : a; }
 static inline double hmin2(double a, double b) { return (b < a) ? b : a; }
+
+// quad_cell%initialize for every halo'd cell: round-off snapping of the node
+// coordinates (quad_cell.f90:78-91), volume (160-175), edge lengths with their
+// equalisation (100-112), min_dx / min_dy (119-123) and the unit normals of the
+// right and top edges from the edge midpoints (193-229, vector.f90:58-96).
+static int build_quad_geometry(int ni, int nj, int nh, const double *nx, const double *ny, Geometry1D &g) {
+  const double EPS = 2.220446049250313080847e-16;
+  const int nnx = ni + 2 * nh + 1;
+  const int ncx = ni + 2 * nh, ncy = nj + 2 * nh;
+  const size_t n = (size_t)ncx * ncy;
+  auto NX = [&](int a, int b) { return nx[(size_t)a + (size_t)b * nnx]; };
+  auto NY = [&](int a, int b) { return ny[(size_t)a + (size_t)b * nnx]; };
+  g.quad = true;
+  g.q.assign((size_t)Q_COUNT * n, 0.0);
+  g.dxi.assign(ncx, 0.0);
+  g.lxi.assign(ncx, 0.0);
+  g.dyj.assign(ncy, 0.0);
+  g.lyj.assign(ncy, 0.0);
+  for (int b = 0; b < ncy; ++b)
+    for (int a = 0; a < ncx; ++a) {
+      double x[4] = {NX(a, b), NX(a + 1, b), NX(a + 1, b + 1), NX(a, b + 1)};
+      double y[4] = {NY(a, b), NY(a + 1, b), NY(a + 1, b + 1), NY(a, b + 1)};
+      if (std::fabs(y[1] - y[0]) < EPS) y[1] = y[0];
+      if (std::fabs(y[3] - y[2]) < EPS) y[3] = y[2];
+      if (std::fabs(x[2] - x[1]) < EPS) x[2] = x[1];
+      if (std::fabs(x[3] - x[0]) < EPS) x[3] = x[0];
+      const double vol = (x[1] - x[0]) * (y[3] - y[0]) - (x[3] - x[0]) * (y[1] - y[0]);
+      if (!(vol > 0.0)) return fail(CATO_ERR_INVALID, "non-positive cell volume (nodes must run counter-clockwise)");
+      double el[4];
+      for (int e = 0; e < 4; ++e) {
+        const int h = (e + 1) & 3;
+        const double ex = x[h] - x[e], ey = y[h] - y[e];
+        el[e] = std::sqrt(ex * ex + ey * ey);
+      }
+      double mx = el[0], mn = el[0];
+      for (int e = 1; e < 4; ++e) {
+        mx = hmax2(mx, el[e]);
+        mn = hmin2(mn, el[e]);
+      }
+      if (mx - mn < 2.0 * EPS) el[0] = el[1] = el[2] = el[3] = mx;
+      const size_t c = (size_t)a + (size_t)b * ncx;
+      for (int e = 0; e < 4; ++e) g.q[(size_t)(Q_LEN0 + e) * n + c] = el[e];
+      g.q[(size_t)Q_VOL * n + c] = vol;
+      g.q[(size_t)Q_MDX * n + c] = hmin2(std::fabs(x[1] - x[0]), std::fabs(x[2] - x[3]));
+      g.q[(size_t)Q_MDY * n + c] = hmin2(std::fabs(y[3] - y[0]), std::fabs(y[2] - y[1]));
+      for (int e = 1; e <= 2; ++e) {  // right edge (i-flux) and top edge (j-flux)
+        const int h = e + 1;
+        const double mxp = (x[h] + x[e]) / 2.0, myp = (y[h] + y[e]) / 2.0;
+        const double ddx = x[h] - mxp, ddy = y[h] - myp;
+        const double vx = (mxp + ddy) - mxp, vy = (myp - ddx) - myp;
+        const double len = std::sqrt(vx * vx + vy * vy);
+        g.q[(size_t)(e == 1 ? Q_NRX : Q_NTX) * n + c] = vx / len;
+        g.q[(size_t)(e == 1 ? Q_NRY : Q_NTY) * n + c] = vy / len;
+      }
+    }
+  return CATO_OK;
+}
 
 int build_geometry(int ni, int nj, int nh, const double *nx, const double *ny, Geometry1D &g) {
   const int nnx = ni + 2 * nh + 1, nny = nj + 2 * nh + 1;
   auto NX = [&](int a, int b) { return nx[(size_t)a + (size_t)b * nnx]; };
   auto NY = [&](int a, int b) { return ny[(size_t)a + (size_t)b * nnx]; };
+  g.quad = false;
+  g.q.clear();
   for (int b = 0; b < nny; ++b)
     for (int a = 0; a < nnx; ++a) {
-      if (NX(a, b) != NX(a, 0) || NY(a, b) != NY(0, b))
-        return fail(CATO_ERR_UNSUPPORTED,
-                    "grid is not rectilinear (node_x must depend on i only and node_y on j only)");
+      if (NX(a, b) != NX(a, 0) || NY(a, b) != NY(0, b)) return build_quad_geometry(ni, nj, nh, nx, ny, g);
     }
   const int ncx = ni + 2 * nh, ncy = nj + 2 * nh;
   g.dxi.resize(ncx);
@@ -108,13 +166,14 @@ int build_geometry(int ni, int nj, int nh, const double *nx, const double *ny, G
     if (!(d > 0.0)) return fail(CATO_ERR_INVALID, "non-positive cell width in y");
   }
   // Cross-check the separable form against the full quad-cell algorithm on
-  // every cell (normals must be exactly axis aligned for the kernel's fluxes).
+  // every cell; a grid whose volumes or normals it would not reproduce exactly
+  // takes the per-cell geometry instead.
   for (int b = 0; b < ncy; ++b)
     for (int a = 0; a < ncx; ++a) {
       double x[4] = {NX(a, b), NX(a + 1, b), NX(a + 1, b + 1), NX(a, b + 1)};
       double y[4] = {NY(a, b), NY(a + 1, b), NY(a + 1, b + 1), NY(a, b + 1)};
       double vol = (x[1] - x[0]) * (y[3] - y[0]) - (x[3] - x[0]) * (y[1] - y[0]);
-      if (vol != g.dxi[a] * g.dyj[b]) return fail(CATO_ERR_UNSUPPORTED, "cell volume is not separable");
+      if (vol != g.dxi[a] * g.dyj[b]) return build_quad_geometry(ni, nj, nh, nx, ny, g);
       const int head[4] = {1, 2, 3, 0};
       for (int e = 1; e <= 2; ++e) {  // RIGHT and TOP edges carry the i and j fluxes
         double mx = (x[head[e]] + x[e]) / 2.0, my = (y[head[e]] + y[e]) / 2.0;
@@ -122,8 +181,7 @@ int build_geometry(int ni, int nj, int nh, const double *nx, const double *ny, G
         double vx = (mx + ddy) - mx, vy = (my - ddx) - my;
         double len = std::sqrt(vx * vx + vy * vy);
         double ex = (e == 1) ? 1.0 : 0.0, ey = (e == 1) ? 0.0 : 1.0;
-        if (vx / len != ex || vy / len != ey)
-          return fail(CATO_ERR_UNSUPPORTED, "edge normals are not axis aligned on this grid");
+        if (vx / len != ex || vy / len != ey) return build_quad_geometry(ni, nj, nh, nx, ny, g);
       }
     }
   return CATO_OK;
@@ -173,7 +231,7 @@ int default_tile(const cato_config &c, int ni, int nj) {
   return 0;  // 32 x 8
 }
 
-int Solve::init(const cato_config &c, const cato_geom &g) {
+int Solve::init(const cato_config &c, const cato_geom &g, bool allow_quad) {
   int rc = validate_config(c);
   if (rc) return rc;
   if (g.ni < c.n_ghost_layers || g.nj < c.n_ghost_layers)
@@ -187,10 +245,14 @@ int Solve::init(const cato_config &c, const cato_geom &g) {
   pitch = (ni + (ghost_column_bc(c.bc_plus_x) ? 1 : 0) + 15) & ~15;
   rc = build_geometry(ni, nj, nh, g.node_x, g.node_y, geo);
   if (rc) return rc;
-  tile = default_tile(c, ni, nj);
+  if (geo.quad && !allow_quad)
+    return fail(CATO_ERR_UNSUPPORTED,
+                "general quadrilateral grids run as single solves (cato_ctx); this form takes rectilinear grids");
+  // quad cells: the one tile shape their kernels are instantiated for
+  tile = geo.quad ? kQuadTile : default_tile(c, ni, nj);
   tiles_x = (ni + kTiles[tile].tx - 1) / kTiles[tile].tx;
   tiles_y = (nj + kTiles[tile].ty - 1) / kTiles[tile].ty;
-  launcher = stage_launcher(c.flux, c.limiter, c.precision, tile);
+  launcher = stage_launcher(c.flux, c.limiter, c.precision, tile, geo.quad ? 1 : 0);
   if (!launcher) return fail(CATO_ERR_INVALID, "no stage kernel for this configuration");
   return CATO_OK;
 }
@@ -201,8 +263,8 @@ size_t Solve::device_bytes() const {
   size_t f = field_elems();
   size_t nb = (size_t)tiles_x * tiles_y;
   size_t geo_n = (size_t)2 * (ni + 2 * nh) + 2 * (nj + 2 * nh);
-  return (12 + 12) * f * sizeof(double) + geo_n * sizeof(double) + nb * (sizeof(double) + sizeof(int)) +
-         sizeof(LoopState) + 4 * sizeof(double) + 1024;
+  return (12 + 12) * f * sizeof(double) + geo_n * sizeof(double) + (size_t)Q_COUNT * quad_elems() * sizeof(double) +
+         nb * (sizeof(double) + sizeof(int)) + sizeof(LoopState) + 4 * sizeof(double) + 1024;
 }
 
 // Carve the device arrays of one solve out of `base` (device_bytes() long).
@@ -223,6 +285,8 @@ void Solve::bind(char *base) {
   lxi = p; p += ni + 2 * nh;
   dyj = p; p += nj + 2 * nh;
   lyj = p; p += nj + 2 * nh;
+  quad = geo.quad ? p : nullptr;
+  p += (size_t)Q_COUNT * quad_elems();
   block_min = p; p += (size_t)tiles_x * tiles_y;
   result = p; p += 4;
   char *c = reinterpret_cast<char *>(p);
@@ -237,6 +301,7 @@ int Solve::upload_geometry(hipStream_t s) {
   HIPCK(hipMemcpyAsync(lxi, geo.lxi.data(), geo.lxi.size() * 8, hipMemcpyHostToDevice, s));
   HIPCK(hipMemcpyAsync(dyj, geo.dyj.data(), geo.dyj.size() * 8, hipMemcpyHostToDevice, s));
   HIPCK(hipMemcpyAsync(lyj, geo.lyj.data(), geo.lyj.size() * 8, hipMemcpyHostToDevice, s));
+  if (geo.quad) HIPCK(hipMemcpyAsync(quad, geo.q.data(), geo.q.size() * 8, hipMemcpyHostToDevice, s));
   HIPCK(hipMemsetAsync(block_err, 0, (size_t)tiles_x * tiles_y * sizeof(int), s));
   LoopState L{};
   HIPCK(hipMemcpyAsync(loop, &L, sizeof(L), hipMemcpyHostToDevice, s));
@@ -258,6 +323,9 @@ FieldGeom Solve::fgeom() const {
   g.bc_pressure = cfg.bc_pressure;
   g.dxi = dxi;
   g.dyj = dyj;
+  g.quad = quad;
+  g.quad_ncx = ni + 2 * nh;
+  g.quad_n = (long long)quad_elems();
   return g;
 }
 
@@ -303,6 +371,9 @@ StageArgs Solve::stage_args(int mode, double a, double b, double c, int in_buf, 
   s.block_err = block_err;
   s.halo_mask = halo_mask;
   s.halo = halo;
+  s.quad = quad;
+  s.quad_ncx = ni + 2 * nh;
+  s.quad_n = (long long)quad_elems();
   return s;
 }
 
@@ -739,7 +810,7 @@ int cato_abi_version(void) { return CATO_HIP_ABI_VERSION; }
 int cato_device_bytes(const cato_config *cfg, const cato_geom *geom, unsigned long long *bytes) {
   if (!cfg || !geom || !bytes) return fail(CATO_ERR_INVALID, "null argument");
   Solve s;
-  int rc = s.init(*cfg, *geom);
+  int rc = s.init(*cfg, *geom, true);
   if (rc) return rc;
   *bytes = (unsigned long long)s.device_bytes();
   return CATO_OK;
@@ -758,7 +829,7 @@ int cato_ctx_create(const cato_config *cfg, const cato_geom *geom, cato_ctx **ou
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(CATO_ERR_NO_DEVICE, "no HIP device");
   cato_ctx *c = new cato_ctx();
-  int rc = c->s.init(*cfg, *geom);
+  int rc = c->s.init(*cfg, *geom, true);
   if (rc) {
     delete c;
     return rc;

@@ -16,8 +16,13 @@ int hip_check(hipError_t e, const char *what);
 
 // Separable geometry of a rectilinear block, indexed by halo'd cell (0-based):
 // dxi = x(i+1)-x(i), lxi = sqrt(dxi**2 + 0**2) (bottom/top edge length), same for j.
+// Any other grid (quad = true): the quad_cell geometry of every halo'd cell in
+// Q_COUNT arrays of (ni + 2 nh) (nj + 2 nh) values (QuadSlot order); the 1D
+// arrays are then zeros.
 struct Geometry1D {
   std::vector<double> dxi, lxi, dyj, lyj;
+  bool quad = false;
+  std::vector<double> q;
 };
 int build_geometry(int ni, int nj, int nh, const double *nx, const double *ny, Geometry1D &g);
 int default_tile(const cato_config &c, int ni, int nj);
@@ -35,6 +40,7 @@ struct Solve {
   double *P[3][4] = {};      // primitive buffer sets; P[0] = primitives of U^n between steps
                              // (precision form: P[0] only, refreshed on demand, see prims_stale)
   double *dxi = nullptr, *lxi = nullptr, *dyj = nullptr, *lyj = nullptr;
+  double *quad = nullptr;    // Q_COUNT per-cell geometry arrays (general quadrilateral grids only)
   double *block_min = nullptr;
   double *result = nullptr;
   int *block_err = nullptr;
@@ -53,7 +59,10 @@ struct Solve {
   bool prims_valid = false; // P[0] holds the caller's primitives (not yet checked by a stage)
   bool prims_stale = false; // precision form: stages left P[0] behind U[cur] (they store no primitives)
 
-  int init(const cato_config &c, const cato_geom &g);
+  // allow_quad: accept a general quadrilateral grid (single solves; the batched,
+  // served and split forms take rectilinear grids only)
+  int init(const cato_config &c, const cato_geom &g, bool allow_quad = false);
+  size_t quad_elems() const { return geo.quad ? (size_t)(ni + 2 * nh) * (nj + 2 * nh) : 0; }
   size_t field_elems() const;
   size_t device_bytes() const;
   void bind(char *base);

@@ -251,8 +251,10 @@ struct RedSmem {
 // Phase 2 of a stage tile: every i- and j-edge of the tile once, reconstructed
 // from the LDS primitives and its flux written to LDS.  D / DF: the division
 // and square root of the FP64 reconstruction / flux (IeeeOps, or FastOps on
-// in-range tiles).
-template <int FLUX, int LIMC, typename R, int TX, int TY, class D, class DF>
+// in-range tiles).  GEOM: general quadrilateral cells, each edge's normal read
+// from the per-cell geometry (the right edge of its left cell for an i-edge, the
+// top edge for a j-edge).
+template <int FLUX, int LIMC, typename R, int TX, int TY, class D, class DF, bool GEOM>
 __device__ inline void edge_pass(const StageArgs &a, TileSmem<LIMC, R, TX, TY> &sm, int i0, int j0) {
   constexpr int H = halo_of<LIMC>();
   constexpr int RX = TX + 2 * H;
@@ -303,13 +305,20 @@ __device__ inline void edge_pass(const StageArgs &a, TileSmem<LIMC, R, TX, TY> &
       pn.p1p = pc[sn + st];
       pn.p1m = pc[sn - st];
     }
+    if constexpr (GEOM) {
+      const size_t gc = (size_t)(ci + a.nh - 1) + (size_t)(cj + a.nh - 1) * (size_t)a.quad_ncx;
+      const size_t n = (size_t)a.quad_n;
+      s.nx = (R)a.quad[(idir ? Q_NRX : Q_NTX) * n + gc];
+      s.ny = (R)a.quad[(idir ? Q_NRY : Q_NTY) * n + gc];
+    }
+    constexpr int DG = GEOM ? 2 : 0;
     R f[4];
     if (idir) {
-      edge_flux<R, FLUX, 0, DF>(gamma, a.low_mach_fix, s, pn, f, &sb);
+      edge_flux<R, FLUX, DG, DF>(gamma, a.low_mach_fix, s, pn, f, &sb);
 #pragma unroll
       for (int v = 0; v < 4; ++v) sfi[v][ey][ex] = f[v];
     } else {
-      edge_flux<R, FLUX, 1, DF>(gamma, a.low_mach_fix, s, pn, f, &sb);
+      edge_flux<R, FLUX, DG + 1, DF>(gamma, a.low_mach_fix, s, pn, f, &sb);
 #pragma unroll
       for (int v = 0; v < 4; ++v) sfj[v][ey][ex] = f[v];
     }
@@ -321,7 +330,7 @@ __device__ inline void edge_pass(const StageArgs &a, TileSmem<LIMC, R, TX, TY> &
 // also writes the +x outflow ghost column of the new state (the batched
 // kernel); the single-solve launch leaves that to k_outflow_column, since the
 // extra epilogue slows every tile of the fused kernel by ~5%.
-template <int FLUX, int LIMC, typename R, int TX, int TY, bool OUTFLOW_COL = false>
+template <int FLUX, int LIMC, typename R, int TX, int TY, bool OUTFLOW_COL = false, bool GEOM = false>
 __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileSmem<LIMC, R, TX, TY> &sm,
                                   RedSmem &red) {
   constexpr int H = halo_of<LIMC>();
@@ -380,11 +389,11 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
     for (int w = 0; w < kThreads / 64; ++w) fast = fast && red.fast[w] != 0;
     using DF = typename std::conditional<fast_ops_flux<FLUX>(), FastOps, IeeeOps>::type;
     if (fast)
-      edge_pass<FLUX, LIMC, R, TX, TY, FastOps, DF>(a, sm, i0, j0);
+      edge_pass<FLUX, LIMC, R, TX, TY, FastOps, DF, GEOM>(a, sm, i0, j0);
     else
-      edge_pass<FLUX, LIMC, R, TX, TY, IeeeOps, IeeeOps>(a, sm, i0, j0);
+      edge_pass<FLUX, LIMC, R, TX, TY, IeeeOps, IeeeOps, GEOM>(a, sm, i0, j0);
   } else {
-    edge_pass<FLUX, LIMC, R, TX, TY, IeeeOps, IeeeOps>(a, sm, i0, j0);
+    edge_pass<FLUX, LIMC, R, TX, TY, IeeeOps, IeeeOps, GEOM>(a, sm, i0, j0);
   }
   __syncthreads();
 
@@ -401,13 +410,28 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
     int cy = c / TX, cx = c - cy * TX;
     int gi = i0 + cx, gj = j0 + cy;
     if (gi > ni || gj > nj) continue;
-    double dxv = a.dxi[gi + nh - 1], dyv = a.dyj[gj + nh - 1];
-    double l1 = a.lxi[gi + nh - 1], l2 = a.lyj[gj + nh - 1];
-    // edge-length equalisation (quad_cell.f90:110-112) over (l1, l2, l1, l2)
-    double mx = fmax2(l1, l2), mn = fmin2(l1, l2);
-    double lb = l1, lr = l2, lt = l1, ll = l2;
-    if (mx - mn < 2.0 * EPSD) lb = lr = lt = ll = mx;
-    double vol = dxv * dyv;
+    double dxv, dyv, lb, lr, lt, ll, vol;
+    if constexpr (GEOM) {
+      // quad_cell geometry of the cell: min_dx / min_dy, bottom / right / top / left lengths, volume
+      const size_t gc = (size_t)(gi + nh - 1) + (size_t)(gj + nh - 1) * (size_t)a.quad_ncx;
+      const size_t n = (size_t)a.quad_n;
+      dxv = a.quad[Q_MDX * n + gc];
+      dyv = a.quad[Q_MDY * n + gc];
+      lb = a.quad[Q_LEN0 * n + gc];
+      lr = a.quad[Q_LEN1 * n + gc];
+      lt = a.quad[Q_LEN2 * n + gc];
+      ll = a.quad[Q_LEN3 * n + gc];
+      vol = a.quad[Q_VOL * n + gc];
+    } else {
+      dxv = a.dxi[gi + nh - 1];
+      dyv = a.dyj[gj + nh - 1];
+      double l1 = a.lxi[gi + nh - 1], l2 = a.lyj[gj + nh - 1];
+      // edge-length equalisation (quad_cell.f90:110-112) over (l1, l2, l1, l2)
+      double mx = fmax2(l1, l2), mn = fmin2(l1, l2);
+      lb = l1, lr = l2, lt = l1, ll = l2;
+      if (mx - mn < 2.0 * EPSD) lb = lr = lt = ll = mx;
+      vol = dxv * dyv;
+    }
     uint64_t vb = (uint64_t)__double_as_longlong(vol);
     bool pow2 = (vb & 0x000FFFFFFFFFFFFFull) == 0 && ((vb >> 52) & 0x7FF) > 1 && ((vb >> 52) & 0x7FF) < 2046;
     double inv_vol = kExact ? (pow2 ? 1.0 / vol : 0.0) : rcp64(vol);
@@ -478,6 +502,7 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
       if (q.rho < 0.0) err |= 2;
       if (q.p < 0.0) err |= 4;
       // get_timestep (mod_fluid.f90:554-558); min_dx/min_dy = |dx|, |dy| on rectilinear cells
+      // (quad cells: the non-negative min_dx / min_dy themselves)
       double val = a.cfl / (((absd(q.u) + cs) / absd(dxv)) + ((absd(q.v) + cs) / absd(dyv)));
       lmin = (val < lmin) ? val : lmin;
     }
@@ -523,13 +548,13 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
 }
 
 // Single-solve stage launch: one workgroup per tile, XCD-aware tile order.
-template <int FLUX, int LIMC, typename R, int TX, int TY>
+template <int FLUX, int LIMC, typename R, int TX, int TY, bool GEOM = false>
 __global__ __launch_bounds__(kThreads) void stage_kernel(StageArgs a) {
   __shared__ TileSmem<LIMC, R, TX, TY> sm;
   __shared__ RedSmem red;
   if (a.loop && a.loop->done) return;
   const double dt = a.loop ? a.loop->dt : a.dt_value;
-  stage_tile<FLUX, LIMC, R, TX, TY>(a, xcd_tile(blockIdx.x, gridDim.x), dt, sm, red);
+  stage_tile<FLUX, LIMC, R, TX, TY, false, GEOM>(a, xcd_tile(blockIdx.x, gridDim.x), dt, sm, red);
 }
 
 }  // namespace cato

@@ -25,6 +25,19 @@ def uniform_nodes(ni: int, nj: int, nh: int, xrange=(0.0, 1.0), yrange=(0.0, 1.0
     return np.asfortranarray(x2), np.asfortranarray(y2)
 
 
+def wavy_nodes(ni: int, nj: int, nh: int, amp=0.04, xrange=(0.0, 1.0), yrange=(0.0, 1.0)):
+    """A curvilinear grid: the uniform nodes displaced by amp * (domain length) * sin(2 pi x) sin(2 pi y)
+    (x) and sin(2 pi x + 1) sin(2 pi y) (y), periodic in both directions, cells skewed by up to
+    atan(2 pi amp) (14 degrees at amp = 0.04) and every volume positive for amp < 0.11."""
+    x, y = uniform_nodes(ni, nj, nh, xrange, yrange)
+    lx, ly = float(xrange[1] - xrange[0]), float(yrange[1] - yrange[0])
+    xn, yn = (x - xrange[0]) / lx, (y - yrange[0]) / ly
+    s = np.sin(2 * np.pi * yn)
+    xw = x + amp * lx * np.sin(2 * np.pi * xn) * s
+    yw = y + amp * ly * np.sin(2 * np.pi * xn + 1.0) * s
+    return np.asfortranarray(xw), np.asfortranarray(yw)
+
+
 def cell_centers(node_x, node_y, nh):
     """Interior cell centres as pycato computes them (x_2d[:-1,:-1] + dx/2)."""
     xc = 0.5 * (node_x[:-1, :-1] + node_x[1:, :-1])

@@ -85,7 +85,7 @@ enum { CATO_PRECISION_FP64 = 0, CATO_PRECISION_FP32_FLUX = 1 };
 enum {
   CATO_OK = 0,
   CATO_ERR_INVALID = -1,     /* bad argument or configuration */
-  CATO_ERR_UNSUPPORTED = -2, /* valid for the reference but not for this path (e.g. curvilinear grid) */
+  CATO_ERR_UNSUPPORTED = -2, /* valid for the reference but not for this path (e.g. a quad grid in a batch) */
   CATO_ERR_HIP = -3,         /* HIP runtime failure */
   CATO_ERR_NO_DEVICE = -4
 };
@@ -125,8 +125,12 @@ int cato_num_tiles(void);
 int cato_tile_shape(int tile, int *tile_x, int *tile_y);
 
 /* fluid_t%initialize + grid_block_2d_t geometry: validates the configuration,
- * builds the per-cell geometry from the nodes (quad_cell.f90) and allocates the
- * device-resident state.  Curvilinear grids return CATO_ERR_UNSUPPORTED. */
+ * builds the per-cell geometry from the nodes (quad_cell.f90:69-229) and
+ * allocates the device-resident state.  Rectilinear grids keep separable 1D
+ * geometry; any other quadrilateral grid keeps per-cell volumes, edge lengths,
+ * min_dx / min_dy and edge normals (the flux path of flux_solver.f90:254-367,
+ * 398-422 reads them) and runs the 16 x 16 tile.  A non-positive cell volume is
+ * CATO_ERR_INVALID. */
 int cato_ctx_create(const cato_config *cfg, const cato_geom *geom, cato_ctx **out);
 int cato_ctx_destroy(cato_ctx *ctx);
 
@@ -261,7 +265,7 @@ int cato_residual(cato_ctx *ctx, double *rho, double *u, double *v, double *p, d
 
 /* ---- batched form: many independent solves in one persistent launch per stage ---- */
 typedef struct cato_batch cato_batch;
-/* Each solve k has its own config and geometry; all solves share the precision and tile. */
+/* Each solve k has its own config and (rectilinear) geometry; all solves share the precision and tile. */
 int cato_batch_create(int n, const cato_config *cfgs, const cato_geom *geoms, int precision, int tile,
                       cato_batch **out);
 int cato_batch_destroy(cato_batch *b);
@@ -318,7 +322,8 @@ double cato_server_clock(cato_server *s);
  * (mod_fluid.f90:560).  Here every block is a sub-domain on the same GPU and the exchange is a pack kernel
  * (each block gathers what its neighbours need into a send buffer) and an unpack kernel (each block pulls
  * its neighbours' send regions into its halo store).  Results are bit-identical to the unsplit solve.
- * FP64 form; periodic, zero-gradient and symmetry sides.  Host arrays as for cato_ctx (whole grid). */
+ * FP64 form, rectilinear grids; periodic, zero-gradient and symmetry sides.  Host arrays as for cato_ctx
+ * (whole grid). */
 typedef struct cato_split cato_split;
 int cato_split_create(const cato_config *cfg, const cato_geom *geom, int px, int py, cato_split **out);
 int cato_split_destroy(cato_split *split);
