@@ -96,8 +96,10 @@ def rel_errors(ref, got):
     return out
 
 
-def timed_kh(n, precision, steps, warmup, tile, profile_only=False, nj=None):
-    """Advance the KH workload; returns (wall seconds, stage-kernel mean ms, launches, final state)."""
+def timed_kh(n, precision, steps, warmup, tile, profile_only=False, nj=None, extra_steps=None):
+    """Advance the KH workload; returns (wall seconds, stage-kernel mean ms, launches, final state).
+    extra_steps: take exactly that many untimed steps after the warm-up instead of stepping for WARM_S
+    seconds (a second form of the same workload ends on the same step count, so the states compare)."""
     from fvleg_2d_amd import Solver
     from fvleg_2d_amd.problems import kelvin_helmholtz
 
@@ -118,7 +120,8 @@ def timed_kh(n, precision, steps, warmup, tile, profile_only=False, nj=None):
         # solve started, so that the timed window does not open on clocks that idled during the
         # host-side set-up
         extra, t_w = 0, time.perf_counter()
-        while not profile_only and time.perf_counter() - t_w < WARM_S:
+        while not profile_only and (time.perf_counter() - t_w < WARM_S if extra_steps is None
+                                    else extra < extra_steps):
             s.advance(8)
             s.synchronize()
             extra += 8
@@ -332,7 +335,8 @@ def main():
     out["config"]["extra_warmup_steps"] = pb.meta.get("extra_warmup_steps")
     if args.forms == "all" and args.precision == "fp64":
         say("[bench] precision form (FP32 flux, FP64 state) on the same workload")
-        pb32, wall32, ms32, l32, state32 = timed_kh(args.n, "fp32_flux", args.steps, args.warmup, args.tile)
+        pb32, wall32, ms32, l32, state32 = timed_kh(args.n, "fp32_flux", args.steps, args.warmup, args.tile,
+                                                    extra_steps=pb.meta.get("extra_warmup_steps"))
         out["precision_form"] = {
             "value": cells * args.steps / wall32,
             "unit": "cell-updates/s",
@@ -342,6 +346,7 @@ def main():
             "roofline": roofline(pb32, ms32, l32, "fp32_flux", args.n, triad_gbs),
             "valu": valu_figure(ms32, "fp32_flux", args.n, pb32.cfg.tile),
             "difference_from_fp64_state": rel_errors(state, state32),
+            "difference_after_steps": args.warmup + pb32.meta.get("extra_warmup_steps", 0) + args.steps,
         }
         say("[bench] one GPU's share of the 2048^2 solve split over 8 GPUs: 512x1024")
         pbs, walls, mss, ls, _ = timed_kh(512, "fp64", 1000, args.warmup, pb.cfg.tile, nj=1024)

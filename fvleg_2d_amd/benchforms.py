@@ -4,6 +4,7 @@ from __future__ import annotations
 import copy
 import dataclasses
 import math
+import sys
 import time
 
 import numpy as np
@@ -121,10 +122,17 @@ def _as_precision(pb, precision):
     return dataclasses.replace(pb, cfg=cfg)
 
 
+def progress(msg):
+    """A progress line on stderr (long legs print one at least every few tens of seconds)."""
+    print(msg, file=sys.stderr, flush=True)
+
+
 def plain_queue_service_times(reqs):
     """Each request alone through the plain FP64 path (create, upload, run, download)."""
     times, results = [], []
-    for pb in reqs:
+    for k, pb in enumerate(reqs):
+        if k and k % 16 == 0:
+            progress(f"[served] plain FP64 queue: {k} of {len(reqs)} requests, {sum(times):.1f} s")
         q = _as_precision(pb, "fp64")
         t0 = time.perf_counter()
         with Solver(q.cfg, q.node_x, q.node_y) as s:
@@ -156,6 +164,7 @@ def served_run(reqs, arrivals, budget_bytes, max_slots=64, steps_per_tick=8, pre
         n = len(reqs)
         ticks = 0
         max_res = 0
+        reported = 0
         while True:
             now = srv.clock() - t0
             while nxt < n and arrivals[nxt] <= now:
@@ -166,6 +175,9 @@ def served_run(reqs, arrivals, budget_bytes, max_slots=64, steps_per_tick=8, pre
             max_res = max(max_res, r)
             if c == n:
                 break
+            if c >= reported + 16:
+                reported = c - c % 16
+                progress(f"[served] {c} of {n} requests done at {srv.clock() - t0:.1f} s")
             if r == 0 and q == 0 and nxt < n:
                 time.sleep(max(0.0, min(0.01, arrivals[nxt] - (srv.clock() - t0))))
         polls = [srv.poll(i) for i in ids]
