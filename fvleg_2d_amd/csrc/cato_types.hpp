@@ -130,6 +130,10 @@ struct StageArgs {
   const double *quad;
   int quad_ncx;
   long long quad_n;
+  // Elements between the four variables of every state and primitive set above: the kernel reads
+  // uk[0] + v * fstride (likewise u0, prim, out, prim_out, conv_first), so it keeps one base pointer
+  // per set in scalar registers instead of four (validate_stage_args checks the layout)
+  long long fstride;
 };
 
 // Per-cell geometry of a general quadrilateral grid (quad_cell.f90:69-229), one

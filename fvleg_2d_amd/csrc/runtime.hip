@@ -374,6 +374,7 @@ StageArgs Solve::stage_args(int mode, double a, double b, double c, int in_buf, 
   s.quad = quad;
   s.quad_ncx = ni + 2 * nh;
   s.quad_n = (long long)quad_elems();
+  s.fstride = (long long)field_elems();
   return s;
 }
 
@@ -542,7 +543,19 @@ int validate_stage_args(const StageArgs &s) {
       if (!p[v]) return fail(CATO_ERR_INVALID, std::string("internal: stage launch without ") + what);
     return CATO_OK;
   };
+  // every set's four variables sit fstride elements apart (the kernel addresses them from p[0])
+  auto strided = [&](const double *const *p, const char *what) -> int {
+    if (!p[0]) return CATO_OK;
+    for (int v = 1; v < 4; ++v)
+      if (p[v] != p[0] + (size_t)v * (size_t)s.fstride)
+        return fail(CATO_ERR_INVALID, std::string("internal: ") + what + " are not one strided set");
+    return CATO_OK;
+  };
   int rc;
+  if ((rc = strided(s.uk, "U_k arrays")) || (rc = strided(s.u0, "U_0 arrays")) ||
+      (rc = strided(s.prim, "primitive arrays")) || (rc = strided(s.out, "output arrays")) ||
+      (rc = strided(s.prim_out, "output primitive arrays")) || (rc = strided(s.conv_first, "convergence arrays")))
+    return rc;
   if ((rc = need(s.out, "output state"))) return rc;
   if ((rc = need(s.prim, "primitive arrays"))) return rc;
   if (s.mode != MODE_RESIDUAL && (rc = need(s.prim_out, "output primitive arrays"))) return rc;

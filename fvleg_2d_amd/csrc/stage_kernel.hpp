@@ -99,6 +99,12 @@ __device__ inline int wave_or(int v) {
   return v;
 }
 
+// The four variables of a strided set at one cell.
+__device__ inline Prim<double> load4(const double *p, size_t off, long long fstride) {
+  const size_t fs = (size_t)fstride;
+  return Prim<double>{p[off], p[off + fs], p[off + 2 * fs], p[off + 3 * fs]};
+}
+
 // Sub-domain of a split solve: halo'd cell (gi, gj) when a side takes its
 // ghosts from the halo store.  The physical sides map first (as the reference
 // fills boundary ghosts after the exchange, over the full transverse extent),
@@ -114,7 +120,7 @@ __device__ inline Prim<double> load_prim_split(const StageArgs &a, int gi, int g
   Prim<double> pr;
   if (!xo && !yo) {
     const size_t off = (size_t)(sj - 1) * a.pitch + (size_t)(si - 1);
-    pr = Prim<double>{a.prim[0][off], a.prim[1][off], a.prim[2][off], a.prim[3][off]};
+    pr = load4(a.prim[0], off, a.fstride);
   } else {
     const int kx = si > ni ? si - ni - 1 : si + nh - 1, ky = sj > nj ? sj - nj - 1 : sj + nh - 1;
     int region;
@@ -160,11 +166,7 @@ __device__ inline Prim<double> load_prim(const StageArgs &a, int gi, int gj) {
   int sj = map_axis(gj, a.nj, a.bc[2], a.bc[3], fv);
   if (gcol && a.bc[0] == BC_PRESSURE_INPUT) fv = false;
   size_t off = (size_t)(sj - 1) * a.pitch + (size_t)(si - 1);
-  Prim<double> pr;
-  pr.rho = a.prim[0][off];
-  pr.u = a.prim[1][off];
-  pr.v = a.prim[2][off];
-  pr.p = a.prim[3][off];
+  Prim<double> pr = load4(a.prim[0], off, a.fstride);
   if (fu) pr.u = -pr.u;
   if (fv) pr.v = -pr.v;
   return pr;
@@ -198,14 +200,15 @@ __device__ inline Prim<double> load_recovered(const StageArgs &a, int gi, int gj
   // a pass before the stage stores them in column ni+1 of the primitive arrays
   if (gi > a.ni && a.bc[0] == BC_PRESSURE_INPUT) {
     size_t off = (size_t)(map_axis(gj, a.nj, a.bc[2], a.bc[3], fv) - 1) * a.pitch + (size_t)a.ni;
-    return Prim<double>{a.prim[0][off], a.prim[1][off], a.prim[2][off], a.prim[3][off]};
+    return load4(a.prim[0], off, a.fstride);
   }
   int si = map_axis(gi, a.ni, a.bc[0], a.bc[1], fu);
   int sj = map_axis(gj, a.nj, a.bc[2], a.bc[3], fv);
   size_t off = (size_t)(sj - 1) * a.pitch + (size_t)(si - 1);
   double cs;
-  Prim<double> pr = cons_to_prim_fast(a.uk[0][off], a.uk[1][off], a.uk[2][off], a.uk[3][off], a.gamma,
-                                      a.gamma - 1.0, 0, &cs);
+  const size_t fs = (size_t)a.fstride;
+  Prim<double> pr = cons_to_prim_fast(a.uk[0][off], a.uk[0][off + fs], a.uk[0][off + 2 * fs],
+                                      a.uk[0][off + 3 * fs], a.gamma, a.gamma - 1.0, 0, &cs);
   if (fu) pr.u = -pr.u;
   if (fv) pr.v = -pr.v;
   if (gi > a.ni && a.bc[0] == BC_OUTFLOW) pr = outflow_ghost(pr, a.gamma, a.bc_pressure);
@@ -360,21 +363,48 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
   constexpr bool kFastScheme =
       kExact && (fast_ops_flux<FLUX>() || LIMC == LIM_MLP3 || LIMC == LIM_MLP5);
   bool in_range = true;
-  for (int idx = tid; idx < RX * RY; idx += kThreads) {
-    int ly = idx / RX, lx = idx - ly * RX;
-    int gi = i0 - H + lx, gj = j0 - H + ly;
-    Prim<double> pr{0.0, 0.0, 0.0, 0.0};
-    if (gi >= 1 - nh && gi <= ni + nh && gj >= 1 - nh && gj <= nj + nh) {
-      pr = recover ? load_recovered(a, gi, gj) : load_prim(a, gi, gj);
-      if ((recover || a.check_input) && gi >= 1 && gi <= ni && gj >= 1 && gj <= nj &&
-          (pr.rho != pr.rho || pr.u != pr.u || pr.v != pr.v || pr.p != pr.p))
-        err |= 1;
+  // A tile whose halo lies inside the domain (all but the tiles along its
+  // sides) reads its cells directly, without the boundary map
+  const bool inner = !a.halo_mask && i0 - H >= 1 && j0 - H >= 1 && i0 + TX - 1 + H <= ni && j0 + TY - 1 + H <= nj;
+  if (inner) {
+    const bool check = recover || a.check_input;
+    const size_t base = (size_t)(j0 - H - 1) * a.pitch + (size_t)(i0 - H - 1);
+    const size_t fs = (size_t)a.fstride;
+    for (int idx = tid; idx < RX * RY; idx += kThreads) {
+      int ly = idx / RX, lx = idx - ly * RX;
+      const size_t off = base + (size_t)ly * a.pitch + (size_t)lx;
+      Prim<double> pr;
+      if (recover) {
+        double cs;
+        pr = cons_to_prim_fast(a.uk[0][off], a.uk[0][off + fs], a.uk[0][off + 2 * fs], a.uk[0][off + 3 * fs],
+                               a.gamma, a.gamma - 1.0, 0, &cs);
+      } else {
+        pr = load4(a.prim[0], off, a.fstride);
+      }
+      if (check && (pr.rho != pr.rho || pr.u != pr.u || pr.v != pr.v || pr.p != pr.p)) err |= 1;
       if constexpr (kFastScheme) in_range = in_range && fast_ops_cell<fast_ops_bound<FLUX, LIMC>()>(pr);
+      sp[0][ly][lx] = (R)pr.rho;
+      sp[1][ly][lx] = (R)pr.u;
+      sp[2][ly][lx] = (R)pr.v;
+      sp[3][ly][lx] = (R)pr.p;
     }
-    sp[0][ly][lx] = (R)pr.rho;
-    sp[1][ly][lx] = (R)pr.u;
-    sp[2][ly][lx] = (R)pr.v;
-    sp[3][ly][lx] = (R)pr.p;
+  } else {
+    for (int idx = tid; idx < RX * RY; idx += kThreads) {
+      int ly = idx / RX, lx = idx - ly * RX;
+      int gi = i0 - H + lx, gj = j0 - H + ly;
+      Prim<double> pr{0.0, 0.0, 0.0, 0.0};
+      if (gi >= 1 - nh && gi <= ni + nh && gj >= 1 - nh && gj <= nj + nh) {
+        pr = recover ? load_recovered(a, gi, gj) : load_prim(a, gi, gj);
+        if ((recover || a.check_input) && gi >= 1 && gi <= ni && gj >= 1 && gj <= nj &&
+            (pr.rho != pr.rho || pr.u != pr.u || pr.v != pr.v || pr.p != pr.p))
+          err |= 1;
+        if constexpr (kFastScheme) in_range = in_range && fast_ops_cell<fast_ops_bound<FLUX, LIMC>()>(pr);
+      }
+      sp[0][ly][lx] = (R)pr.rho;
+      sp[1][ly][lx] = (R)pr.u;
+      sp[2][ly][lx] = (R)pr.v;
+      sp[3][ly][lx] = (R)pr.p;
+    }
   }
   if constexpr (kFastScheme) {
     const int vote = __all(in_range ? 1 : 0);
@@ -436,10 +466,11 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
     bool pow2 = (vb & 0x000FFFFFFFFFFFFFull) == 0 && ((vb >> 52) & 0x7FF) > 1 && ((vb >> 52) & 0x7FF) < 2046;
     double inv_vol = kExact ? (pow2 ? 1.0 / vol : 0.0) : rcp64(vol);
     size_t off = (size_t)(gj - 1) * a.pitch + (size_t)(gi - 1);
+    const size_t fs = (size_t)a.fstride;
     double fst[4] = {0.0, 0.0, 0.0, 0.0};  // issued before the flux balance to hide their latency
     if (conv) {
 #pragma unroll
-      for (int v = 0; v < 4; ++v) fst[v] = a.conv_first[v][off];
+      for (int v = 0; v < 4; ++v) fst[v] = a.conv_first[0][off + v * fs];
     }
     double nv[4];
 #pragma unroll
@@ -460,11 +491,11 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
       if (a.mode == MODE_RESIDUAL) {
         r = L;
       } else if (a.mode == MODE_FIRST) {
-        r = op_add(a.uk[v][off], L * (a.c * dt));
+        r = op_add(a.uk[0][off + v * fs], L * (a.c * dt));
       } else if (a.mode == MODE_BASE) {
-        r = op_add(ld_cell<kExact>(&a.u0[v][off]), L * (dt * a.c));
+        r = op_add(ld_cell<kExact>(&a.u0[0][off + v * fs]), L * (dt * a.c));
       } else {
-        double t = op_add(ld_cell<kExact>(&a.u0[v][off]) * a.a, a.uk[v][off] * a.b);
+        double t = op_add(ld_cell<kExact>(&a.u0[0][off + v * fs]) * a.a, a.uk[0][off + v * fs] * a.b);
         r = op_add(t, L * (a.c * dt));
       }
       nv[v] = r;
@@ -473,7 +504,7 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
         csq[v] = csq[v] + d * d;
         csq[4 + v] = csq[4 + v] + fst[v] * fst[v];
       }
-      st_cell<kExact>(&a.out[v][off], r);
+      st_cell<kExact>(&a.out[0][off + v * fs], r);
     }
     if (a.mode == MODE_RESIDUAL) continue;
     // precision form: the next stage recovers the primitives of this state
@@ -486,16 +517,16 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
                             : cons_to_prim_fast(nv[0], nv[1], nv[2], nv[3], a.gamma, a.gamma - 1.0, 1, &cs);
     if (kExact) {
       st_cell<kExact>(&a.prim_out[0][off], q.rho);
-      st_cell<kExact>(&a.prim_out[1][off], q.u);
-      st_cell<kExact>(&a.prim_out[2][off], q.v);
-      st_cell<kExact>(&a.prim_out[3][off], q.p);
+      st_cell<kExact>(&a.prim_out[0][off + fs], q.u);
+      st_cell<kExact>(&a.prim_out[0][off + 2 * fs], q.v);
+      st_cell<kExact>(&a.prim_out[0][off + 3 * fs], q.p);
     }
     if (kExact && OUTFLOW_COL && gi == ni && a.bc[0] == BC_OUTFLOW) {
       Prim<double> g = outflow_ghost(q, a.gamma, a.bc_pressure);
       a.prim_out[0][off + 1] = g.rho;
-      a.prim_out[1][off + 1] = g.u;
-      a.prim_out[2][off + 1] = g.v;
-      a.prim_out[3][off + 1] = g.p;
+      a.prim_out[0][off + fs + 1] = g.u;
+      a.prim_out[0][off + 2 * fs + 1] = g.v;
+      a.prim_out[0][off + 3 * fs + 1] = g.p;
     }
     if (q.rho != q.rho || q.u != q.u || q.v != q.v || q.p != q.p) err |= 1;
     if (a.last_stage) {
