@@ -328,14 +328,23 @@ __device__ inline void recon_mlp(R qm3, R qm, R q0, R q1, R q2, R q3, R a0p, R a
 // ---------------------------------------------------------------------------
 template <typename R> struct EdgeState {
   R rL, uL, vL, pL, rR, uR, vR, pR;
-  R nx, ny;  // unit normal of the edge (direction codes 2 / 3 only)
+  R nx, ny;  // unit normal of the edge (direction codes 2 to 4)
+  int ax;    // index direction of the edge, 0 (i-edge) or 1 (j-edge) (direction code 4)
 };
 
 // Edge direction codes of the flux templates: 0 / 1, the i- / j-edge of a
 // rectilinear grid (normal (1,0) / (0,1), folded at compile time); 2 / 3, the
-// same edges of a general quadrilateral grid, normal (s.nx, s.ny).  The
-// direction-dependent choices of the reference (AUSMPW+'s normal velocity and
-// get_f cells, the sensor stencils) follow the index direction DIR & 1.
+// same edges with the normal (s.nx, s.ny) of a general quadrilateral grid; 4,
+// either edge, its index direction s.ax and normal (s.nx, s.ny) given at run
+// time, so that i- and j-edges in one wave run one code path (the normal
+// (1,0) or (0,1) of a rectilinear grid times u or v is u or v exactly, or the
+// same signed zero / NaN the folded form gives).  The direction-dependent
+// choices of the reference (AUSMPW+'s normal velocity and get_f cells, the
+// sensor stencils) follow the index direction.
+template <typename R, int DIR> __device__ inline bool edge_ax0(const EdgeState<R> &s) {
+  if constexpr (DIR == 4) return s.ax == 0;
+  else return (DIR & 1) == 0;
+}
 template <typename R, int DIR> __device__ inline R edge_nx(const EdgeState<R> &s) {
   if constexpr (DIR >= 2) return s.nx;
   else return DIR == 0 ? R(1) : R(0);
@@ -414,7 +423,7 @@ __device__ inline void flux_slau(R gamma, const EdgeState<R> &s, const PressNbr<
   if constexpr (SD) {
     // slau_solver.fypp:410-448, cell pressures in the edge's (normal, transverse) frame
     R d[6];
-    if constexpr ((DIR & 1) == 0) {
+    if (edge_ax0<R, DIR>(s)) {
       d[0] = absd(pn.p0p - pn.p00);   // p(i,j+1)   - p(i,j)
       d[1] = absd(pn.pm10 - pn.p00);  // p(i-1,j)   - p(i,j)
       d[2] = absd(pn.p0m - pn.p00);   // p(i,j-1)   - p(i,j)
@@ -493,8 +502,8 @@ __device__ inline void flux_ausmpw(R gamma, int low_mach_fix, const EdgeState<R>
   const R LOW_MACH = R(0.1);  // globals.f90:32
   R rho_L = s.rL, u_LHS = s.uL, v_LHS = s.vL, p_L = s.pL;
   R rho_R = s.rR, u_RHS = s.uR, v_RHS = s.vR, p_R = s.pR;
-  R U_L = (DIR & 1) == 0 ? u_LHS : v_LHS, U_R = (DIR & 1) == 0 ? u_RHS : v_RHS;
-  R V_L = (DIR & 1) == 0 ? v_LHS : u_LHS, V_R = (DIR & 1) == 0 ? v_RHS : u_RHS;
+  R U_L = edge_ax0<R, DIR>(s) ? u_LHS : v_LHS, U_R = edge_ax0<R, DIR>(s) ? u_RHS : v_RHS;
+  R V_L = edge_ax0<R, DIR>(s) ? v_LHS : u_LHS, V_R = edge_ax0<R, DIR>(s) ? v_RHS : u_RHS;
   R cs_L = sqrtr(gamma * p_L / rho_L);
   R cs_R = sqrtr(gamma * p_R / rho_R);
   R M_LHS = sqrtr(sq(u_LHS) + sq(v_LHS)) / cs_L;
@@ -524,8 +533,8 @@ __device__ inline void flux_ausmpw(R gamma, int low_mach_fix, const EdgeState<R>
   R w = R(1) - mt * mt * mt;
   // get_f neighbours (ausmpw_plus_solver.fypp:433-441): i-dir p1=(0,-1),(1,-1) p2=(0,+1),(1,+1);
   // j-dir p1=(i+1,j),(i+1,j+1) p2=(i-1,j),(i-1,j+1), i.e. transverse +1 then -1.
-  R p1L = (DIR & 1) == 0 ? pn.p0m : pn.p0p, p1R = (DIR & 1) == 0 ? pn.p1m : pn.p1p;
-  R p2L = (DIR & 1) == 0 ? pn.p0p : pn.p0m, p2R = (DIR & 1) == 0 ? pn.p1p : pn.p1m;
+  R p1L = edge_ax0<R, DIR>(s) ? pn.p0m : pn.p0p, p1R = edge_ax0<R, DIR>(s) ? pn.p1m : pn.p1p;
+  R p2L = edge_ax0<R, DIR>(s) ? pn.p0p : pn.p0m, p2R = edge_ax0<R, DIR>(s) ? pn.p1p : pn.p1m;
   R f_L, f_R;
   if (absd(p_s) < Lim<R>::tiny) {
     f_L = R(0);
@@ -604,7 +613,7 @@ __device__ inline void flux_roe(R gamma, const EdgeState<R> &s, const PressNbr<R
   out[3] = R(0.5) * (FR3 + FL3 - rm3);
   // f_p (roe_solver.fypp:496-507); f_h is symmetric in its two pressures.
   R fp;
-  if constexpr ((DIR & 1) == 0) {
+  if (edge_ax0<R, DIR>(s)) {
     fp = roe_f_h(pn.p00, pn.p10);            // (i,j)   (i+1,j)
     fp = fmax2(fp, roe_f_h(pn.p00, pn.p0p)); // (i,j)   (i,j+1)
     fp = fmax2(fp, roe_f_h(pn.p00, pn.p0m)); // (i,j)   (i,j-1)
@@ -753,8 +762,8 @@ __device__ inline void flux_m_ausmpw(R gamma, int low_mach_fix, const EdgeState<
   const R FEPS = R(Lim<double>::eps * 1e3);
   R rho_L = s.rL, u_LHS = s.uL, v_LHS = s.vL, p_L = s.pL;
   R rho_R = s.rR, u_RHS = s.uR, v_RHS = s.vR, p_R = s.pR;
-  R U_L = (DIR & 1) == 0 ? u_LHS : v_LHS, U_R = (DIR & 1) == 0 ? u_RHS : v_RHS;
-  R V_L = (DIR & 1) == 0 ? v_LHS : u_LHS, V_R = (DIR & 1) == 0 ? v_RHS : u_RHS;
+  R U_L = edge_ax0<R, DIR>(s) ? u_LHS : v_LHS, U_R = edge_ax0<R, DIR>(s) ? u_RHS : v_RHS;
+  R V_L = edge_ax0<R, DIR>(s) ? v_LHS : u_LHS, V_R = edge_ax0<R, DIR>(s) ? v_RHS : u_RHS;
   R cs_L = sqrtr(gamma * p_L / rho_L);
   R cs_R = sqrtr(gamma * p_R / rho_R);
   R M_LHS = sqrtr(sq(u_LHS) + sq(v_LHS)) / cs_L;
@@ -782,8 +791,8 @@ __device__ inline void flux_m_ausmpw(R gamma, int low_mach_fix, const EdgeState<
   R mt = fmin2((p_L / p_R), (p_R / p_L));
   R w = R(1) - (mt * mt * mt);
   // get_f neighbours as AUSMPW+: i-dir p1 = (0,-1),(1,-1), p2 = (0,+1),(1,+1); j-dir the transverse +1 then -1
-  R p1L = (DIR & 1) == 0 ? pn.p0m : pn.p0p, p1R = (DIR & 1) == 0 ? pn.p1m : pn.p1p;
-  R p2L = (DIR & 1) == 0 ? pn.p0p : pn.p0m, p2R = (DIR & 1) == 0 ? pn.p1p : pn.p1m;
+  R p1L = edge_ax0<R, DIR>(s) ? pn.p0m : pn.p0p, p1R = edge_ax0<R, DIR>(s) ? pn.p1m : pn.p1p;
+  R p2L = edge_ax0<R, DIR>(s) ? pn.p0p : pn.p0m, p2R = edge_ax0<R, DIR>(s) ? pn.p1p : pn.p1m;
   R f_L, f_R;
   if (absd(p_s) < Lim<R>::tiny) {
     f_L = R(0);

@@ -308,23 +308,24 @@ __device__ inline void edge_pass(const StageArgs &a, TileSmem<LIMC, R, TX, TY> &
       pn.p1p = pc[sn + st];
       pn.p1m = pc[sn - st];
     }
+    // one flux code path for i- and j-edges (direction code 4): a wave holding
+    // both kinds runs it once instead of once per kind
+    s.ax = idir ? 0 : 1;
     if constexpr (GEOM) {
       const size_t gc = (size_t)(ci + a.nh - 1) + (size_t)(cj + a.nh - 1) * (size_t)a.quad_ncx;
       const size_t n = (size_t)a.quad_n;
       s.nx = (R)a.quad[(idir ? Q_NRX : Q_NTX) * n + gc];
       s.ny = (R)a.quad[(idir ? Q_NRY : Q_NTY) * n + gc];
-    }
-    constexpr int DG = GEOM ? 2 : 0;
-    R f[4];
-    if (idir) {
-      edge_flux<R, FLUX, DG, DF>(gamma, a.low_mach_fix, s, pn, f, &sb);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) sfi[v][ey][ex] = f[v];
     } else {
-      edge_flux<R, FLUX, DG + 1, DF>(gamma, a.low_mach_fix, s, pn, f, &sb);
-#pragma unroll
-      for (int v = 0; v < 4; ++v) sfj[v][ey][ex] = f[v];
+      s.nx = idir ? R(1) : R(0);
+      s.ny = idir ? R(0) : R(1);
     }
+    R f[4];
+    edge_flux<R, FLUX, 4, DF>(gamma, a.low_mach_fix, s, pn, f, &sb);
+    R *dst = idir ? &sfi[0][ey][ex] : &sfj[0][ey][ex];
+    const int fstride = idir ? TY * (TX + 1) : (TY + 1) * TX;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) dst[v * fstride] = f[v];
   }
 }
 
