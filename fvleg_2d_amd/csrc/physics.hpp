@@ -36,13 +36,14 @@ template <> struct Lim<float> {
 };
 
 // Arithmetic type of the FP32-flux precision form: IEEE binary32 except that
-// division and square root use the hardware reciprocal and square root
-// (v_rcp_f32 plus one Newton correction, v_sqrt_f32: about 1 ulp) instead of
-// the correctly rounded sequences, whose scale steps dominate the cost of the
-// reconstruction and flux code, and that a product feeding a sum may be fused
-// into one FMA (the form's results are judged by their error against the
-// FP64 path, not bit for bit).  Only explicit conversions exist, so an
-// unintended double or int operand fails to compile.
+// a / b is a times the hardware reciprocal of b (v_rcp_f32, 1 ulp; the
+// quotient within 1.5 ulp, and the IEEE result for zero, infinite and NaN
+// operands) and the square root is v_sqrt_f32, instead of the correctly
+// rounded sequences, whose scale and fix-up steps dominate the cost of the
+// reconstruction and flux code; and a product feeding a sum may be fused into
+// one FMA (the form's results are judged by their error against the FP64 path,
+// not bit for bit).  Only explicit conversions exist, so an unintended double
+// or int operand fails to compile.
 struct f32x {
   float v;
   f32x() = default;
@@ -65,14 +66,7 @@ __device__ inline f32x operator*(f32x a, f32x b) {
   return f32x(a.v * b.v);
 }
 __device__ inline f32x operator-(f32x a) { return f32x(-a.v); }
-__device__ inline f32x operator/(f32x a, f32x b) {
-  const float r = __builtin_amdgcn_rcpf(b.v);
-  const float q = a.v * r;
-  const float e = __builtin_fmaf(-b.v, q, a.v);
-  // v_div_fixup_f32 passes the corrected quotient through (with the sign of
-  // a/b) and substitutes the IEEE result when a or b is 0, inf or NaN
-  return f32x(__builtin_amdgcn_div_fixupf(__builtin_fmaf(e, r, q), b.v, a.v));
-}
+__device__ inline f32x operator/(f32x a, f32x b) { return f32x(a.v * __builtin_amdgcn_rcpf(b.v)); }
 __device__ inline bool operator<(f32x a, f32x b) { return a.v < b.v; }
 __device__ inline bool operator>(f32x a, f32x b) { return a.v > b.v; }
 __device__ inline bool operator<=(f32x a, f32x b) { return a.v <= b.v; }
