@@ -488,16 +488,19 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
       }
       // time_derivative's energy source (mod_fluid.f90:500-509): a plain sum
       if (v == 3 && a.src_E) L = L + a.src_E[off];
+      // the field operators' sum (its epsilon flush only shapes rounding at 1e-16,
+      // which the precision form leaves out as in its flux balance)
+      auto sum = [](double x, double y) { return kExact ? op_add(x, y) : x + y; };
       double r;
       if (a.mode == MODE_RESIDUAL) {
         r = L;
       } else if (a.mode == MODE_FIRST) {
-        r = op_add(a.uk[0][off + v * fs], L * (a.c * dt));
+        r = sum(a.uk[0][off + v * fs], L * (a.c * dt));
       } else if (a.mode == MODE_BASE) {
-        r = op_add(ld_cell<kExact>(&a.u0[0][off + v * fs]), L * (dt * a.c));
+        r = sum(ld_cell<kExact>(&a.u0[0][off + v * fs]), L * (dt * a.c));
       } else {
-        double t = op_add(ld_cell<kExact>(&a.u0[0][off + v * fs]) * a.a, a.uk[0][off + v * fs] * a.b);
-        r = op_add(t, L * (a.c * dt));
+        double t = sum(ld_cell<kExact>(&a.u0[0][off + v * fs]) * a.a, a.uk[0][off + v * fs] * a.b);
+        r = sum(t, L * (a.c * dt));
       }
       nv[v] = r;
       if (conv) {
