@@ -74,6 +74,7 @@ def measure_batched(n_solves=256, seed=2024, side_range=(64, 512), max_time=0.2,
     if check:
         identical = all(np.array_equal(a, r) for br, pr in zip(b["results"], plain_res) for a, r in zip(br, pr))
     cell_updates = float(sum(pb.ni * pb.nj * int(n) for pb, n in zip(pbs, b["iterations"])))
+    mix_rate, mix_rates = single_solve_mix_rate(pbs, b["iterations"], tile)
     return {
         "solves": n_solves,
         "cells_total": cells,
@@ -84,6 +85,10 @@ def measure_batched(n_solves=256, seed=2024, side_range=(64, 512), max_time=0.2,
         "plain_total_s": plain_s,
         "speedup": plain_s / b["total_s"],
         "batched_cell_updates_per_s": cell_updates / b["run_s"],
+        # the single-solve kernel on the same scheme mix at 1024^2 (weighted by the batch's own updates)
+        "single_solve_cell_updates_per_s_same_mix_1024": mix_rate,
+        "batched_over_single_solve_rate": cell_updates / b["run_s"] / mix_rate,
+        "single_solve_rates_1024": mix_rates,
         "bit_identical_to_plain": identical,
         "errors": int(np.count_nonzero(b["errors"])),
         "tile": tile,
@@ -94,6 +99,31 @@ def measure_batched(n_solves=256, seed=2024, side_range=(64, 512), max_time=0.2,
         "plain_fixed_cost_share": 1.0 - parts["run_s"] / plain_s,
         "speedup_ceiling_without_fixed_costs": plain_s / parts["run_s"],
     }
+
+
+def single_solve_mix_rate(pbs, iterations, tile, side=1024, steps=60, warmup=10):
+    """Cell-updates/s of the single-solve kernel on the batch's scheme mix at a large size: each flux x
+    limiter pair of the batch timed on one side x side Riemann solve (FP64, the batch's tile), combined
+    with the weights of the batch's own cell updates per pair (time = sum of updates / rate)."""
+    weights = {}
+    for pb, n in zip(pbs, iterations):
+        key = (pb.cfg.flux, pb.cfg.limiter)
+        weights[key] = weights.get(key, 0.0) + float(pb.ni * pb.nj * int(n))
+    rates = {}
+    for flux, lim in sorted(weights):
+        pb = riemann2d(n=side, seed=3, flux=flux, limiter=lim)
+        pb.cfg.tile = tile
+        with Solver(pb.cfg, pb.node_x, pb.node_y) as s:
+            s.set_primitive(pb.rho, pb.u, pb.v, pb.p)
+            s.advance(warmup)
+            s.synchronize()
+            t0 = time.perf_counter()
+            s.advance(steps)
+            s.synchronize()
+            rates[f"{flux}/{lim}"] = side * side * steps / (time.perf_counter() - t0)
+    total = sum(weights.values())
+    t = sum(w / rates[f"{f}/{l}"] for (f, l), w in weights.items())
+    return total / t, rates
 
 
 def serving_requests(n=16, seed=4242, side_range=(128, 2048), t_range=(0.1, 0.3), precision="fp32_flux"):
