@@ -464,8 +464,11 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
       vol = dxv * dyv;
     }
     uint64_t vb = (uint64_t)__double_as_longlong(vol);
-    bool pow2 = (vb & 0x000FFFFFFFFFFFFFull) == 0 && ((vb >> 52) & 0x7FF) > 1 && ((vb >> 52) & 0x7FF) < 2046;
-    double inv_vol = kExact ? (pow2 ? 1.0 / vol : 0.0) : rcp64(vol);
+    const uint64_t ve = (vb >> 52) & 0x7FF;
+    bool pow2 = (vb & 0x000FFFFFFFFFFFFFull) == 0 && ve > 1 && ve < 2046;
+    // 1 / 2^k = 2^-k exactly: the sign of vol with the biased exponent 2046 - ve
+    const double inv_pow2 = __longlong_as_double((long long)((vb & 0x8000000000000000ull) | ((2046 - ve) << 52)));
+    double inv_vol = kExact ? (pow2 ? inv_pow2 : 0.0) : rcp64(vol);
     size_t off = (size_t)(gj - 1) * a.pitch + (size_t)(gi - 1);
     const size_t fs = (size_t)a.fstride;
     double fst[4] = {0.0, 0.0, 0.0, 0.0};  // issued before the flux balance to hide their latency
