@@ -84,6 +84,7 @@ SIGNATURES = {
     "cato_batch_destroy": (C.c_int, [_CTX]),
     "cato_batch_set_primitive": (C.c_int, [_CTX, C.c_int, _DP, _DP, _DP, _DP, C.c_int, C.c_int]),
     "cato_batch_get_conserved": (C.c_int, [_CTX, C.c_int, _DP, _DP, _DP, _DP, C.c_int, C.c_int]),
+    "cato_batch_set_pressure_input": (C.c_int, [_CTX, C.c_int, _DP, _DP, _DP, C.c_int]),
     "cato_batch_run": (C.c_int, [_CTX, _DP, _DP, C.c_longlong, C.c_int, C.POINTER(C.c_longlong), _DP,
                                  C.POINTER(C.c_int)]),
     "cato_device_bytes": (C.c_int, [C.POINTER(CatoConfig), C.POINTER(CatoGeom), C.POINTER(C.c_ulonglong)]),

@@ -58,6 +58,19 @@ class Batch:
         N.check(self.lib.cato_batch_set_primitive(self.h, k, *[_p(a) for a in arrs], ni, 0),
                 "cato_batch_set_primitive")
 
+    def set_pressure_input(self, k, table):
+        """Time-tabled +x pressure input of solve k: table = (time, pressure, density) rows, already scaled;
+        None returns to its config's constant bc_pressure / bc_density."""
+        if table is None:
+            N.check(self.lib.cato_batch_set_pressure_input(self.h, k, None, None, None, 0),
+                    "cato_batch_set_pressure_input")
+            return
+        t, p, r = (np.ascontiguousarray(a, dtype=np.float64) for a in table)
+        if not (len(t) == len(p) == len(r)):
+            raise ValueError("time, pressure and density need the same length")
+        N.check(self.lib.cato_batch_set_pressure_input(self.h, k, _p(t), _p(p), _p(r), len(t)),
+                "cato_batch_set_pressure_input")
+
     def conserved(self, k):
         ni, nj = self.shapes[k]
         out = [np.zeros((ni, nj), order="F") for _ in range(4)]

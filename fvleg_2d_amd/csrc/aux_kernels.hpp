@@ -463,10 +463,9 @@ template <typename Get> __device__ inline double column_max(int n, Get &&get) {
 // themselves (src = the same set) or, RECOVER, of the state src = U_k
 // recovered as the precision form's stages recover it.  One workgroup.
 template <bool RECOVER>
-static __global__ __launch_bounds__(1024) void k_pressure_column(FieldGeom g, PressureInput a, const double *s0,
-                                                                 const double *s1, const double *s2,
-                                                                 const double *s3, double *rho, double *u, double *v,
-                                                                 double *p) {
+__device__ inline void pressure_column(const FieldGeom &g, const PressureInput &a, const double *s0,
+                                       const double *s1, const double *s2, const double *s3, double *rho, double *u,
+                                       double *v, double *p) {
   if (a.loop && a.loop->done) return;
   auto cell = [&](int j) {
     size_t k = (size_t)j * g.pitch + (size_t)(g.ni - 1);
@@ -489,6 +488,38 @@ static __global__ __launch_bounds__(1024) void k_pressure_column(FieldGeom g, Pr
     v[k] = q.v;
     p[k] = q.p;
   }
+}
+
+template <bool RECOVER>
+static __global__ __launch_bounds__(1024) void k_pressure_column(FieldGeom g, PressureInput a, const double *s0,
+                                                                 const double *s1, const double *s2,
+                                                                 const double *s3, double *rho, double *u, double *v,
+                                                                 double *p) {
+  pressure_column<RECOVER>(g, a, s0, s1, s2, s3, rho, u, v, p);
+}
+
+// The pass of one stage for several solves of a batch or a server (one
+// workgroup each): where it reads (the stage's primitives, or its U_k to
+// recover) and the primitive set whose column ni+1 it writes, four fields
+// `fs` elements apart.
+struct PressurePass {
+  FieldGeom g;
+  PressureInput pa;
+  const double *src;
+  double *dst;
+  long long fs;
+  int recover;
+};
+
+static __global__ __launch_bounds__(1024) void k_pressure_columns(const PressurePass *passes) {
+  const PressurePass &q = passes[blockIdx.x];
+  const size_t fs = (size_t)q.fs;
+  if (q.recover)
+    pressure_column<true>(q.g, q.pa, q.src, q.src + fs, q.src + 2 * fs, q.src + 3 * fs, q.dst, q.dst + fs,
+                          q.dst + 2 * fs, q.dst + 3 * fs);
+  else
+    pressure_column<false>(q.g, q.pa, q.src, q.src + fs, q.src + 2 * fs, q.src + 3 * fs, q.dst, q.dst + fs,
+                           q.dst + 2 * fs, q.dst + 3 * fs);
 }
 
 // The same on halo'd arrays (leading dimension ld) after the other sides are

@@ -39,10 +39,9 @@ int BatchEngine::init(int n, const cato_config *cfgs, const cato_geom *geoms, in
     c.precision = precision;
     c.tile = tile;
     if (c.integrator != integ) return fail(CATO_ERR_UNSUPPORTED, "all solves of a batch share one integrator");
-    if (c.bc_plus_x == BC_PRESSURE_INPUT)
-      return fail(CATO_ERR_UNSUPPORTED, "pressure_input boundaries run in single-solve contexts only");
     int rc = solves[k].init(c, geoms[k]);
     if (rc) return rc;
+    if (c.bc_plus_x == BC_PRESSURE_INPUT) pin.push_back(k);
   }
   // one allocation: per-solve regions, then shared tables
   offsets.resize(n);
@@ -68,6 +67,8 @@ int BatchEngine::init(int n, const cato_config *cfgs, const cato_geom *geoms, in
   off += 256;
   cursor_off = off;
   off += align256((size_t)nstages * kNumXcd * sizeof(int));
+  pass_off = off;
+  off += align256((size_t)(nstages + 1) * pin.size() * sizeof(PressurePass));
   total_bytes = off;
   BCK(hipMalloc(&mem, total_bytes));
   for (int k = 0; k < n; ++k) {
@@ -121,6 +122,33 @@ void BatchEngine::release() {
   graph = nullptr;
   if (mem) (void)hipFree(mem);
   mem = nullptr;
+  for (auto &sv : solves) {
+    if (sv.pi_tab) (void)hipFree(sv.pi_tab);
+    sv.pi_tab = nullptr;
+    sv.pi_n = 0;
+  }
+}
+
+// pressure_input_bc_t's table of solve k (times, pressures, densities; n = 0: the constant drive).
+int BatchEngine::set_pressure_input(int k, const double *time, const double *pressure, const double *density,
+                                    int n) {
+  Solve &sv = solves[k];
+  if (sv.cfg.bc_plus_x != BC_PRESSURE_INPUT) return fail(CATO_ERR_INVALID, "no pressure_input boundary at +x");
+  int rc = check_pressure_table(time, pressure, density, n);
+  if (rc) return rc;
+  BCK(hipStreamSynchronize(stream));
+  if (sv.pi_tab) (void)hipFree(sv.pi_tab);
+  sv.pi_tab = nullptr;
+  sv.pi_n = 0;
+  if (n == 0) return CATO_OK;
+  std::vector<double> tab((size_t)3 * n);
+  std::copy(time, time + n, tab.begin());
+  std::copy(pressure, pressure + n, tab.begin() + n);
+  std::copy(density, density + n, tab.begin() + 2 * n);
+  BCK(hipMalloc((void **)&sv.pi_tab, tab.size() * sizeof(double)));
+  sv.pi_n = n;
+  BCK(hipMemcpyAsync(sv.pi_tab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice, stream));
+  return hip_check(hipStreamSynchronize(stream), "batch set_pressure_input");
 }
 
 // Stage arguments of every solve: table[stage][solve]; slot `nstages` holds the
@@ -145,15 +173,34 @@ int BatchEngine::upload_args() {
     tab[(size_t)nstages * n + k] = first[0];
   }
   BCK(hipMemcpyAsync(mem + args_off, tab.data(), tab.size() * sizeof(StageArgs), hipMemcpyHostToDevice, stream));
-  return hip_check(hipStreamSynchronize(stream), "batch args");  // tab is pageable and local
+  // the pressure-input passes before each stage, from the same stage arguments
+  const size_t np = pin.size();
+  std::vector<PressurePass> passes((size_t)(nstages + 1) * np);
+  for (size_t q = 0; q < np; ++q) {
+    const int k = pin[q];
+    int rc = check_pressure_constants(solves[k]);
+    if (rc) return rc;
+    for (int slot = 0; slot <= nstages; ++slot)
+      passes[(size_t)slot * np + q] = stage_pressure_pass(solves[k], tab[(size_t)slot * n + k], slot == nstages ? 0 : slot);
+  }
+  if (np)
+    BCK(hipMemcpyAsync(mem + pass_off, passes.data(), passes.size() * sizeof(PressurePass), hipMemcpyHostToDevice,
+                       stream));
+  return hip_check(hipStreamSynchronize(stream), "batch args");  // tab and passes are pageable and local
 }
 
 int BatchEngine::launch_step(bool first) {
   const int n = (int)solves.size();
   const StageArgs *args = reinterpret_cast<const StageArgs *>(mem + args_off);
   for (int s = 0; s < nstages; ++s) {
+    const int slot = (first && s == 0) ? nstages : s;
+    if (!pin.empty()) {
+      hipLaunchKernelGGL(k_pressure_columns, dim3((unsigned)pin.size()), dim3(1024), 0, stream,
+                         reinterpret_cast<const PressurePass *>(mem + pass_off) + (size_t)slot * pin.size());
+      BCK(hipGetLastError());
+    }
     BatchStageArgs b;
-    b.args = (first && s == 0) ? args + (size_t)nstages * n : args + (size_t)s * n;
+    b.args = args + (size_t)slot * n;
     b.variant = reinterpret_cast<const int *>(mem + variant_off);
     b.items = reinterpret_cast<const BatchItem *>(mem + items_off);
     b.count = reinterpret_cast<const int *>(mem + count_off);
@@ -332,6 +379,14 @@ int cato_batch_set_primitive(cato_batch *b, int k, const double *rho, const doub
   s->prims_valid = true;
   s->prims_stale = false;
   return hip_check(hipStreamSynchronize(b->stream), "batch set_primitive");
+}
+
+int cato_batch_set_pressure_input(cato_batch *b, int k, const double *time, const double *pressure,
+                                  const double *density, int n) {
+  Solve *s;
+  int rc = batch_solve(b, k, &s);
+  if (rc) return rc;
+  return b->e.set_pressure_input(k, time, pressure, density, n);
 }
 
 int cato_batch_get_conserved(cato_batch *b, int k, double *rho, double *rho_u, double *rho_v, double *rho_E, int ld,

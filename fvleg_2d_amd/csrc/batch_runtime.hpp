@@ -17,6 +17,9 @@ struct BatchEngine {
   char *mem = nullptr;
   size_t total_bytes = 0;
   size_t loops_off = 0, args_off = 0, variant_off = 0, items_off = 0, refs_off = 0, count_off = 0, cursor_off = 0;
+  // +x pressure input: the solves that have one, and their column passes [stage slot][solve of `pin`]
+  std::vector<int> pin;
+  size_t pass_off = 0;
   std::vector<BatchItem> live_items;  // host copy of the current work list
   std::vector<int> order;             // solves ordered by kernel variant
   int live_solves = 0, live_count = 0;
@@ -30,6 +33,7 @@ struct BatchEngine {
   int upload_args();
   int launch_step(bool first);
   int compact(const std::vector<LoopState> &H);
+  int set_pressure_input(int k, const double *time, const double *pressure, const double *density, int n);
   int run(const double *max_time, const double *contour_dt, long long max_iterations, int use_graph,
           long long *iters, double *times, int *errors);
 };

@@ -689,6 +689,45 @@ PressureInput pressure_args(const cato_ctx *c, int mode, int m, const LoopState 
   return a;
 }
 
+}  // namespace
+
+namespace cato {
+
+// The pressure-input pass of stage m of a step for a solve that its device-side
+// loop advances from fluid time 0 (batched and served solves): the pass of
+// launch_stages with the clocks of a fresh context.
+PressurePass stage_pressure_pass(const Solve &s, const StageArgs &a, int m) {
+  PressurePass q{};
+  q.g = s.fgeom();
+  PressureInput &p = q.pa;
+  p.tab = s.pi_n ? s.pi_tab : nullptr;
+  p.n = s.pi_n;
+  p.p_const = s.cfg.bc_pressure;
+  p.rho_const = s.cfg.bc_density;
+  p.mode = 1;
+  const bool multistage = s.cfg.integrator != CATO_RK_SSP2 && s.cfg.integrator != CATO_RK_SSP3;
+  p.first_multistage = (multistage && m == 0) ? 1 : 0;
+  p.t_at = 0.0;
+  p.dt_at = 0.0;
+  p.t0 = 0.0;
+  p.loop = a.loop;
+  q.recover = (s.cfg.precision != CATO_PRECISION_FP64 && a.mode != MODE_RESIDUAL) ? 1 : 0;
+  q.src = q.recover ? a.uk[0] : a.prim[0];
+  q.dst = const_cast<double *>(a.prim[0]);
+  q.fs = a.fstride;
+  return q;
+}
+
+// A pressure-input table (initialize_1d / check_inputs, linear_interpolation_module.f90:451-478,
+// 1743-1810): n = 0 (the constants) or n >= 2 strictly increasing times.
+int check_pressure_table(const double *time, const double *pressure, const double *density, int n) {
+  if (n != 0 && (!time || !pressure || !density)) return fail(CATO_ERR_INVALID, "null table");
+  if (n == 1 || n < 0) return fail(CATO_ERR_INVALID, "cannot use linear interpolation for only one point");
+  for (int k = 1; k < n; ++k)
+    if (!(time[k] > time[k - 1])) return fail(CATO_ERR_INVALID, "pressure input times are not strictly increasing");
+  return CATO_OK;
+}
+
 // The constant inputs are checked where the reference checks them: when a
 // boundary fill uses them (pressure_input_bc.f90:204-206, 227-229).
 int check_pressure_constants(const Solve &s) {
@@ -699,6 +738,10 @@ int check_pressure_constants(const Solve &s) {
     return fail(CATO_ERR_INVALID, "Error in pressure_input_bc_t%get_desired_density: desired_density < tiny(1.0_rk)");
   return CATO_OK;
 }
+
+}  // namespace cato
+
+namespace {
 
 // Ghost column ni+1 of the primitives a stage reads, for a +x pressure input:
 // from those primitives (FP64 form, residual) or recovered from the stage's
@@ -960,11 +1003,7 @@ int cato_set_pressure_input(cato_ctx *c, const double *time, const double *press
   if (rc) return rc;
   Solve &s = c->s;
   if (s.cfg.bc_plus_x != BC_PRESSURE_INPUT) return fail(CATO_ERR_INVALID, "no pressure_input boundary at +x");
-  if (n != 0 && (!time || !pressure || !density)) return fail(CATO_ERR_INVALID, "null table");
-  // initialize_1d / check_inputs (linear_interpolation_module.f90:451-478, 1743-1810)
-  if (n == 1 || n < 0) return fail(CATO_ERR_INVALID, "cannot use linear interpolation for only one point");
-  for (int k = 1; k < n; ++k)
-    if (!(time[k] > time[k - 1])) return fail(CATO_ERR_INVALID, "pressure input times are not strictly increasing");
+  if ((rc = check_pressure_table(time, pressure, density, n))) return rc;
   HIPCK(hipStreamSynchronize(c->stream));
   drop_steps_graph(c);  // the pressure-input passes carry the table
   if (s.pi_tab) (void)hipFree(s.pi_tab);

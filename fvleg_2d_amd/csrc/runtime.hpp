@@ -81,6 +81,11 @@ struct Solve {
 // copy, repacked on the device through `staging` (a free device field of the
 // same solve) when the device pitch differs; other layouts use a 2D copy.
 int derive_primitives(hipStream_t st, const Solve &s);
+// +x pressure input of batched and served solves: the pass before stage m of a step, and the checks of a
+// table and of the constant drive
+PressurePass stage_pressure_pass(const Solve &s, const StageArgs &a, int m);
+int check_pressure_table(const double *time, const double *pressure, const double *density, int n);
+int check_pressure_constants(const Solve &s);
 // Column ni+1 of a primitive set for a +x outflow side (no-op otherwise).
 int fill_outflow_column(hipStream_t st, const Solve &s, double *const *P);
 // Precision form: P[0] from U[cur] when stages have run since it was last set.

@@ -96,7 +96,8 @@ struct cato_server {
   char *arena_mem = nullptr;
   char *table_mem = nullptr;
   size_t table_bytes = 0, loops_off = 0, args_off = 0, variant_off = 0, refs_off = 0, items_off = 0, count_off = 0,
-         cursor_off = 0;
+         cursor_off = 0, pass_off = 0;
+  int npin = 0;  // resident solves with a +x pressure input (their column passes: [stage slot][npin])
   std::vector<BatchItem> items;  // host copy of the work list (kept until the next tick)
   int item_count = 0;
   int max_items = 0, resident_items = 0;
@@ -144,6 +145,8 @@ int server_tables(cato_server *S) {
   off += 256;
   S->cursor_off = off;
   off += a((size_t)S->nstages * kNumXcd * sizeof(int));
+  S->pass_off = off;
+  off += a((size_t)(S->nstages + 1) * n * sizeof(PressurePass));
   S->table_bytes = off;
   int rc = hip_check(hipMalloc(&S->table_mem, S->table_bytes), "server tables");
   if (rc) return rc;
@@ -202,8 +205,22 @@ int server_upload(cato_server *S, int &nitems, std::vector<int> &active_ids) {
   nitems = (int)items.size();
   S->item_count = nitems;
   if (nitems > S->max_items) return fail(CATO_ERR_INVALID, "internal: server work list overflow");
+  // +x pressure-input passes of the resident solves that have one, before every stage
+  std::vector<int> pin;
+  for (int k : active_ids)
+    if (S->slots[k].s->cfg.bc_plus_x == BC_PRESSURE_INPUT) pin.push_back(k);
+  S->npin = (int)pin.size();
+  std::vector<PressurePass> passes((size_t)(S->nstages + 1) * S->npin);
+  for (int q = 0; q < S->npin; ++q)
+    for (int slot = 0; slot <= S->nstages; ++slot)
+      passes[(size_t)slot * S->npin + q] =
+          stage_pressure_pass(*S->slots[pin[q]].s, tab[(size_t)slot * n + pin[q]], slot == S->nstages ? 0 : slot);
   hipStream_t st = S->stream;
   int rc;
+  if (S->npin && (rc = hip_check(hipMemcpyAsync(S->table_mem + S->pass_off, passes.data(),
+                                                passes.size() * sizeof(PressurePass), hipMemcpyHostToDevice, st),
+                                 "passes")))
+    return rc;
   if ((rc = hip_check(hipMemcpyAsync(S->table_mem + S->args_off, tab.data(), tab.size() * sizeof(StageArgs),
                                      hipMemcpyHostToDevice, st), "args")))
     return rc;
@@ -342,6 +359,9 @@ int enqueue_tick(cato_server *S, bool first_fresh, int grid, int resident) {
   for (int step = 0; step < S->steps_per_tick; ++step) {
     for (int s = 0; s < S->nstages; ++s) {
       const int slot = (step == 0 && s == 0 && first_fresh) ? S->nstages : s;
+      if (S->npin > 0)
+        hipLaunchKernelGGL(k_pressure_columns, dim3((unsigned)S->npin), dim3(1024), 0, S->stream,
+                           reinterpret_cast<const PressurePass *>(S->table_mem + S->pass_off) + (size_t)slot * S->npin);
       if (S->nitems > 0) {
         BatchStageArgs b;
         b.args = args + (size_t)slot * n;
@@ -432,8 +452,12 @@ int cato_server_submit(cato_server *S, const cato_request *req, double arrival_t
   int nh = req->cfg.n_ghost_layers;
   if (g.ni <= 0 || g.nj <= 0 || nh <= 0 || !g.node_x || !g.node_y || !req->rho || !req->u || !req->v || !req->p)
     return fail(CATO_ERR_INVALID, "incomplete request");
-  if (req->cfg.bc_plus_x == BC_PRESSURE_INPUT)
-    return fail(CATO_ERR_UNSUPPORTED, "pressure_input boundaries run in single-solve contexts only");
+  if (req->cfg.bc_plus_x == BC_PRESSURE_INPUT) {  // the constant drive (a request carries no table)
+    Solve probe;
+    probe.cfg = req->cfg;
+    int rc = check_pressure_constants(probe);
+    if (rc) return rc;
+  }
   if (S->integrator < 0) {
     S->integrator = req->cfg.integrator;
     Solve probe;

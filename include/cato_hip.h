@@ -273,6 +273,11 @@ int cato_batch_set_primitive(cato_batch *b, int k, const double *rho, const doub
                              const double *p, int ld, int halo);
 int cato_batch_get_conserved(cato_batch *b, int k, double *rho, double *rho_u, double *rho_v, double *rho_E, int ld,
                              int halo);
+/* Time-tabled +x pressure input of solve k (as cato_set_pressure_input, pressure_input_bc.f90:93-178); a
+ * solve with CATO_BC_PRESSURE_INPUT and no table uses its config's constant bc_pressure / bc_density.  Its
+ * column pass runs before every stage of the batch (the served form takes the constant drive). */
+int cato_batch_set_pressure_input(cato_batch *b, int k, const double *time, const double *pressure,
+                                  const double *density, int n);
 /* Run every solve to its own max_time (main.f90 semantics per solve); steps are
  * replayed from a captured hipGraph.  iterations/times/errors may be NULL. */
 int cato_batch_run(cato_batch *b, const double *max_time, const double *contour_interval_dt, long long max_iterations,
@@ -287,7 +292,8 @@ int cato_device_bytes(const cato_config *cfg, const cato_geom *geom, unsigned lo
  * No counterpart in the reference (one solve per program run there).  A
  * server owns one device arena of hbm_budget_bytes, allocated at create; the
  * request loop itself never allocates device memory.  All requests of a
- * server share its precision and the integrator of the first request. */
+ * server share its precision and the integrator of the first request; a
+ * request with CATO_BC_PRESSURE_INPUT takes its config's constant drive. */
 typedef struct cato_server cato_server;
 typedef struct cato_request {
   cato_config cfg;

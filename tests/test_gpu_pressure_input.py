@@ -241,5 +241,89 @@ def test_errors():
 
     from fvleg_2d_amd.batch import Batch
 
-    with pytest.raises(CatoError, match="single-solve"):
-        Batch([SimpleNamespace(cfg=cfg, node_x=nx, node_y=ny)])
+    with Batch([SimpleNamespace(cfg=cfg, node_x=nx, node_y=ny)]) as bt:  # constants left at 0
+        bt.set_primitive(0, *prims)
+        with pytest.raises(CatoError, match="desired_pressure"):
+            bt.run([0.01])
+        with pytest.raises(CatoError, match="strictly increasing"):
+            bt.set_pressure_input(0, ([0.0, 0.0], [1.0, 1.0], [1.0, 1.0]))
+
+
+def drive_problem(ni, nj, seed, flux, limiter, bc_pressure, bc_density, y, precision):
+    from types import SimpleNamespace
+
+    cfg = SchemeConfig(flux=flux, limiter=limiter, integrator="ssp_rk3", bcs=("pressure_input", "zero_gradient", y, y),
+                       cfl=0.4, bc_pressure=bc_pressure, bc_density=bc_density, precision=precision)
+    nx, ny = uniform_nodes(ni, nj, cfg.n_ghost)
+    rho, u, v, p = drive_state(ni, nj, seed)
+    return SimpleNamespace(cfg=cfg, node_x=nx, node_y=ny, rho=rho, u=u, v=v, p=p)
+
+
+def drive_mix(precision):
+    """Three driven solves (a table, two constant drives; sub- and supersonic rows in and out) and one
+    Riemann solve without a pressure input, all SSP-RK3."""
+    import dataclasses
+
+    from fvleg_2d_amd.problems import riemann2d
+
+    plain = riemann2d(n=40, seed=2, flux="Roe", limiter="minmod", precision=precision)
+    plain = dataclasses.replace(plain, max_time=0.1)
+    return [drive_problem(36, 20, 9, "SLAU", "van_leer", 1.5, 1.1, "symmetry", precision),
+            drive_problem(30, 24, 3, "HLLC", "MLP5", 0.8, 0.7, "periodic", precision),
+            plain,
+            drive_problem(28, 18, 5, "AUSMPW+", "minmod", 1.2, 0.9, "zero_gradient", precision)]
+
+
+def alone(pb, table, max_time):
+    with Solver(pb.cfg, pb.node_x, pb.node_y) as s:
+        if table is not None:
+            s.set_pressure_input(table)
+        s.set_primitive(pb.rho, pb.u, pb.v, pb.p)
+        n, t, e = s.run(max_time, 0, max_time)
+        return s.conserved(), n, t, e
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_flux"])
+def test_batched_pressure_input_equals_solving_alone(precision):
+    """The batched form's column passes (one workgroup per driven solve before every stage, in the captured
+    step graph) give each solve the single-solve result bit for bit, a time table included."""
+    from fvleg_2d_amd.batch import Batch
+
+    say("batched pressure_input", precision)
+    pbs = drive_mix(precision)
+    tables = {0: TABLE}
+    max_time = 0.1
+    with Batch(pbs, precision=precision) as b:
+        for k, pb in enumerate(pbs):
+            b.set_primitive(k, pb.rho, pb.u, pb.v, pb.p)
+        b.set_pressure_input(0, TABLE)
+        it, tt, er = b.run([max_time] * len(pbs), [max_time] * len(pbs))
+        got = [b.conserved(k) for k in range(len(pbs))]
+    for k, pb in enumerate(pbs):
+        ref, n, t, e = alone(pb, tables.get(k), max_time)
+        say("  solve", k, "steps", it[k], n)
+        assert er[k] == e == 0 and it[k] == n > 20 and tt[k] == t
+        for v in range(4):
+            assert np.array_equal(got[k][v], ref[v]), (k, v)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_flux"])
+def test_served_pressure_input_equals_solving_alone(precision):
+    """Requests with a constant +x drive through the server, admitted while others run: each result is the
+    single-solve result bit for bit."""
+    from fvleg_2d_amd.serve import Server
+
+    say("served pressure_input", precision)
+    pbs = drive_mix(precision)[1:]  # the constant drives and the plain solve
+    max_time = 0.08
+    with Server(2e9, max_slots=4, precision=precision, steps_per_tick=8) as srv:
+        ids = [srv.submit(pbs[0], max_time=max_time)]
+        srv.tick()
+        ids += [srv.submit(pb, max_time=max_time) for pb in pbs[1:]]
+        srv.drain()
+        got = [(srv.poll(i), srv.result(i)) for i in ids]
+    for pb, ((done, _, n, e), res) in zip(pbs, got):
+        ref, n1, _, e1 = alone(pb, None, max_time)
+        assert done and e == e1 == 0 and n == n1 > 10
+        for v in range(4):
+            assert np.array_equal(res[v], ref[v]), v
