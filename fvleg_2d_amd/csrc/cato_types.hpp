@@ -125,7 +125,7 @@ struct StageArgs {
   // condition; layout in halo_store_index
   int halo_mask;
   const double *halo;
-  // General quadrilateral grid (null: rectilinear, dxi..lyj): G_COUNT per-cell arrays of quad_cell
+  // General quadrilateral grid (null: rectilinear, dxi..lyj): Q_COUNT per-cell arrays of quad_cell
   // geometry over the halo'd cells, quad_ncx = ni + 2 nh per row, quad_n = quad_ncx (nj + 2 nh) per array
   const double *quad;
   int quad_ncx;

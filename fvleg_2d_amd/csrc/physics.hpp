@@ -316,9 +316,9 @@ __device__ inline void recon_mlp(R qm3, R qm, R q0, R q1, R q2, R q3, R a0p, R a
 }
 
 // ---------------------------------------------------------------------------
-// Edge fluxes.  DIR 0: i-edge (normal (1,0), the cell's RIGHT edge);
-// DIR 1: j-edge (normal (0,1), TOP edge).  Rectilinear grids give exactly these
-// normals from quad_cell.f90:193-229.
+// Edge fluxes of the cell's RIGHT edge (i-edge) and TOP edge (j-edge), whose
+// normals quad_cell.f90:193-229 gives as exactly (1,0) and (0,1) on rectilinear
+// grids (direction codes below).
 // ---------------------------------------------------------------------------
 template <typename R> struct EdgeState {
   R rL, uL, vL, pL, rR, uR, vR, pR;

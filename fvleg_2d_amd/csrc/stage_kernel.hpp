@@ -17,7 +17,9 @@
 // writes on a kernel whose time is spent in FP64 division, not in HBM
 // traffic.  Precision form (R = f32x): the balance is the other way round, so
 // step 1 reads U_k and recovers the primitives of tile and halo itself, and
-// step 4 stores none.  The geometry of rectilinear grids is two 1D arrays.
+// step 4 stores none.  The geometry of rectilinear grids is four 1D arrays;
+// general quadrilateral grids (GEOM) read per-cell lengths, volumes and edge
+// normals.
 #pragma once
 
 #include "physics.hpp"
