@@ -24,6 +24,7 @@
 
 #include "physics.hpp"
 
+
 namespace cato {
 
 constexpr int kThreads = 256;
@@ -235,13 +236,39 @@ __device__ inline void recon_var(const R *base, int sn, int st, R &L, R &Rv) {
   }
 }
 
-// LDS image of one tile: primitives with halo, i-edge and j-edge fluxes.
+// FP64 tiles of one cell per thread split the edge pass into rounds of one edge
+// direction (split_edge_rounds), whose flux takes the direction folded at
+// compile time.  (The precision form keeps one loop over all edges: it fuses
+// products into FMAs where the compiler finds them, which can round one formula
+// differently in two code copies, and a tile-boundary edge, which both
+// neighbouring tiles compute, must round alike in both.)  With an MLP5
+// reconstruction the FP64 edge pass also keeps each cell's bottom j-edge flux in
+// the registers of the thread that updates the cell (hold_j_flux): the j-flux
+// array then lives in the primitives' LDS once the edge pass has read them,
+// which takes the tile's LDS from 33 KB to 25 KB and lets five workgroups share
+// a CU instead of four; the other reconstructions keep the j-flux array (their
+// extra barrier costs more than the occupancy gains).
+template <typename R, int TX, int TY> constexpr bool split_edge_rounds() {
+  return std::is_same<R, double>::value && TX * TY == kThreads;
+}
+template <int LIMC, typename R, int TX, int TY> constexpr bool hold_j_flux() {
+  return split_edge_rounds<R, TX, TY>() && LIMC == LIM_MLP5;
+}
+
+// LDS image of one tile: primitives with halo, i-edge and j-edge fluxes.  With
+// hold_j_flux the j-edge fluxes below the tile's cells (sgj) overlay the
+// primitives after the edge pass, and sfj holds only the row above the tile.
 template <int LIMC, typename R, int TX, int TY> struct TileSmem {
   static constexpr int H = halo_of<LIMC>();
   static constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
-  R sp[4][RY][RX];
+  static constexpr bool kHoldJ = hold_j_flux<LIMC, R, TX, TY>();
+  static constexpr int kTopRow = kHoldJ ? 0 : TY;  // row of sfj holding the j-edges above the tile
+  union {
+    R sp[4][RY][RX];
+    R sgj[4][TY][TX];
+  };
   R sfi[4][TY][TX + 1];
-  R sfj[4][TY + 1][TX];
+  R sfj[4][kTopRow + 1][TX];
 };
 
 // Block-wide scratch for the CFL / sanity / convergence reductions and the
@@ -253,81 +280,134 @@ struct RedSmem {
   double s[kThreads / 64][8];
 };
 
-// Phase 2 of a stage tile: every i- and j-edge of the tile once, reconstructed
-// from the LDS primitives and its flux written to LDS.  D / DF: the division
-// and square root of the FP64 reconstruction / flux (IeeeOps, or FastOps on
-// in-range tiles).  GEOM: general quadrilateral cells, each edge's normal read
-// from the per-cell geometry (the right edge of its left cell for an i-edge, the
-// top edge for a j-edge).
-template <int FLUX, int LIMC, typename R, int TX, int TY, class D, class DF, bool GEOM>
-__device__ inline void edge_pass(const StageArgs &a, TileSmem<LIMC, R, TX, TY> &sm, int i0, int j0) {
+// Edge (ex, ey) of a tile, an i-edge (idir) or a j-edge: its L/R
+// reconstruction from the LDS primitives and its flux.  i-edge (ex, ey) lies
+// between tile cells (ex - 1, ey) and (ex, ey), j-edge (ex, ey) between
+// (ex, ey - 1) and (ex, ey).  Returns false for an edge outside the domain.
+// KIND 0 / 1: every edge of the caller is an i- / j-edge (the flux's direction
+// folded at compile time); -1: either, read from idir (direction code 4).
+// D / DF: the division and square root of the FP64 reconstruction / flux
+// (IeeeOps, or FastOps on in-range tiles).  GEOM: general quadrilateral cells,
+// the normal read from the per-cell geometry (the right edge of its left cell
+// for an i-edge, the top edge for a j-edge).
+template <int FLUX, int LIMC, typename R, int TX, int TY, class D, class DF, bool GEOM, int KIND>
+__device__ __forceinline__ bool tile_edge(const StageArgs &a, TileSmem<LIMC, R, TX, TY> &sm, int i0, int j0,
+                                          bool idir, int ex, int ey, R (&f)[4]) {
   constexpr int H = halo_of<LIMC>();
   constexpr int RX = TX + 2 * H;
-  constexpr int NIE = (TX + 1) * TY, NJE = TX * (TY + 1);
+  if constexpr (KIND >= 0) idir = KIND == 0;
   auto &sp = sm.sp;
-  auto &sfi = sm.sfi;
-  auto &sfj = sm.sfj;
-  const int ni = a.ni, nj = a.nj;
-  const R gamma = (R)a.gamma;
-  for (int it = threadIdx.x; it < NIE + NJE; it += kThreads) {
-    const bool idir = it < NIE;
-    int ex, ey;
-    if (idir) {
-      ey = it / (TX + 1);
-      ex = it - ey * (TX + 1);
-    } else {
-      int k = it - NIE;
-      ey = k / TX;
-      ex = k - ey * TX;
-    }
-    // edge between cell (ci, cj) and its +normal neighbour
-    int ci = idir ? i0 - 1 + ex : i0 + ex;
-    int cj = idir ? j0 + ey : j0 - 1 + ey;
-    if (ci > ni || cj > nj) continue;
-    int lx = ci - (i0 - H), ly = cj - (j0 - H);
-    const int sn = idir ? 1 : RX, st = idir ? RX : 1;
-    EdgeState<R> s;
-    recon_var<R, LIMC, D>(&sp[0][ly][lx], sn, st, s.rL, s.rR);
-    recon_var<R, LIMC, D>(&sp[1][ly][lx], sn, st, s.uL, s.uR);
-    recon_var<R, LIMC, D>(&sp[2][ly][lx], sn, st, s.vL, s.vR);
-    recon_var<R, LIMC, D>(&sp[3][ly][lx], sn, st, s.pL, s.pR);
-    EdgeState<R> sb;  // superbee reconstruction of the same edge (M-AUSMPW+ only)
-    if constexpr (needs_superbee_states<FLUX>()) {
-      recon_var<R, LIM_SUPERBEE, D>(&sp[0][ly][lx], sn, st, sb.rL, sb.rR);
-      recon_var<R, LIM_SUPERBEE, D>(&sp[1][ly][lx], sn, st, sb.uL, sb.uR);
-      recon_var<R, LIM_SUPERBEE, D>(&sp[2][ly][lx], sn, st, sb.vL, sb.vR);
-      recon_var<R, LIM_SUPERBEE, D>(&sp[3][ly][lx], sn, st, sb.pL, sb.pR);
-    }
-    PressNbr<R> pn;
-    if constexpr (needs_pressure_nbrs<FLUX>()) {
-      const R *pc = &sp[3][ly][lx];
-      pn.p00 = pc[0];
-      pn.p10 = pc[sn];
-      pn.p20 = pc[2 * sn];
-      pn.pm10 = pc[-sn];
-      pn.p0p = pc[st];
-      pn.p0m = pc[-st];
-      pn.p1p = pc[sn + st];
-      pn.p1m = pc[sn - st];
-    }
-    // one flux code path for i- and j-edges (direction code 4): a wave holding
-    // both kinds runs it once instead of once per kind
-    s.ax = idir ? 0 : 1;
-    if constexpr (GEOM) {
-      const size_t gc = (size_t)(ci + a.nh - 1) + (size_t)(cj + a.nh - 1) * (size_t)a.quad_ncx;
-      const size_t n = (size_t)a.quad_n;
-      s.nx = (R)a.quad[(idir ? Q_NRX : Q_NTX) * n + gc];
-      s.ny = (R)a.quad[(idir ? Q_NRY : Q_NTY) * n + gc];
-    } else {
-      s.nx = idir ? R(1) : R(0);
-      s.ny = idir ? R(0) : R(1);
-    }
+  // edge between cell (ci, cj) and its +normal neighbour
+  int ci = idir ? i0 - 1 + ex : i0 + ex;
+  int cj = idir ? j0 + ey : j0 - 1 + ey;
+  if (ci > a.ni || cj > a.nj) return false;
+  int lx = ci - (i0 - H), ly = cj - (j0 - H);
+  const int sn = idir ? 1 : RX, st = idir ? RX : 1;
+  EdgeState<R> s;
+  recon_var<R, LIMC, D>(&sp[0][ly][lx], sn, st, s.rL, s.rR);
+  recon_var<R, LIMC, D>(&sp[1][ly][lx], sn, st, s.uL, s.uR);
+  recon_var<R, LIMC, D>(&sp[2][ly][lx], sn, st, s.vL, s.vR);
+  recon_var<R, LIMC, D>(&sp[3][ly][lx], sn, st, s.pL, s.pR);
+  EdgeState<R> sb;  // superbee reconstruction of the same edge (M-AUSMPW+ only)
+  if constexpr (needs_superbee_states<FLUX>()) {
+    recon_var<R, LIM_SUPERBEE, D>(&sp[0][ly][lx], sn, st, sb.rL, sb.rR);
+    recon_var<R, LIM_SUPERBEE, D>(&sp[1][ly][lx], sn, st, sb.uL, sb.uR);
+    recon_var<R, LIM_SUPERBEE, D>(&sp[2][ly][lx], sn, st, sb.vL, sb.vR);
+    recon_var<R, LIM_SUPERBEE, D>(&sp[3][ly][lx], sn, st, sb.pL, sb.pR);
+  }
+  PressNbr<R> pn;
+  if constexpr (needs_pressure_nbrs<FLUX>()) {
+    const R *pc = &sp[3][ly][lx];
+    pn.p00 = pc[0];
+    pn.p10 = pc[sn];
+    pn.p20 = pc[2 * sn];
+    pn.pm10 = pc[-sn];
+    pn.p0p = pc[st];
+    pn.p0m = pc[-st];
+    pn.p1p = pc[sn + st];
+    pn.p1m = pc[sn - st];
+  }
+  // direction code: 0 / 1 (2 / 3 with the normals of quadrilateral cells) when
+  // the caller's edges are all of one kind, else 4: one flux code path for i-
+  // and j-edges, so that a wave holding both kinds runs it once
+  constexpr int DIR = KIND < 0 ? 4 : (GEOM ? 2 : 0) + KIND;
+  s.ax = idir ? 0 : 1;
+  if constexpr (GEOM) {
+    const size_t gc = (size_t)(ci + a.nh - 1) + (size_t)(cj + a.nh - 1) * (size_t)a.quad_ncx;
+    const size_t n = (size_t)a.quad_n;
+    s.nx = (R)a.quad[(idir ? Q_NRX : Q_NTX) * n + gc];
+    s.ny = (R)a.quad[(idir ? Q_NRY : Q_NTY) * n + gc];
+  } else {
+    s.nx = idir ? R(1) : R(0);
+    s.ny = idir ? R(0) : R(1);
+  }
+  edge_flux<R, FLUX, DIR, DF>((R)a.gamma, a.low_mach_fix, s, pn, f, &sb);
+  return true;
+}
+
+// Phase 2 of a stage tile: every i- and j-edge of the tile once, its flux
+// written to LDS.  With hold_j_flux (one cell per thread): the left i-edge of
+// each thread's cell, then the i-edges right of the tile and the j-edges above
+// it (the first TY + TX threads), then the j-edge below each thread's cell,
+// which stays in `hold` (zero outside the domain); the first and last rounds
+// take a single edge direction, folded at compile time.  Otherwise the edges
+// in order, i-edges first, with the direction at run time.
+template <int FLUX, int LIMC, typename R, int TX, int TY, class D, class DF, bool GEOM>
+__device__ inline void edge_pass(const StageArgs &a, TileSmem<LIMC, R, TX, TY> &sm, int i0, int j0,
+                                 R (&hold)[4]) {
+  constexpr int NIE = (TX + 1) * TY, NJE = TX * (TY + 1);
+  const int tid = threadIdx.x;
+  using Smem = TileSmem<LIMC, R, TX, TY>;
+  if constexpr (split_edge_rounds<R, TX, TY>()) {
+    const int cy = tid / TX, cx = tid - cy * TX;
     R f[4];
-    edge_flux<R, FLUX, 4, DF>(gamma, a.low_mach_fix, s, pn, f, &sb);
-    R *dst = idir ? &sfi[0][ey][ex] : &sfj[0][ey][ex];
-    const int fstride = idir ? TY * (TX + 1) : (TY + 1) * TX;
+    if (tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, 0>(a, sm, i0, j0, true, cx, cy, f)) {
 #pragma unroll
-    for (int v = 0; v < 4; ++v) dst[v * fstride] = f[v];
+      for (int v = 0; v < 4; ++v) sm.sfi[v][cy][cx] = f[v];
+    }
+    if (tid < TY + TX) {
+      const bool idir = tid < TY;
+      const int ex = idir ? TX : tid - TY, ey = idir ? tid : TY;
+      if (tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, -1>(a, sm, i0, j0, idir, ex, ey, f)) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          if (idir)
+            sm.sfi[v][ey][ex] = f[v];
+          else
+            sm.sfj[v][Smem::kTopRow][ex] = f[v];
+        }
+      }
+    }
+    if constexpr (Smem::kHoldJ) {
+      if (!tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, 1>(a, sm, i0, j0, false, cx, cy, hold)) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) hold[v] = R(0);
+      }
+    } else {
+      if (tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, 1>(a, sm, i0, j0, false, cx, cy, f)) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sm.sfj[v][cy][cx] = f[v];
+      }
+    }
+  } else {
+    for (int it = tid; it < NIE + NJE; it += kThreads) {
+      const bool idir = it < NIE;
+      int ex, ey;
+      if (idir) {
+        ey = it / (TX + 1);
+        ex = it - ey * (TX + 1);
+      } else {
+        int k = it - NIE;
+        ey = k / TX;
+        ex = k - ey * TX;
+      }
+      R f[4];
+      if (!tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, -1>(a, sm, i0, j0, idir, ex, ey, f)) continue;
+      R *dst = idir ? &sm.sfi[0][ey][ex] : &sm.sfj[0][ey][ex];
+      const int fstride = idir ? TY * (TX + 1) : (TY + 1) * TX;
+#pragma unroll
+      for (int v = 0; v < 4; ++v) dst[v * fstride] = f[v];
+    }
   }
 }
 
@@ -416,19 +496,28 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
   __syncthreads();
 
   // ---- 2. edge states and fluxes, each edge once --------------------------
+  constexpr bool kHoldJ = TileSmem<LIMC, R, TX, TY>::kHoldJ;
+  R hold[4];  // hold_j_flux: the flux through the bottom edge of this thread's cell
   if constexpr (kFastScheme) {
     bool fast = a.fast_ops != 0;
 #pragma unroll
     for (int w = 0; w < kThreads / 64; ++w) fast = fast && red.fast[w] != 0;
     using DF = typename std::conditional<fast_ops_flux<FLUX>(), FastOps, IeeeOps>::type;
     if (fast)
-      edge_pass<FLUX, LIMC, R, TX, TY, FastOps, DF, GEOM>(a, sm, i0, j0);
+      edge_pass<FLUX, LIMC, R, TX, TY, FastOps, DF, GEOM>(a, sm, i0, j0, hold);
     else
-      edge_pass<FLUX, LIMC, R, TX, TY, IeeeOps, IeeeOps, GEOM>(a, sm, i0, j0);
+      edge_pass<FLUX, LIMC, R, TX, TY, IeeeOps, IeeeOps, GEOM>(a, sm, i0, j0, hold);
   } else {
-    edge_pass<FLUX, LIMC, R, TX, TY, IeeeOps, IeeeOps, GEOM>(a, sm, i0, j0);
+    edge_pass<FLUX, LIMC, R, TX, TY, IeeeOps, IeeeOps, GEOM>(a, sm, i0, j0, hold);
   }
   __syncthreads();
+  if constexpr (kHoldJ) {
+    // the primitives are dead: their LDS takes the held j-fluxes for the cells below
+    const int cy = tid / TX, cx = tid - cy * TX;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) sm.sgj[v][cy][cx] = hold[v];
+    __syncthreads();
+  }
 
   // ---- 3. divergence + stage combination ---------------------------------
   double lmin = __builtin_inf();
@@ -482,7 +571,14 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       double Fi = (double)sfi[v][cy][cx + 1], Fim = (double)sfi[v][cy][cx];
-      double Gj = (double)sfj[v][cy + 1][cx], Gjm = (double)sfj[v][cy][cx];
+      double Gj, Gjm;
+      if constexpr (kHoldJ) {
+        Gj = cy + 1 < TY ? (double)sm.sgj[v][cy + 1][cx] : (double)sfj[v][0][cx];
+        Gjm = (double)hold[v];
+      } else {
+        Gj = (double)sfj[v][cy + 1][cx];
+        Gjm = (double)sfj[v][cy][cx];
+      }
       double L;
       if constexpr (kExact) {
         L = divergence(Fi, Fim, Gj, Gjm, lb, lr, lt, ll, vol, pow2, inv_vol);
