@@ -236,32 +236,44 @@ __device__ inline void recon_var(const R *base, int sn, int st, R &L, R &Rv) {
   }
 }
 
+// Threads of the single-solve stage kernel for a tile: 256, or one per cell on
+// the 512-cell tiles (32x16, 64x8) in FP64 and in the precision form with MLP5,
+// where that was measured faster (FP64 KH 2048^2 32x16: 348 -> 255 us; Riemann
+// 1024^2 HLLC + MLP5 32x16: 136 -> 112 us, precision form 55.8 -> 53.1 us; the
+// precision form with TVD was 5 % slower).  The batched kernel runs every tile
+// with kThreads.
+template <typename R, int LIMC, int TX, int TY> constexpr int stage_threads() {
+  return (TX * TY == 2 * kThreads && (std::is_same<R, double>::value || LIMC == LIM_MLP5)) ? 2 * kThreads
+                                                                                          : kThreads;
+}
+
 // FP64 tiles of one cell per thread split the edge pass into rounds of one edge
 // direction (split_edge_rounds), whose flux takes the direction folded at
 // compile time.  (The precision form keeps one loop over all edges: it fuses
 // products into FMAs where the compiler finds them, which can round one formula
 // differently in two code copies, and a tile-boundary edge, which both
 // neighbouring tiles compute, must round alike in both.)  With an MLP5
-// reconstruction the FP64 edge pass also keeps each cell's bottom j-edge flux in
-// the registers of the thread that updates the cell (hold_j_flux): the j-flux
-// array then lives in the primitives' LDS once the edge pass has read them,
-// which takes the tile's LDS from 33 KB to 25 KB and lets five workgroups share
-// a CU instead of four; the other reconstructions keep the j-flux array (their
-// extra barrier costs more than the occupancy gains).
-template <typename R, int TX, int TY> constexpr bool split_edge_rounds() {
-  return std::is_same<R, double>::value && TX * TY == kThreads;
+// reconstruction, and on the 512-cell tiles, the FP64 edge pass also keeps each
+// cell's bottom j-edge flux in the registers of the thread that updates the
+// cell (hold_j_flux): the j-flux array then lives in the primitives' LDS once
+// the edge pass has read them, which takes an MLP5 16x16 tile's LDS from 33 KB
+// to 25 KB (five workgroups per CU instead of four) and a TVD 32x16 tile's from
+// 57 KB to 41 KB (three instead of two); the 256-cell TVD tiles keep the j-flux
+// array (their extra barrier costs more than the occupancy gains).
+template <typename R, int TX, int TY, int NT> constexpr bool split_edge_rounds() {
+  return std::is_same<R, double>::value && TX * TY == NT;
 }
-template <int LIMC, typename R, int TX, int TY> constexpr bool hold_j_flux() {
-  return split_edge_rounds<R, TX, TY>() && LIMC == LIM_MLP5;
+template <int LIMC, typename R, int TX, int TY, int NT> constexpr bool hold_j_flux() {
+  return split_edge_rounds<R, TX, TY, NT>() && (LIMC == LIM_MLP5 || NT > kThreads);
 }
 
 // LDS image of one tile: primitives with halo, i-edge and j-edge fluxes.  With
 // hold_j_flux the j-edge fluxes below the tile's cells (sgj) overlay the
 // primitives after the edge pass, and sfj holds only the row above the tile.
-template <int LIMC, typename R, int TX, int TY> struct TileSmem {
+template <int LIMC, typename R, int TX, int TY, int NT = kThreads> struct TileSmem {
   static constexpr int H = halo_of<LIMC>();
   static constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
-  static constexpr bool kHoldJ = hold_j_flux<LIMC, R, TX, TY>();
+  static constexpr bool kHoldJ = hold_j_flux<LIMC, R, TX, TY, NT>();
   static constexpr int kTopRow = kHoldJ ? 0 : TY;  // row of sfj holding the j-edges above the tile
   union {
     R sp[4][RY][RX];
@@ -273,12 +285,13 @@ template <int LIMC, typename R, int TX, int TY> struct TileSmem {
 
 // Block-wide scratch for the CFL / sanity / convergence reductions and the
 // per-wave votes on the FastOps operand ranges.
-struct RedSmem {
-  double m[kThreads / 64];
-  int e[kThreads / 64];
-  int fast[kThreads / 64];
-  double s[kThreads / 64][8];
+template <int NW> struct RedSmemT {
+  double m[NW];
+  int e[NW];
+  int fast[NW];
+  double s[NW][8];
 };
+using RedSmem = RedSmemT<kThreads / 64>;
 
 // Edge (ex, ey) of a tile, an i-edge (idir) or a j-edge: its L/R
 // reconstruction from the LDS primitives and its flux.  i-edge (ex, ey) lies
@@ -290,8 +303,8 @@ struct RedSmem {
 // (IeeeOps, or FastOps on in-range tiles).  GEOM: general quadrilateral cells,
 // the normal read from the per-cell geometry (the right edge of its left cell
 // for an i-edge, the top edge for a j-edge).
-template <int FLUX, int LIMC, typename R, int TX, int TY, class D, class DF, bool GEOM, int KIND>
-__device__ __forceinline__ bool tile_edge(const StageArgs &a, TileSmem<LIMC, R, TX, TY> &sm, int i0, int j0,
+template <int FLUX, int LIMC, typename R, int TX, int TY, class D, class DF, bool GEOM, int KIND, int NT>
+__device__ __forceinline__ bool tile_edge(const StageArgs &a, TileSmem<LIMC, R, TX, TY, NT> &sm, int i0, int j0,
                                           bool idir, int ex, int ey, R (&f)[4]) {
   constexpr int H = halo_of<LIMC>();
   constexpr int RX = TX + 2 * H;
@@ -352,23 +365,23 @@ __device__ __forceinline__ bool tile_edge(const StageArgs &a, TileSmem<LIMC, R, 
 // which stays in `hold` (zero outside the domain); the first and last rounds
 // take a single edge direction, folded at compile time.  Otherwise the edges
 // in order, i-edges first, with the direction at run time.
-template <int FLUX, int LIMC, typename R, int TX, int TY, class D, class DF, bool GEOM>
-__device__ inline void edge_pass(const StageArgs &a, TileSmem<LIMC, R, TX, TY> &sm, int i0, int j0,
+template <int FLUX, int LIMC, typename R, int TX, int TY, class D, class DF, bool GEOM, int NT>
+__device__ inline void edge_pass(const StageArgs &a, TileSmem<LIMC, R, TX, TY, NT> &sm, int i0, int j0,
                                  R (&hold)[4]) {
   constexpr int NIE = (TX + 1) * TY, NJE = TX * (TY + 1);
   const int tid = threadIdx.x;
-  using Smem = TileSmem<LIMC, R, TX, TY>;
-  if constexpr (split_edge_rounds<R, TX, TY>()) {
+  using Smem = TileSmem<LIMC, R, TX, TY, NT>;
+  if constexpr (split_edge_rounds<R, TX, TY, NT>()) {
     const int cy = tid / TX, cx = tid - cy * TX;
     R f[4];
-    if (tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, 0>(a, sm, i0, j0, true, cx, cy, f)) {
+    if (tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, 0, NT>(a, sm, i0, j0, true, cx, cy, f)) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) sm.sfi[v][cy][cx] = f[v];
     }
     if (tid < TY + TX) {
       const bool idir = tid < TY;
       const int ex = idir ? TX : tid - TY, ey = idir ? tid : TY;
-      if (tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, -1>(a, sm, i0, j0, idir, ex, ey, f)) {
+      if (tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, -1, NT>(a, sm, i0, j0, idir, ex, ey, f)) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           if (idir)
@@ -379,18 +392,18 @@ __device__ inline void edge_pass(const StageArgs &a, TileSmem<LIMC, R, TX, TY> &
       }
     }
     if constexpr (Smem::kHoldJ) {
-      if (!tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, 1>(a, sm, i0, j0, false, cx, cy, hold)) {
+      if (!tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, 1, NT>(a, sm, i0, j0, false, cx, cy, hold)) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) hold[v] = R(0);
       }
     } else {
-      if (tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, 1>(a, sm, i0, j0, false, cx, cy, f)) {
+      if (tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, 1, NT>(a, sm, i0, j0, false, cx, cy, f)) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) sm.sfj[v][cy][cx] = f[v];
       }
     }
   } else {
-    for (int it = tid; it < NIE + NJE; it += kThreads) {
+    for (int it = tid; it < NIE + NJE; it += NT) {
       const bool idir = it < NIE;
       int ex, ey;
       if (idir) {
@@ -402,7 +415,7 @@ __device__ inline void edge_pass(const StageArgs &a, TileSmem<LIMC, R, TX, TY> &
         ex = k - ey * TX;
       }
       R f[4];
-      if (!tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, -1>(a, sm, i0, j0, idir, ex, ey, f)) continue;
+      if (!tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, -1, NT>(a, sm, i0, j0, idir, ex, ey, f)) continue;
       R *dst = idir ? &sm.sfi[0][ey][ex] : &sm.sfj[0][ey][ex];
       const int fstride = idir ? TY * (TX + 1) : (TY + 1) * TX;
 #pragma unroll
@@ -416,9 +429,10 @@ __device__ inline void edge_pass(const StageArgs &a, TileSmem<LIMC, R, TX, TY> &
 // also writes the +x outflow ghost column of the new state (the batched
 // kernel); the single-solve launch leaves that to k_outflow_column, since the
 // extra epilogue slows every tile of the fused kernel by ~5%.
-template <int FLUX, int LIMC, typename R, int TX, int TY, bool OUTFLOW_COL = false, bool GEOM = false>
-__device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileSmem<LIMC, R, TX, TY> &sm,
-                                  RedSmem &red) {
+template <int FLUX, int LIMC, typename R, int TX, int TY, bool OUTFLOW_COL = false, bool GEOM = false,
+          int NT = kThreads>
+__device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileSmem<LIMC, R, TX, TY, NT> &sm,
+                                  RedSmemT<NT / 64> &red) {
   constexpr int H = halo_of<LIMC>();
   constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
   // FP64 form: the reference's arithmetic bit for bit; precision form (f32x):
@@ -453,7 +467,7 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
     const bool check = recover || a.check_input;
     const size_t base = (size_t)(j0 - H - 1) * a.pitch + (size_t)(i0 - H - 1);
     const size_t fs = (size_t)a.fstride;
-    for (int idx = tid; idx < RX * RY; idx += kThreads) {
+    for (int idx = tid; idx < RX * RY; idx += NT) {
       int ly = idx / RX, lx = idx - ly * RX;
       const size_t off = base + (size_t)ly * a.pitch + (size_t)lx;
       Prim<double> pr;
@@ -472,7 +486,7 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
       sp[3][ly][lx] = (R)pr.p;
     }
   } else {
-    for (int idx = tid; idx < RX * RY; idx += kThreads) {
+    for (int idx = tid; idx < RX * RY; idx += NT) {
       int ly = idx / RX, lx = idx - ly * RX;
       int gi = i0 - H + lx, gj = j0 - H + ly;
       Prim<double> pr{0.0, 0.0, 0.0, 0.0};
@@ -496,19 +510,19 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
   __syncthreads();
 
   // ---- 2. edge states and fluxes, each edge once --------------------------
-  constexpr bool kHoldJ = TileSmem<LIMC, R, TX, TY>::kHoldJ;
+  constexpr bool kHoldJ = TileSmem<LIMC, R, TX, TY, NT>::kHoldJ;
   R hold[4];  // hold_j_flux: the flux through the bottom edge of this thread's cell
   if constexpr (kFastScheme) {
     bool fast = a.fast_ops != 0;
 #pragma unroll
-    for (int w = 0; w < kThreads / 64; ++w) fast = fast && red.fast[w] != 0;
+    for (int w = 0; w < NT / 64; ++w) fast = fast && red.fast[w] != 0;
     using DF = typename std::conditional<fast_ops_flux<FLUX>(), FastOps, IeeeOps>::type;
     if (fast)
-      edge_pass<FLUX, LIMC, R, TX, TY, FastOps, DF, GEOM>(a, sm, i0, j0, hold);
+      edge_pass<FLUX, LIMC, R, TX, TY, FastOps, DF, GEOM, NT>(a, sm, i0, j0, hold);
     else
-      edge_pass<FLUX, LIMC, R, TX, TY, IeeeOps, IeeeOps, GEOM>(a, sm, i0, j0, hold);
+      edge_pass<FLUX, LIMC, R, TX, TY, IeeeOps, IeeeOps, GEOM, NT>(a, sm, i0, j0, hold);
   } else {
-    edge_pass<FLUX, LIMC, R, TX, TY, IeeeOps, IeeeOps, GEOM>(a, sm, i0, j0, hold);
+    edge_pass<FLUX, LIMC, R, TX, TY, IeeeOps, IeeeOps, GEOM, NT>(a, sm, i0, j0, hold);
   }
   __syncthreads();
   if constexpr (kHoldJ) {
@@ -528,7 +542,7 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
 #pragma unroll
   for (int v = 0; v < 8; ++v) csq[v] = 0.0;
   const double EPSD = Lim<double>::eps;
-  for (int c = tid; c < TX * TY; c += kThreads) {
+  for (int c = tid; c < TX * TY; c += NT) {
     int cy = c / TX, cx = c - cy * TX;
     int gi = i0 + cx, gj = j0 + cy;
     if (gi > ni || gj > nj) continue;
@@ -655,7 +669,7 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
     if (tid < 8) {
       double t = red.s[0][tid];
 #pragma unroll
-      for (int k = 1; k < kThreads / 64; ++k) t = t + red.s[k][tid];
+      for (int k = 1; k < NT / 64; ++k) t = t + red.s[k][tid];
       a.block_conv[(size_t)tid * (size_t)(a.tiles_x * a.tiles_y) + tile] = t;
     }
   }
@@ -673,7 +687,7 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
       double m = red_min[0];
       int e = red_err[0];
 #pragma unroll
-      for (int k = 1; k < kThreads / 64; ++k) {
+      for (int k = 1; k < NT / 64; ++k) {
         m = (red_min[k] < m) ? red_min[k] : m;
         e |= red_err[k];
       }
@@ -685,12 +699,13 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
 
 // Single-solve stage launch: one workgroup per tile, XCD-aware tile order.
 template <int FLUX, int LIMC, typename R, int TX, int TY, bool GEOM = false>
-__global__ __launch_bounds__(kThreads) void stage_kernel(StageArgs a) {
-  __shared__ TileSmem<LIMC, R, TX, TY> sm;
-  __shared__ RedSmem red;
+__global__ __launch_bounds__((stage_threads<R, LIMC, TX, TY>())) void stage_kernel(StageArgs a) {
+  constexpr int NT = stage_threads<R, LIMC, TX, TY>();
+  __shared__ TileSmem<LIMC, R, TX, TY, NT> sm;
+  __shared__ RedSmemT<NT / 64> red;
   if (a.loop && a.loop->done) return;
   const double dt = a.loop ? a.loop->dt : a.dt_value;
-  stage_tile<FLUX, LIMC, R, TX, TY, false, GEOM>(a, xcd_tile(blockIdx.x, gridDim.x), dt, sm, red);
+  stage_tile<FLUX, LIMC, R, TX, TY, false, GEOM, NT>(a, xcd_tile(blockIdx.x, gridDim.x), dt, sm, red);
 }
 
 }  // namespace cato
