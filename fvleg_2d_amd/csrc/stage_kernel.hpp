@@ -247,9 +247,12 @@ template <typename R, int LIMC, int TX, int TY> constexpr int stage_threads() {
                                                                                           : kThreads;
 }
 
-// FP64 tiles of one cell per thread split the edge pass into rounds of one edge
-// direction (split_edge_rounds), whose flux takes the direction folded at
-// compile time.  (The precision form keeps one loop over all edges: it fuses
+// In the single-solve kernel, FP64 tiles of one cell per thread split the edge
+// pass into rounds of one edge direction (split_edge_rounds, the SPLIT layout),
+// whose flux takes the direction folded at compile time.  (The batched kernel,
+// whose body holds every scheme, keeps one loop: the three code copies per
+// scheme made its 256-solve workload 6 % slower.  The precision form keeps one
+// loop over all edges too: it fuses
 // products into FMAs where the compiler finds them, which can round one formula
 // differently in two code copies, and a tile-boundary edge, which both
 // neighbouring tiles compute, must round alike in both.)  With an MLP5
@@ -263,17 +266,18 @@ template <typename R, int LIMC, int TX, int TY> constexpr int stage_threads() {
 template <typename R, int TX, int TY, int NT> constexpr bool split_edge_rounds() {
   return std::is_same<R, double>::value && TX * TY == NT;
 }
-template <int LIMC, typename R, int TX, int TY, int NT> constexpr bool hold_j_flux() {
-  return split_edge_rounds<R, TX, TY, NT>() && (LIMC == LIM_MLP5 || NT > kThreads);
+template <int LIMC, int NT, bool SPLIT> constexpr bool hold_j_flux() {
+  return SPLIT && (LIMC == LIM_MLP5 || NT > kThreads);
 }
 
 // LDS image of one tile: primitives with halo, i-edge and j-edge fluxes.  With
 // hold_j_flux the j-edge fluxes below the tile's cells (sgj) overlay the
 // primitives after the edge pass, and sfj holds only the row above the tile.
-template <int LIMC, typename R, int TX, int TY, int NT = kThreads> struct TileSmem {
+template <int LIMC, typename R, int TX, int TY, int NT = kThreads, bool SPLIT = false> struct TileSmem {
   static constexpr int H = halo_of<LIMC>();
   static constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
-  static constexpr bool kHoldJ = hold_j_flux<LIMC, R, TX, TY, NT>();
+  static constexpr bool kSplit = SPLIT;
+  static constexpr bool kHoldJ = hold_j_flux<LIMC, NT, SPLIT>();
   static constexpr int kTopRow = kHoldJ ? 0 : TY;  // row of sfj holding the j-edges above the tile
   union {
     R sp[4][RY][RX];
@@ -303,8 +307,8 @@ using RedSmem = RedSmemT<kThreads / 64>;
 // (IeeeOps, or FastOps on in-range tiles).  GEOM: general quadrilateral cells,
 // the normal read from the per-cell geometry (the right edge of its left cell
 // for an i-edge, the top edge for a j-edge).
-template <int FLUX, int LIMC, typename R, int TX, int TY, class D, class DF, bool GEOM, int KIND, int NT>
-__device__ __forceinline__ bool tile_edge(const StageArgs &a, TileSmem<LIMC, R, TX, TY, NT> &sm, int i0, int j0,
+template <int FLUX, int LIMC, typename R, int TX, int TY, class D, class DF, bool GEOM, int KIND, int NT, bool SPLIT>
+__device__ __forceinline__ bool tile_edge(const StageArgs &a, TileSmem<LIMC, R, TX, TY, NT, SPLIT> &sm, int i0, int j0,
                                           bool idir, int ex, int ey, R (&f)[4]) {
   constexpr int H = halo_of<LIMC>();
   constexpr int RX = TX + 2 * H;
@@ -365,23 +369,23 @@ __device__ __forceinline__ bool tile_edge(const StageArgs &a, TileSmem<LIMC, R, 
 // which stays in `hold` (zero outside the domain); the first and last rounds
 // take a single edge direction, folded at compile time.  Otherwise the edges
 // in order, i-edges first, with the direction at run time.
-template <int FLUX, int LIMC, typename R, int TX, int TY, class D, class DF, bool GEOM, int NT>
-__device__ inline void edge_pass(const StageArgs &a, TileSmem<LIMC, R, TX, TY, NT> &sm, int i0, int j0,
+template <int FLUX, int LIMC, typename R, int TX, int TY, class D, class DF, bool GEOM, int NT, bool SPLIT>
+__device__ inline void edge_pass(const StageArgs &a, TileSmem<LIMC, R, TX, TY, NT, SPLIT> &sm, int i0, int j0,
                                  R (&hold)[4]) {
   constexpr int NIE = (TX + 1) * TY, NJE = TX * (TY + 1);
   const int tid = threadIdx.x;
-  using Smem = TileSmem<LIMC, R, TX, TY, NT>;
-  if constexpr (split_edge_rounds<R, TX, TY, NT>()) {
+  using Smem = TileSmem<LIMC, R, TX, TY, NT, SPLIT>;
+  if constexpr (SPLIT) {
     const int cy = tid / TX, cx = tid - cy * TX;
     R f[4];
-    if (tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, 0, NT>(a, sm, i0, j0, true, cx, cy, f)) {
+    if (tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, 0, NT, SPLIT>(a, sm, i0, j0, true, cx, cy, f)) {
 #pragma unroll
       for (int v = 0; v < 4; ++v) sm.sfi[v][cy][cx] = f[v];
     }
     if (tid < TY + TX) {
       const bool idir = tid < TY;
       const int ex = idir ? TX : tid - TY, ey = idir ? tid : TY;
-      if (tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, -1, NT>(a, sm, i0, j0, idir, ex, ey, f)) {
+      if (tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, -1, NT, SPLIT>(a, sm, i0, j0, idir, ex, ey, f)) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
           if (idir)
@@ -392,12 +396,12 @@ __device__ inline void edge_pass(const StageArgs &a, TileSmem<LIMC, R, TX, TY, N
       }
     }
     if constexpr (Smem::kHoldJ) {
-      if (!tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, 1, NT>(a, sm, i0, j0, false, cx, cy, hold)) {
+      if (!tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, 1, NT, SPLIT>(a, sm, i0, j0, false, cx, cy, hold)) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) hold[v] = R(0);
       }
     } else {
-      if (tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, 1, NT>(a, sm, i0, j0, false, cx, cy, f)) {
+      if (tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, 1, NT, SPLIT>(a, sm, i0, j0, false, cx, cy, f)) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) sm.sfj[v][cy][cx] = f[v];
       }
@@ -415,7 +419,7 @@ __device__ inline void edge_pass(const StageArgs &a, TileSmem<LIMC, R, TX, TY, N
         ex = k - ey * TX;
       }
       R f[4];
-      if (!tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, -1, NT>(a, sm, i0, j0, idir, ex, ey, f)) continue;
+      if (!tile_edge<FLUX, LIMC, R, TX, TY, D, DF, GEOM, -1, NT, SPLIT>(a, sm, i0, j0, idir, ex, ey, f)) continue;
       R *dst = idir ? &sm.sfi[0][ey][ex] : &sm.sfj[0][ey][ex];
       const int fstride = idir ? TY * (TX + 1) : (TY + 1) * TX;
 #pragma unroll
@@ -430,8 +434,8 @@ __device__ inline void edge_pass(const StageArgs &a, TileSmem<LIMC, R, TX, TY, N
 // kernel); the single-solve launch leaves that to k_outflow_column, since the
 // extra epilogue slows every tile of the fused kernel by ~5%.
 template <int FLUX, int LIMC, typename R, int TX, int TY, bool OUTFLOW_COL = false, bool GEOM = false,
-          int NT = kThreads>
-__device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileSmem<LIMC, R, TX, TY, NT> &sm,
+          int NT = kThreads, bool SPLIT = false>
+__device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileSmem<LIMC, R, TX, TY, NT, SPLIT> &sm,
                                   RedSmemT<NT / 64> &red) {
   constexpr int H = halo_of<LIMC>();
   constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
@@ -510,7 +514,7 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
   __syncthreads();
 
   // ---- 2. edge states and fluxes, each edge once --------------------------
-  constexpr bool kHoldJ = TileSmem<LIMC, R, TX, TY, NT>::kHoldJ;
+  constexpr bool kHoldJ = TileSmem<LIMC, R, TX, TY, NT, SPLIT>::kHoldJ;
   R hold[4];  // hold_j_flux: the flux through the bottom edge of this thread's cell
   if constexpr (kFastScheme) {
     bool fast = a.fast_ops != 0;
@@ -518,11 +522,11 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
     for (int w = 0; w < NT / 64; ++w) fast = fast && red.fast[w] != 0;
     using DF = typename std::conditional<fast_ops_flux<FLUX>(), FastOps, IeeeOps>::type;
     if (fast)
-      edge_pass<FLUX, LIMC, R, TX, TY, FastOps, DF, GEOM, NT>(a, sm, i0, j0, hold);
+      edge_pass<FLUX, LIMC, R, TX, TY, FastOps, DF, GEOM, NT, SPLIT>(a, sm, i0, j0, hold);
     else
-      edge_pass<FLUX, LIMC, R, TX, TY, IeeeOps, IeeeOps, GEOM, NT>(a, sm, i0, j0, hold);
+      edge_pass<FLUX, LIMC, R, TX, TY, IeeeOps, IeeeOps, GEOM, NT, SPLIT>(a, sm, i0, j0, hold);
   } else {
-    edge_pass<FLUX, LIMC, R, TX, TY, IeeeOps, IeeeOps, GEOM, NT>(a, sm, i0, j0, hold);
+    edge_pass<FLUX, LIMC, R, TX, TY, IeeeOps, IeeeOps, GEOM, NT, SPLIT>(a, sm, i0, j0, hold);
   }
   __syncthreads();
   if constexpr (kHoldJ) {
@@ -701,11 +705,12 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
 template <int FLUX, int LIMC, typename R, int TX, int TY, bool GEOM = false>
 __global__ __launch_bounds__((stage_threads<R, LIMC, TX, TY>())) void stage_kernel(StageArgs a) {
   constexpr int NT = stage_threads<R, LIMC, TX, TY>();
-  __shared__ TileSmem<LIMC, R, TX, TY, NT> sm;
+  constexpr bool SPLIT = split_edge_rounds<R, TX, TY, NT>();
+  __shared__ TileSmem<LIMC, R, TX, TY, NT, SPLIT> sm;
   __shared__ RedSmemT<NT / 64> red;
   if (a.loop && a.loop->done) return;
   const double dt = a.loop ? a.loop->dt : a.dt_value;
-  stage_tile<FLUX, LIMC, R, TX, TY, false, GEOM, NT>(a, xcd_tile(blockIdx.x, gridDim.x), dt, sm, red);
+  stage_tile<FLUX, LIMC, R, TX, TY, false, GEOM, NT, SPLIT>(a, xcd_tile(blockIdx.x, gridDim.x), dt, sm, red);
 }
 
 }  // namespace cato
