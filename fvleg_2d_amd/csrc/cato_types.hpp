@@ -114,6 +114,12 @@ struct StageArgs {
   const double *src_E;       // energy source added to d(rho E)/dt (null: none; always null in MODE_RESIDUAL)
   const double *dxi, *lxi;   // per halo'd i: x spacing and x-edge length
   const double *dyj, *lyj;   // per halo'd j
+  // Uniform rectilinear grid (every interior cell alike): the cell geometry the stage epilogue
+  // derives from dxi..lyj, computed once on the host with the same operations: dx, dy, the
+  // bottom / right / top / left edge lengths after their equalisation, the volume and, when
+  // the volume is a power of two (ugeom_pow2), its exact reciprocal.  ugeom = 0: per cell
+  int ugeom, ugeom_pow2;
+  double ug[8];
   const LoopState *loop;     // dt source and done flag (may be null -> dt_value)
   double dt_value;
   double *block_min;         // per-block CFL minimum (last stage)

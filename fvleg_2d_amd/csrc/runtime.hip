@@ -138,6 +138,35 @@ static int build_quad_geometry(int ni, int nj, int nh, const double *nx, const d
   return CATO_OK;
 }
 
+// The stage epilogue's cell geometry (stage_kernel.hpp, phase 3) when every
+// interior cell has the same dxi, lxi, dyj and lyj, in the kernel's operations
+// (IEEE double on both sides, so the same bits): edge-length equalisation
+// (quad_cell.f90:110-112) over (l1, l2, l1, l2), volume dx dy, and the exact
+// reciprocal of a power-of-two volume.
+static void uniform_cell_geometry(int ni, int nj, int nh, Geometry1D &g) {
+  g.uniform = false;
+  for (int a = nh; a < nh + ni; ++a)
+    if (g.dxi[a] != g.dxi[nh] || g.lxi[a] != g.lxi[nh]) return;
+  for (int b = nh; b < nh + nj; ++b)
+    if (g.dyj[b] != g.dyj[nh] || g.lyj[b] != g.lyj[nh]) return;
+  const double dx = g.dxi[nh], dy = g.dyj[nh], l1 = g.lxi[nh], l2 = g.lyj[nh];
+  const double mx = (l2 > l1) ? l2 : l1, mn = (l2 < l1) ? l2 : l1;
+  double lb = l1, lr = l2, lt = l1, ll = l2;
+  if (mx - mn < 2.0 * 2.220446049250313080847e-16) lb = lr = lt = ll = mx;
+  const double vol = dx * dy;
+  uint64_t vb;
+  std::memcpy(&vb, &vol, 8);
+  const uint64_t ve = (vb >> 52) & 0x7FF;
+  const bool pow2 = (vb & 0x000FFFFFFFFFFFFFull) == 0 && ve > 1 && ve < 2046;
+  const uint64_t ib = (vb & 0x8000000000000000ull) | ((2046 - ve) << 52);
+  double inv;
+  std::memcpy(&inv, &ib, 8);
+  const double u[8] = {dx, dy, lb, lr, lt, ll, vol, pow2 ? inv : 0.0};
+  for (int k = 0; k < 8; ++k) g.u[k] = u[k];
+  g.u_pow2 = pow2;
+  g.uniform = true;
+}
+
 int build_geometry(int ni, int nj, int nh, const double *nx, const double *ny, Geometry1D &g) {
   const int nnx = ni + 2 * nh + 1, nny = nj + 2 * nh + 1;
   auto NX = [&](int a, int b) { return nx[(size_t)a + (size_t)b * nnx]; };
@@ -184,6 +213,7 @@ int build_geometry(int ni, int nj, int nh, const double *nx, const double *ny, G
         if (vx / len != ex || vy / len != ey) return build_quad_geometry(ni, nj, nh, nx, ny, g);
       }
     }
+  uniform_cell_geometry(ni, nj, nh, g);
   return CATO_OK;
 }
 
@@ -365,6 +395,9 @@ StageArgs Solve::stage_args(int mode, double a, double b, double c, int in_buf, 
   s.lxi = lxi;
   s.dyj = dyj;
   s.lyj = lyj;
+  s.ugeom = geo.uniform ? 1 : 0;
+  s.ugeom_pow2 = geo.u_pow2 ? 1 : 0;
+  for (int k = 0; k < 8; ++k) s.ug[k] = geo.u[k];
   s.loop = lp;
   s.dt_value = dt_value;
   s.block_min = block_min;

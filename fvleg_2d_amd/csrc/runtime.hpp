@@ -23,6 +23,11 @@ struct Geometry1D {
   std::vector<double> dxi, lxi, dyj, lyj;
   bool quad = false;
   std::vector<double> q;
+  // Rectilinear grid whose interior cells all have one dxi, lxi, dyj and lyj:
+  // StageArgs::ug of those cells (uniform_cell_geometry)
+  bool uniform = false;
+  bool u_pow2 = false;
+  double u[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 int build_geometry(int ni, int nj, int nh, const double *nx, const double *ny, Geometry1D &g);
 int default_tile(const cato_config &c, int ni, int nj);

@@ -550,34 +550,48 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
     int cy = c / TX, cx = c - cy * TX;
     int gi = i0 + cx, gj = j0 + cy;
     if (gi > ni || gj > nj) continue;
-    double dxv, dyv, lb, lr, lt, ll, vol;
-    if constexpr (GEOM) {
-      // quad_cell geometry of the cell: min_dx / min_dy, bottom / right / top / left lengths, volume
-      const size_t gc = (size_t)(gi + nh - 1) + (size_t)(gj + nh - 1) * (size_t)a.quad_ncx;
-      const size_t n = (size_t)a.quad_n;
-      dxv = a.quad[Q_MDX * n + gc];
-      dyv = a.quad[Q_MDY * n + gc];
-      lb = a.quad[Q_LEN0 * n + gc];
-      lr = a.quad[Q_LEN1 * n + gc];
-      lt = a.quad[Q_LEN2 * n + gc];
-      ll = a.quad[Q_LEN3 * n + gc];
-      vol = a.quad[Q_VOL * n + gc];
+    double dxv, dyv, lb, lr, lt, ll, vol, inv_vol;
+    bool pow2;
+    if (!GEOM && a.ugeom) {
+      // uniform rectilinear grid: the same values, computed once on the host
+      dxv = a.ug[0];
+      dyv = a.ug[1];
+      lb = a.ug[2];
+      lr = a.ug[3];
+      lt = a.ug[4];
+      ll = a.ug[5];
+      vol = a.ug[6];
+      pow2 = a.ugeom_pow2 != 0;
+      inv_vol = kExact ? a.ug[7] : rcp64(vol);
     } else {
-      dxv = a.dxi[gi + nh - 1];
-      dyv = a.dyj[gj + nh - 1];
-      double l1 = a.lxi[gi + nh - 1], l2 = a.lyj[gj + nh - 1];
-      // edge-length equalisation (quad_cell.f90:110-112) over (l1, l2, l1, l2)
-      double mx = fmax2(l1, l2), mn = fmin2(l1, l2);
-      lb = l1, lr = l2, lt = l1, ll = l2;
-      if (mx - mn < 2.0 * EPSD) lb = lr = lt = ll = mx;
-      vol = dxv * dyv;
+      if constexpr (GEOM) {
+        // quad_cell geometry of the cell: min_dx / min_dy, bottom / right / top / left lengths, volume
+        const size_t gc = (size_t)(gi + nh - 1) + (size_t)(gj + nh - 1) * (size_t)a.quad_ncx;
+        const size_t n = (size_t)a.quad_n;
+        dxv = a.quad[Q_MDX * n + gc];
+        dyv = a.quad[Q_MDY * n + gc];
+        lb = a.quad[Q_LEN0 * n + gc];
+        lr = a.quad[Q_LEN1 * n + gc];
+        lt = a.quad[Q_LEN2 * n + gc];
+        ll = a.quad[Q_LEN3 * n + gc];
+        vol = a.quad[Q_VOL * n + gc];
+      } else {
+        dxv = a.dxi[gi + nh - 1];
+        dyv = a.dyj[gj + nh - 1];
+        double l1 = a.lxi[gi + nh - 1], l2 = a.lyj[gj + nh - 1];
+        // edge-length equalisation (quad_cell.f90:110-112) over (l1, l2, l1, l2)
+        double mx = fmax2(l1, l2), mn = fmin2(l1, l2);
+        lb = l1, lr = l2, lt = l1, ll = l2;
+        if (mx - mn < 2.0 * EPSD) lb = lr = lt = ll = mx;
+        vol = dxv * dyv;
+      }
+      uint64_t vb = (uint64_t)__double_as_longlong(vol);
+      const uint64_t ve = (vb >> 52) & 0x7FF;
+      pow2 = (vb & 0x000FFFFFFFFFFFFFull) == 0 && ve > 1 && ve < 2046;
+      // 1 / 2^k = 2^-k exactly: the sign of vol with the biased exponent 2046 - ve
+      const double inv_pow2 = __longlong_as_double((long long)((vb & 0x8000000000000000ull) | ((2046 - ve) << 52)));
+      inv_vol = kExact ? (pow2 ? inv_pow2 : 0.0) : rcp64(vol);
     }
-    uint64_t vb = (uint64_t)__double_as_longlong(vol);
-    const uint64_t ve = (vb >> 52) & 0x7FF;
-    bool pow2 = (vb & 0x000FFFFFFFFFFFFFull) == 0 && ve > 1 && ve < 2046;
-    // 1 / 2^k = 2^-k exactly: the sign of vol with the biased exponent 2046 - ve
-    const double inv_pow2 = __longlong_as_double((long long)((vb & 0x8000000000000000ull) | ((2046 - ve) << 52)));
-    double inv_vol = kExact ? (pow2 ? inv_pow2 : 0.0) : rcp64(vol);
     size_t off = (size_t)(gj - 1) * a.pitch + (size_t)(gi - 1);
     const size_t fs = (size_t)a.fstride;
     double fst[4] = {0.0, 0.0, 0.0, 0.0};  // issued before the flux balance to hide their latency

@@ -192,6 +192,46 @@ def test_step_parity_ssp_rk3(oracle, flux, lim):
         assert np.array_equal(gp[k], op[k]), ("prim", k)
 
 
+def rectilinear_nodes(xs, ys):
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    return np.asfortranarray(X), np.asfortranarray(Y)
+
+
+# (ni, nj, x nodes, y nodes, every interior cell alike): exact binary spacings take the epilogue's
+# host-computed uniform geometry (power-of-two volume 2^-12; volume 0.5625 with equal edge lengths; dx != dy),
+# a stretched grid the per-cell arrays
+GEOM_GRIDS = {
+    "pow2": (64, 32, lambda nh: (np.arange(-nh, 64 + nh + 1) / 64.0, np.arange(-nh, 32 + nh + 1) / 64.0), True),
+    "square_0.75": (48, 36, lambda nh: (np.arange(-nh, 48 + nh + 1) * 0.75, np.arange(-nh, 36 + nh + 1) * 0.75), True),
+    "dx_ne_dy": (40, 32, lambda nh: (np.arange(-nh, 40 + nh + 1) / 32.0, np.arange(-nh, 32 + nh + 1) * (3.0 / 128.0)),
+                 True),
+    "stretched": (40, 32, lambda nh: (np.cumsum(np.r_[0.0, 0.02 + 0.01 * np.sin(np.arange(40 + 2 * nh))]) - 0.05,
+                                      np.cumsum(np.r_[0.0, 0.03 + 0.005 * np.cos(np.arange(32 + 2 * nh))]) - 0.07),
+                  False),
+}
+
+
+@pytest.mark.parametrize("grid", list(GEOM_GRIDS))
+@pytest.mark.parametrize("flux, lim", [("SLAU", "van_leer"), ("HLLC", "MLP5"), ("AUSMPW+", "minmod")])
+def test_rectilinear_cell_geometry_step_parity(oracle, grid, flux, lim):
+    """SSP-RK3 steps on rectilinear grids whose interior cells are all alike (the stage epilogue takes the
+    cell geometry computed once on the host) and on a stretched grid (per-cell arrays): CFL dt, state and
+    primitives bit for bit the oracle's."""
+    ni, nj, nodes, uniform = GEOM_GRIDS[grid]
+    cfg = SchemeConfig(flux=flux, limiter=lim, integrator="ssp_rk3", bcs=("zero_gradient",) * 4, cfl=0.4)
+    xs, ys = nodes(cfg.n_ghost)
+    nh = cfg.n_ghost
+    dx, dy = np.diff(xs)[nh:nh + ni], np.diff(ys)[nh:nh + nj]
+    assert (np.all(dx == dx[0]) and np.all(dy == dy[0])) == uniform
+    nx, ny = rectilinear_nodes(xs, ys)
+    say("rectilinear geometry", grid, flux, lim)
+    g, o, gp, op = run_steps(oracle, cfg, nx, ny, shock_state(ni, nj, 11), 6)
+    for k in range(4):
+        assert np.array_equal(g[k], o[k]), (k, rel_linf(g[k], o[k]))
+    for k in range(5):
+        assert np.array_equal(gp[k], op[k]), ("prim", k)
+
+
 def fast_ops_edge_states(ni, nj, seed):
     """States around the operand ranges of the FP64 FastOps edge arithmetic (fast_ops_cell: rho and p in
     [2^-B, 2^B], u and v zero or 2^-150 <= |u| <= 2^B; B = 60 for TVD2 with the SLAU family, 20 with MLP or
