@@ -227,6 +227,54 @@ template <int LOG2B> __device__ inline bool fast_ops_cell(const Prim<double> &q)
          (au == 0.0 || au >= VLO) && (av == 0.0 || av >= VLO);
 }
 
+// cons_to_prim with the FastOps division and square root wherever a whole wave
+// keeps them in range, bit for bit cons_to_prim: the four quotients by rho
+// through one reciprocal while rho is in [2^-60, 2^60] and every numerator the
+// field division does divide (|x| >= tiny) lies in [2^-600, 2^600] (quotients
+// then normal, operand exponents at most 660 apart, and no -0 numerator: the
+// tiny flush takes those); the sound speed's root while p is also in
+// [2^-600, 2^600] (its argument then lies in [2^-661, 2^662]).  The Mach
+// filter keeps its IEEE division (rarely taken).  Waves with any lane outside
+// take cons_to_prim's IEEE sequences.
+__device__ inline Prim<double> cons_to_prim_vote(double r, double ru, double rv, double rE, double gamma, double gm1,
+                                                 double *cs_out) {
+  constexpr double RLO = 0x1p-60, RHI = 0x1p60, XLO = 0x1p-600, XHI = 0x1p600;
+  const double TINY = Lim<double>::tiny;
+  auto num_ok = [&](double x) {
+    const double ax = absd(x);
+    return ax < TINY || (ax >= XLO && ax <= XHI);
+  };
+  const bool quot_fast = __all(r >= RLO && r <= RHI && num_ok(ru) && num_ok(rv) && num_ok(rE));
+  double u, v, t2;
+  FastOps::Rcp<double> dr{r, 0.0};
+  if (quot_fast) {
+    dr = FastOps::rcp(r);
+    u = (absd(ru) < TINY) ? 0.0 : FastOps::div(ru, dr);
+    v = (absd(rv) < TINY) ? 0.0 : FastOps::div(rv, dr);
+    t2 = (absd(rE) < TINY) ? 0.0 : FastOps::div(rE, dr);
+  } else {
+    u = op_div(ru, r);
+    v = op_div(rv, r);
+    t2 = op_div(rE, r);
+  }
+  double t1 = r * gm1;
+  double t5 = op_add(u * u, v * v);
+  double t6 = t5 * 0.5;
+  double t7 = op_sub(t2, t6);
+  double p = t1 * t7;
+  double cs;
+  if (quot_fast && __all(p >= XLO && p <= XHI))
+    cs = FastOps::sqrt_pos(FastOps::div(p, dr) * gamma);
+  else
+    cs = sqrtr(op_div(p, r) * gamma);
+  const double MF = 1e-13;
+  const double lim = 2.0 * MF * cs;
+  if (!(absd(v) >= lim) && absd(op_div(v, cs)) < MF) v = 0.0;
+  if (!(absd(u) >= lim) && absd(op_div(u, cs)) < MF) u = 0.0;
+  *cs_out = cs;
+  return Prim<double>{r, u, v, p};
+}
+
 // ---------------------------------------------------------------------------
 // Reconstruction (edge states L/R at the interface between cells 0 and 1).
 // Stencil values along the edge normal: qm3=q(-2), qm=q(-1), q0, q1, q2, q3=q(+3).

@@ -650,7 +650,7 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
     // primitives of the new state for the next stage (assign_fluid ->
     // calculate_derived_quantities, mod_fluid.f90:564-591) and its NaN check
     double cs;
-    Prim<double> q = kExact ? cons_to_prim(nv[0], nv[1], nv[2], nv[3], a.gamma, a.gamma - 1.0, &cs)
+    Prim<double> q = kExact ? cons_to_prim_vote(nv[0], nv[1], nv[2], nv[3], a.gamma, a.gamma - 1.0, &cs)
                             : cons_to_prim_fast(nv[0], nv[1], nv[2], nv[3], a.gamma, a.gamma - 1.0, 1, &cs);
     if (kExact) {
       st_cell<kExact>(&a.prim_out[0][off], q.rho);
