@@ -249,13 +249,12 @@ template <typename R, int LIMC, int TX, int TY> constexpr int stage_threads() {
 
 // In the single-solve kernel, FP64 tiles of one cell per thread split the edge
 // pass into rounds of one edge direction (split_edge_rounds, the SPLIT layout),
-// whose flux takes the direction folded at compile time.  (The batched kernel,
-// whose body holds every scheme, keeps one loop: the three code copies per
-// scheme made its 256-solve workload 6 % slower.  The precision form keeps one
-// loop over all edges too: it fuses
-// products into FMAs where the compiler finds them, which can round one formula
-// differently in two code copies, and a tile-boundary edge, which both
-// neighbouring tiles compute, must round alike in both.)  With an MLP5
+// whose flux takes the direction folded at compile time.  The batched kernel,
+// whose body holds every scheme, keeps one loop: three code copies per scheme
+// made its 256-solve workload 6 % slower.  The precision form keeps one loop
+// too: it lets the compiler fuse products into FMAs, which can round one
+// formula differently in two code copies, and a tile-boundary edge, which both
+// neighbouring tiles compute, must round alike in both.  With an MLP5
 // reconstruction, and on the 512-cell tiles, the FP64 edge pass also keeps each
 // cell's bottom j-edge flux in the registers of the thread that updates the
 // cell (hold_j_flux): the j-flux array then lives in the primitives' LDS once
@@ -599,6 +598,8 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
 #pragma unroll
       for (int v = 0; v < 4; ++v) fst[v] = a.conv_first[0][off + v * fs];
     }
+    // precision form, rectilinear cells (lr == ll, lt == lb): lengths over the volume
+    const double sx = kExact ? 0.0 : lr * inv_vol, sy = kExact ? 0.0 : lb * inv_vol;
     double nv[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -616,8 +617,14 @@ __device__ inline void stage_tile(const StageArgs &a, int tile, double dt, TileS
         L = divergence(Fi, Fim, Gj, Gjm, lb, lr, lt, ll, vol, pow2, inv_vol);
       } else {
         // precision form: the same FP64 flux balance without the reference's
-        // max-normalisation and flushes (they only shape rounding at 1e-16)
-        L = -(((Fi * lr) - (Fim * ll)) + ((Gj * lt) - (Gjm * lb))) * inv_vol;
+        // max-normalisation and flushes (they only shape rounding at 1e-16); on
+        // a rectilinear grid the right and left lengths are equal, as are the
+        // top and bottom ones, so each pair takes one product (the difference of
+        // two binary32 fluxes is exact in FP64)
+        if constexpr (GEOM)
+          L = -(((Fi * lr) - (Fim * ll)) + ((Gj * lt) - (Gjm * lb))) * inv_vol;
+        else
+          L = -(((Fi - Fim) * sx) + ((Gj - Gjm) * sy));
       }
       // time_derivative's energy source (mod_fluid.f90:500-509): a plain sum
       if (v == 3 && a.src_E) L = L + a.src_E[off];
