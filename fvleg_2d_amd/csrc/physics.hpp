@@ -170,6 +170,8 @@ struct IeeeOps {
   template <typename T> static __device__ T div(T a, Rcp<T> r) { return a / r.b; }
   template <typename T> static __device__ T div(T a, T b) { return a / b; }
   template <typename T> static __device__ T sqrt_pos(T x) { return sqrtr(x); }
+  template <typename T> static __device__ T min(T a, T b) { return fmin2(a, b); }
+  template <typename T> static __device__ T max(T a, T b) { return fmax2(a, b); }
 };
 struct FastOps {
   template <typename T> struct Rcp { double b, y; };
@@ -187,6 +189,12 @@ struct FastOps {
     return __builtin_fma(e, r.y, q);
   }
   static __device__ double div(double a, double b) { return div(a, rcp(b)); }
+  // v_min_f64 / v_max_f64 for the flux's min and max: on the finite operands of
+  // an in-range tile the same value as the reference's compare-and-select, at
+  // the sites that use them (no -0 operand meets a +0 one there, or the sign of
+  // that zero is dropped: SLAU's g enters only through 1 - g)
+  static __device__ double min(double a, double b) { return __builtin_fmin(a, b); }
+  static __device__ double max(double a, double b) { return __builtin_fmax(a, b); }
   static __device__ double sqrt_pos(double x) {
     const double r = __builtin_amdgcn_rsq(x);
     double g = x * r, h = r * 0.5;
@@ -441,7 +449,7 @@ __device__ inline void flux_slau(R gamma, const EdgeState<R> &s, const PressNbr<
   };
   R V_n_bar = D::div(rho_L * absd(V_n_plus) + rho_R * absd(V_n_minus), rho_L + rho_R);
   R vel2 = sq(u_L) + sq(v_L) + sq(u_R) + sq(v_R);
-  R M_hat = fmin2(R(1), over_c(sqrtr(R(0.5) * vel2)));
+  R M_hat = D::min(R(1), over_c(sqrtr(R(0.5) * vel2)));
   R Chi = sq(R(1) - M_hat);
   R M_bar = over_c(V_n_bar);
   R p_bar = R(0.5) * (p_L + p_R);
@@ -486,8 +494,8 @@ __device__ inline void flux_slau(R gamma, const EdgeState<R> &s, const PressNbr<
     }
     R mx6 = d[0];
 #pragma unroll
-    for (int k = 1; k < 6; ++k) mx6 = fmax2(mx6, d[k]);
-    R delta_p_max = fmax2(absd(delta_p), mx6);
+    for (int k = 1; k < 6; ++k) mx6 = D::max(mx6, d[k]);
+    R delta_p_max = D::max(absd(delta_p), mx6);
     const auto dp_bar = D::rcp(p_bar);
     theta = fmin2(R(1), sq(D::div((D::div(C_SD1 * absd(delta_p), dp_bar) + C_SD1),
                                   (D::div(delta_p_max, dp_bar) + C_SD1)))) *
@@ -501,7 +509,7 @@ __device__ inline void flux_slau(R gamma, const EdgeState<R> &s, const PressNbr<
   else
     p_tilde = p_bar + (R(0.5) * delta_Beta * p_L_minus_R) +
               sqrtr(R(0.5) * vel2) * (Beta_plus + Beta_minus - R(1)) * rho_bar * c_bar;
-  R g = -(fmax2(fmin2(M_plus, R(0)), R(-1)) * fmin2(fmax2(M_minus, R(0)), R(1)));
+  R g = -(D::max(D::min(M_plus, R(0)), R(-1)) * D::min(D::max(M_minus, R(0)), R(1)));
   R base = R(0.5) * ((rho_L * V_n_plus) + (rho_R * V_n_minus) - absd(V_n_bar) * delta_rho) * (R(1) - g);
   R mass_flux;
   if constexpr (SD) mass_flux = base - ((kRcp ? theta * (R(0.5) * ic) : D::div(theta, R(2) * c_bar)) * delta_p);
@@ -738,7 +746,7 @@ __device__ inline void flux_hllc(R gamma, const EdgeState<R> &s, R out[4]) {
   R rho_bar = R(0.5) * (rL + rR);
   R cs_bar = R(0.5) * (cL + cR);
   R p_pvrs = R(0.5) * (pL + pR) - R(0.5) * (qR - qL) * rho_bar * cs_bar;
-  R p_star = fmax2(R(0), p_pvrs);
+  R p_star = D::max(R(0), p_pvrs);
   R gfac = D::div(gamma + R(1), R(2) * gamma);
   R wL = (p_star <= pL) ? R(1) : D::sqrt_pos(R(1) + gfac * (D::div(p_star, pL) - R(1)));
   R wR = (p_star <= pR) ? R(1) : D::sqrt_pos(R(1) + gfac * (D::div(p_star, pR) - R(1)));
