@@ -14,6 +14,7 @@ struct FieldGeom {
   int ni, nj, nh, pitch;
   int bc[4];
   double gamma, cfl;
+  double gas[3];  // gas_constants(gamma)
   double bc_pressure;
   const double *dxi, *dyj;
   const double *quad;  // per-cell quad_cell geometry (QuadSlot arrays), null on rectilinear grids
@@ -640,19 +641,20 @@ static __global__ void k_edge_flux(FieldGeom g, int ld, const double *__restrict
   pn.p00 = P(0, 0); pn.p10 = P(1, 0); pn.p20 = P(2, 0); pn.pm10 = P(-1, 0);
   pn.p0p = P(0, 1); pn.p0m = P(0, -1); pn.p1p = P(1, 1); pn.p1m = P(1, -1);
   R f[4];
+  const Gas<R> gas = gas_of<R>(g.gamma, g.gas);
   if (g.quad) {  // normal of the right (i-edge) or top (j-edge) edge of cell (i, j)
     const size_t gc = (size_t)(i + g.nh - 1) + (size_t)(j + g.nh - 1) * (size_t)g.quad_ncx;
     const size_t n = (size_t)g.quad_n;
     s.nx = (R)g.quad[(dir ? Q_NTX : Q_NRX) * n + gc];
     s.ny = (R)g.quad[(dir ? Q_NTY : Q_NRY) * n + gc];
     if (!dir)
-      edge_flux<R, FLUX, 2>((R)g.gamma, low_mach_fix, s, pn, f, &sb);
+      edge_flux<R, FLUX, 2>(gas, low_mach_fix, s, pn, f, &sb);
     else
-      edge_flux<R, FLUX, 3>((R)g.gamma, low_mach_fix, s, pn, f, &sb);
+      edge_flux<R, FLUX, 3>(gas, low_mach_fix, s, pn, f, &sb);
   } else if (!dir) {
-    edge_flux<R, FLUX, 0>((R)g.gamma, low_mach_fix, s, pn, f, &sb);
+    edge_flux<R, FLUX, 0>(gas, low_mach_fix, s, pn, f, &sb);
   } else {
-    edge_flux<R, FLUX, 1>((R)g.gamma, low_mach_fix, s, pn, f, &sb);
+    edge_flux<R, FLUX, 1>(gas, low_mach_fix, s, pn, f, &sb);
   }
   double *d = dir ? jflux + 4 * ((size_t)(i - 1) + (size_t)g.ni * j) : iflux + 4 * ((size_t)i + (size_t)(g.ni + 1) * (j - 1));
   for (int v = 0; v < 4; ++v) d[v] = (double)f[v];

@@ -104,6 +104,7 @@ struct StageArgs {
   int bc[4];                 // +x, -x, +y, -y
   int tiles_x, tiles_y;      // tile grid
   double gamma, cfl;
+  double gas[3];             // gas_constants(gamma): the fluxes' gamma / (gamma - 1), ...
   double bc_pressure;        // outflow ambient pressure (BC_OUTFLOW at +x)
   double a, b, c;            // stage coefficients
   const double *uk[4];       // U_k at the cell (MODE_FIRST / MODE_COMBINE)

@@ -411,11 +411,29 @@ template <typename R> struct PressNbr {
   R p0p, p0m, p1p, p1m;    // (0,+1) (0,-1) (1,+1) (1,-1)
 };
 
+// Constants of the ideal-gas law the fluxes take, formed once per solve on the
+// host in FP64 (gas_constants) instead of on every edge: gamma / (gamma - 1),
+// 2 ((gamma - 1) / (gamma + 1)) (which equals (2 (gamma - 1)) / (gamma + 1): a
+// product by 2 is exact) and (gamma + 1) / (2 gamma), each one IEEE operation
+// sequence on gamma, so the FP64 values are the bits the per-edge formulas
+// gave.  The precision form rounds them to binary32.
+template <typename R> struct Gas {
+  R gamma, gg, cr2, hf;
+};
+__host__ __device__ inline void gas_constants(double gamma, double out[3]) {
+  out[0] = gamma / (gamma - 1.0);
+  out[1] = 2.0 * ((gamma - 1.0) / (gamma + 1.0));
+  out[2] = (gamma + 1.0) / (2.0 * gamma);
+}
+template <typename R> __device__ inline Gas<R> gas_of(double gamma, const double gc[3]) {
+  return Gas<R>{R(gamma), R(gc[0]), R(gc[1]), R(gc[2])};
+}
+
 // SLAU family: slau_solver.fypp:355-490.  With D = FastOps: every numerator is
 // +0 or positive except V_n_plus / V_n_minus, whose quotients M_plus / M_minus
 // enter only Beta (even in a zero M) and g (whose sign of zero vanishes in 1 - g).
 template <typename R, int VARIANT, int DIR, class D = IeeeOps>
-__device__ inline void flux_slau(R gamma, const EdgeState<R> &s, const PressNbr<R> &pn, R out[4]) {
+__device__ inline void flux_slau(const Gas<R> &gas, const EdgeState<R> &s, const PressNbr<R> &pn, R out[4]) {
   const R n_x = edge_nx<R, DIR>(s), n_y = edge_ny<R, DIR>(s);
   const R THRESHOLD = R(5e-3), C_SD1 = R(0.01);
   const R eps = Lim<R>::eps;
@@ -438,6 +456,7 @@ __device__ inline void flux_slau(R gamma, const EdgeState<R> &s, const PressNbr<
   };
   R V_n_plus = u_L * n_x + v_L * n_y;
   R V_n_minus = u_R * n_x + v_R * n_y;
+  const R gamma = gas.gamma;
   R c_plus = D::sqrt_pos(over_rho(gamma * p_L, ir_L, dr_L));
   R c_minus = D::sqrt_pos(over_rho(gamma * p_R, ir_R, dr_R));
   R c_bar = R(0.5) * (c_plus + c_minus);
@@ -516,7 +535,7 @@ __device__ inline void flux_slau(R gamma, const EdgeState<R> &s, const PressNbr<
   else mass_flux = base - ((kRcp ? Chi * (R(0.5) * ic) : D::div(Chi, R(2) * c_bar)) * delta_p);
   R mfL = R(0.5) * (mass_flux + absd(mass_flux));
   R mfR = R(0.5) * (mass_flux - absd(mass_flux));
-  R gg = gamma / (gamma - R(1));
+  R gg = gas.gg;
   R H_L = gg * over_rho(p_L, ir_L, dr_L) + R(0.5) * (sq(u_L) + sq(v_L));
   R H_R = gg * over_rho(p_R, ir_R, dr_R) + R(0.5) * (sq(u_R) + sq(v_R));
   out[0] = mfL + mfR;
@@ -546,7 +565,7 @@ template <typename R> __device__ inline R pw_M_minus(R M) {
 }
 
 template <typename R, int DIR>
-__device__ inline void flux_ausmpw(R gamma, int low_mach_fix, const EdgeState<R> &s, const PressNbr<R> &pn,
+__device__ inline void flux_ausmpw(const Gas<R> &gas, int low_mach_fix, const EdgeState<R> &s, const PressNbr<R> &pn,
                                    R out[4]) {
   const R n_x = edge_nx<R, DIR>(s), n_y = edge_ny<R, DIR>(s);
   const R LOW_MACH = R(0.1);  // globals.f90:32
@@ -554,6 +573,7 @@ __device__ inline void flux_ausmpw(R gamma, int low_mach_fix, const EdgeState<R>
   R rho_R = s.rR, u_RHS = s.uR, v_RHS = s.vR, p_R = s.pR;
   R U_L = edge_ax0<R, DIR>(s) ? u_LHS : v_LHS, U_R = edge_ax0<R, DIR>(s) ? u_RHS : v_RHS;
   R V_L = edge_ax0<R, DIR>(s) ? v_LHS : u_LHS, V_R = edge_ax0<R, DIR>(s) ? v_RHS : u_RHS;
+  const R gamma = gas.gamma;
   R cs_L = sqrtr(gamma * p_L / rho_L);
   R cs_R = sqrtr(gamma * p_R / rho_R);
   R M_LHS = sqrtr(sq(u_LHS) + sq(v_LHS)) / cs_L;
@@ -566,11 +586,11 @@ __device__ inline void flux_ausmpw(R gamma, int low_mach_fix, const EdgeState<R>
     R vrf = R(0.5) * (v_LHS + v_RHS) - f_M * R(0.5) * (v_LHS - v_RHS);
     u_LHS = ulf; u_RHS = urf; v_LHS = vlf; v_RHS = vrf;
   }
-  R gg = gamma / (gamma - R(1));
+  R gg = gas.gg;
   R H_L = gg * (p_L / rho_L) + R(0.5) * (sq(u_LHS) + sq(v_LHS));
   R H_R = gg * (p_R / rho_R) + R(0.5) * (sq(u_RHS) + sq(v_RHS));
   R H_normal = fmin2(H_L - R(0.5) * sq(V_L), H_R - R(0.5) * sq(V_R));
-  R c_s = sqrtr(R(2) * ((gamma - R(1)) / (gamma + R(1))) * H_normal);
+  R c_s = sqrtr(gas.cr2 * H_normal);
   R c_half = (R(0.5) * (U_L + U_R) < R(0)) ? sq(c_s) / fmax2(absd(U_R), c_s) : sq(c_s) / fmax2(absd(U_L), c_s);
   R M_L = U_L / c_half;
   R M_R = U_R / c_half;
@@ -619,10 +639,11 @@ template <typename R> __device__ inline R roe_f_h(R pL, R pR) {
 }
 
 template <typename R, int DIR>
-__device__ inline void flux_roe(R gamma, const EdgeState<R> &s, const PressNbr<R> &pn, R out[4]) {
+__device__ inline void flux_roe(const Gas<R> &gas, const EdgeState<R> &s, const PressNbr<R> &pn, R out[4]) {
   const R nx = edge_nx<R, DIR>(s), ny = edge_ny<R, DIR>(s);
   R rl = absd(s.rL), ul = s.uL, vl = s.vL, pl = absd(s.pL);
-  R gg = gamma / (gamma - R(1));
+  const R gamma = gas.gamma;
+  R gg = gas.gg;
   R hl = gg * (pl / rl) + R(0.5) * (sq(ul) + sq(vl));
   R rr = absd(s.rR), ur = s.uR, vr = s.vR, pr = absd(s.pR);
   R hr = gg * (pr / rr) + R(0.5) * (sq(ur) + sq(vr));
@@ -683,15 +704,16 @@ __device__ inline void flux_roe(R gamma, const EdgeState<R> &s, const PressNbr<R
 // AUSM+ from interface_state (ausm_plus_solver.f90:499-643) with the AUSM+-u flags
 // (M_p = p_u = 0), beta = 1/8, alpha = 3/16, upwinded as in the commented solve (312-325).
 template <typename R, int DIR>
-__device__ inline void flux_ausm_plus(R gamma, const EdgeState<R> &s, R out[4]) {
+__device__ inline void flux_ausm_plus(const Gas<R> &gas, const EdgeState<R> &s, R out[4]) {
   const R n_x = edge_nx<R, DIR>(s), n_y = edge_ny<R, DIR>(s);
   const R beta = R(1) / R(8), alpha = R(3) / R(16);
   R rho_L = s.rL, u_L = s.uL, v_L = s.vL, p_L = s.pL;
   R rho_R = s.rR, u_R = s.uR, v_R = s.vR, p_R = s.pR;
-  R gamma_factor = ((R(2) * (gamma - R(1))) / (gamma + R(1)));
+  const R gamma = gas.gamma;
+  R gamma_factor = gas.cr2;
   R vel_L = u_L * n_x + v_L * n_y;
   R vel_R = u_R * n_x + v_R * n_y;
-  R gg = gamma / (gamma - R(1));
+  R gg = gas.gg;
   R H_L = (p_L / rho_L) * gg + R(0.5) * (u_L * u_L + v_L * v_L);
   R H_R = (p_R / rho_R) * gg + R(0.5) * (u_R * u_R + v_R * v_R);
   R a_crit_L = sqrtr(gamma_factor * absd(H_L));
@@ -733,10 +755,11 @@ __device__ inline void flux_ausm_plus(R gamma, const EdgeState<R> &s, R out[4]) 
 // contact speed eq. 10.37, star states eq. 10.39.  D = FastOps shares the
 // reciprocals of rho_L, rho_R and gamma - 1.
 template <typename R, int DIR, class D = IeeeOps>
-__device__ inline void flux_hllc(R gamma, const EdgeState<R> &s, R out[4]) {
+__device__ inline void flux_hllc(const Gas<R> &gas, const EdgeState<R> &s, R out[4]) {
   const R n_x = edge_nx<R, DIR>(s), n_y = edge_ny<R, DIR>(s);
   R rL = s.rL, uL = s.uL, vL = s.vL, pL = s.pL;
   R rR = s.rR, uR = s.uR, vR = s.vR, pR = s.pR;
+  const R gamma = gas.gamma;
   R gm1 = gamma - R(1);
   R qL = uL * n_x + vL * n_y;
   R qR = uR * n_x + vR * n_y;
@@ -747,7 +770,7 @@ __device__ inline void flux_hllc(R gamma, const EdgeState<R> &s, R out[4]) {
   R cs_bar = R(0.5) * (cL + cR);
   R p_pvrs = R(0.5) * (pL + pR) - R(0.5) * (qR - qL) * rho_bar * cs_bar;
   R p_star = D::max(R(0), p_pvrs);
-  R gfac = D::div(gamma + R(1), R(2) * gamma);
+  R gfac = gas.hf;
   R wL = (p_star <= pL) ? R(1) : D::sqrt_pos(R(1) + gfac * (D::div(p_star, pL) - R(1)));
   R wR = (p_star <= pR) ? R(1) : D::sqrt_pos(R(1) + gfac * (D::div(p_star, pR) - R(1)));
   R SL = qL - cL * wL;
@@ -805,7 +828,7 @@ template <typename R> __device__ inline R m_phi_R_half(R phi_L, R phi_R, R phi_R
 }
 
 template <typename R, int DIR>
-__device__ inline void flux_m_ausmpw(R gamma, int low_mach_fix, const EdgeState<R> &s, const EdgeState<R> &sb,
+__device__ inline void flux_m_ausmpw(const Gas<R> &gas, int low_mach_fix, const EdgeState<R> &s, const EdgeState<R> &sb,
                                      const PressNbr<R> &pn, R out[4]) {
   const R n_x = edge_nx<R, DIR>(s), n_y = edge_ny<R, DIR>(s);
   const R LOW_MACH = R(0.1);
@@ -814,6 +837,7 @@ __device__ inline void flux_m_ausmpw(R gamma, int low_mach_fix, const EdgeState<
   R rho_R = s.rR, u_RHS = s.uR, v_RHS = s.vR, p_R = s.pR;
   R U_L = edge_ax0<R, DIR>(s) ? u_LHS : v_LHS, U_R = edge_ax0<R, DIR>(s) ? u_RHS : v_RHS;
   R V_L = edge_ax0<R, DIR>(s) ? v_LHS : u_LHS, V_R = edge_ax0<R, DIR>(s) ? v_RHS : u_RHS;
+  const R gamma = gas.gamma;
   R cs_L = sqrtr(gamma * p_L / rho_L);
   R cs_R = sqrtr(gamma * p_R / rho_R);
   R M_LHS = sqrtr(sq(u_LHS) + sq(v_LHS)) / cs_L;
@@ -826,10 +850,10 @@ __device__ inline void flux_m_ausmpw(R gamma, int low_mach_fix, const EdgeState<
     R vrf = R(0.5) * (v_LHS + v_RHS) - f_M * R(0.5) * (v_LHS - v_RHS);
     u_LHS = ulf; u_RHS = urf; v_LHS = vlf; v_RHS = vrf;
   }
-  R H_L = (gamma / (gamma - R(1))) * (p_L / rho_L) + R(0.5) * (sq(u_LHS) + sq(v_LHS));
-  R H_R = (gamma / (gamma - R(1))) * (p_R / rho_R) + R(0.5) * (sq(u_RHS) + sq(v_RHS));
+  R H_L = gas.gg * (p_L / rho_L) + R(0.5) * (sq(u_LHS) + sq(v_LHS));
+  R H_R = gas.gg * (p_R / rho_R) + R(0.5) * (sq(u_RHS) + sq(v_RHS));
   R H_normal = fmin2(H_L - R(0.5) * sq(V_L), H_R - R(0.5) * sq(V_R));
-  R c_s = sqrtr(R(2) * ((gamma - R(1)) / (gamma + R(1))) * H_normal);
+  R c_s = sqrtr(gas.cr2 * H_normal);
   R c_half = (R(0.5) * (U_L + U_R) < R(0)) ? sq(c_s) / fmax2(absd(U_R), c_s) : sq(c_s) / fmax2(absd(U_L), c_s);
   R M_L = U_L / c_half;
   R M_R = U_R / c_half;
@@ -876,8 +900,8 @@ __device__ inline void flux_m_ausmpw(R gamma, int low_mach_fix, const EdgeState<
   R v_R_half = m_phi_R_half(v_LHS, v_RHS, sb.vR, a);
   R p_L_half = m_phi_L_half(p_L, p_R, sb.pL, a);
   R p_R_half = m_phi_R_half(p_L, p_R, sb.pR, a);
-  R H_L_half = (gamma / (gamma - R(1))) * (p_L_half / rho_L_half) + R(0.5) * (sq(u_L_half) + sq(v_L_half));
-  R H_R_half = (gamma / (gamma - R(1))) * (p_R_half / rho_R_half) + R(0.5) * (sq(u_R_half) + sq(v_R_half));
+  R H_L_half = gas.gg * (p_L_half / rho_L_half) + R(0.5) * (sq(u_L_half) + sq(v_L_half));
+  R H_R_half = gas.gg * (p_R_half / rho_R_half) + R(0.5) * (sq(u_R_half) + sq(v_R_half));
   R mfL = Mb_L * c_half * rho_L_half;
   R mfR = Mb_R * c_half * rho_R_half;
   if (absd(mfL + mfR) < FEPS) {
@@ -898,15 +922,15 @@ __device__ inline void flux_m_ausmpw(R gamma, int low_mach_fix, const EdgeState<
 template <int FLUX> constexpr bool needs_superbee_states() { return FLUX == FLUX_M_AUSMPW_PLUS; }
 
 template <typename R, int FLUX, int DIR, class D = IeeeOps>
-__device__ inline void edge_flux(R gamma, int low_mach_fix, const EdgeState<R> &s, const PressNbr<R> &pn, R out[4],
-                                 const EdgeState<R> *sb = nullptr) {
-  if constexpr (FLUX == FLUX_AUSMPW_PLUS) flux_ausmpw<R, DIR>(gamma, low_mach_fix, s, pn, out);
-  else if constexpr (FLUX == FLUX_M_AUSMPW_PLUS) flux_m_ausmpw<R, DIR>(gamma, low_mach_fix, s, *sb, pn, out);
+__device__ inline void edge_flux(const Gas<R> &gas, int low_mach_fix, const EdgeState<R> &s, const PressNbr<R> &pn,
+                                 R out[4], const EdgeState<R> *sb = nullptr) {
+  if constexpr (FLUX == FLUX_AUSMPW_PLUS) flux_ausmpw<R, DIR>(gas, low_mach_fix, s, pn, out);
+  else if constexpr (FLUX == FLUX_M_AUSMPW_PLUS) flux_m_ausmpw<R, DIR>(gas, low_mach_fix, s, *sb, pn, out);
   else if constexpr (FLUX == FLUX_SLAU || FLUX == FLUX_SLAU2 || FLUX == FLUX_SD_SLAU || FLUX == FLUX_SD_SLAU2)
-    flux_slau<R, FLUX, DIR, D>(gamma, s, pn, out);
-  else if constexpr (FLUX == FLUX_ROE) flux_roe<R, DIR>(gamma, s, pn, out);
-  else if constexpr (FLUX == FLUX_AUSM_PLUS) flux_ausm_plus<R, DIR>(gamma, s, out);
-  else flux_hllc<R, DIR, D>(gamma, s, out);
+    flux_slau<R, FLUX, DIR, D>(gas, s, pn, out);
+  else if constexpr (FLUX == FLUX_ROE) flux_roe<R, DIR>(gas, s, pn, out);
+  else if constexpr (FLUX == FLUX_AUSM_PLUS) flux_ausm_plus<R, DIR>(gas, s, out);
+  else flux_hllc<R, DIR, D>(gas, s, out);
 }
 
 // FP64 fluxes with a FastOps path (the SLAU family and HLLC; every limiter has

@@ -349,6 +349,7 @@ FieldGeom Solve::fgeom() const {
   g.bc[2] = cfg.bc_plus_y;
   g.bc[3] = cfg.bc_minus_y;
   g.gamma = cfg.gamma;
+  gas_constants(cfg.gamma, g.gas);
   g.cfl = cfg.cfl;
   g.bc_pressure = cfg.bc_pressure;
   g.dxi = dxi;
@@ -378,6 +379,7 @@ StageArgs Solve::stage_args(int mode, double a, double b, double c, int in_buf, 
   s.tiles_x = tiles_x;
   s.tiles_y = tiles_y;
   s.gamma = cfg.gamma;
+  gas_constants(cfg.gamma, s.gas);
   s.cfl = cfg.cfl;
   s.bc_pressure = cfg.bc_pressure;
   s.a = a;

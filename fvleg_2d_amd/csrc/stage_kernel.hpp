@@ -357,7 +357,7 @@ __device__ __forceinline__ bool tile_edge(const StageArgs &a, TileSmem<LIMC, R, 
     s.nx = idir ? R(1) : R(0);
     s.ny = idir ? R(0) : R(1);
   }
-  edge_flux<R, FLUX, DIR, DF>((R)a.gamma, a.low_mach_fix, s, pn, f, &sb);
+  edge_flux<R, FLUX, DIR, DF>(gas_of<R>(a.gamma, a.gas), a.low_mach_fix, s, pn, f, &sb);
   return true;
 }
 
