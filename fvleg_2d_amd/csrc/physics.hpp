@@ -466,6 +466,13 @@ __device__ inline void flux_slau(const Gas<R> &gas, const EdgeState<R> &s, const
     if constexpr (kRcp) return x * ic;
     else return D::div(x, dc);
   };
+  // x / (2 c_bar): with FastOps the quotient by c_bar halved, which is exact
+  // (the quotients of an in-range tile stay normal), so no second reciprocal
+  auto over_2c = [&](R x) {
+    if constexpr (kRcp) return x * (R(0.5) * ic);
+    else if constexpr (std::is_same<D, FastOps>::value) return D::div(x, dc) * R(0.5);
+    else return D::div(x, R(2) * c_bar);
+  };
   R V_n_bar = D::div(rho_L * absd(V_n_plus) + rho_R * absd(V_n_minus), rho_L + rho_R);
   R vel2 = sq(u_L) + sq(v_L) + sq(u_R) + sq(v_R);
   R M_hat = D::min(R(1), over_c(sqrtr(R(0.5) * vel2)));
@@ -531,8 +538,8 @@ __device__ inline void flux_slau(const Gas<R> &gas, const EdgeState<R> &s, const
   R g = -(D::max(D::min(M_plus, R(0)), R(-1)) * D::min(D::max(M_minus, R(0)), R(1)));
   R base = R(0.5) * ((rho_L * V_n_plus) + (rho_R * V_n_minus) - absd(V_n_bar) * delta_rho) * (R(1) - g);
   R mass_flux;
-  if constexpr (SD) mass_flux = base - ((kRcp ? theta * (R(0.5) * ic) : D::div(theta, R(2) * c_bar)) * delta_p);
-  else mass_flux = base - ((kRcp ? Chi * (R(0.5) * ic) : D::div(Chi, R(2) * c_bar)) * delta_p);
+  if constexpr (SD) mass_flux = base - (over_2c(theta) * delta_p);
+  else mass_flux = base - (over_2c(Chi) * delta_p);
   R mfL = R(0.5) * (mass_flux + absd(mass_flux));
   R mfR = R(0.5) * (mass_flux - absd(mass_flux));
   R gg = gas.gg;
