@@ -83,6 +83,13 @@ __device__ inline f32x copysignr(f32x a, f32x b) { return f32x(__builtin_copysig
 
 template <typename R> __host__ __device__ inline R fmax2(R a, R b) { return (b > a) ? b : a; }
 template <typename R> __host__ __device__ inline R fmin2(R a, R b) { return (b < a) ? b : a; }
+// The precision form takes the hardware v_max_f32 / v_min_f32 (one instruction
+// instead of a compare and a select): it differs from the compare-and-select
+// only for NaN operands and the sign of a zero result, and that form is judged
+// by its error against the FP64 path (a NaN state is still caught where the
+// next stage recovers the primitives)
+__device__ inline f32x fmax2(f32x a, f32x b) { return f32x(__builtin_fmaxf(a.v, b.v)); }
+__device__ inline f32x fmin2(f32x a, f32x b) { return f32x(__builtin_fminf(a.v, b.v)); }
 template <typename R> __host__ __device__ inline R sq(R x) { return x * x; }
 
 __host__ __device__ inline double absd(double x) { return __builtin_fabs(x); }
