@@ -370,12 +370,16 @@ __device__ inline void recon_mlp(R qm3, R qm, R q0, R q1, R q2, R q3, R a0p, R a
   }
   R tt = D::div(absd(a0p - a0m), absd(q1 - qm) + SMALL);
   R tt1 = D::div(absd(a1p - a1m), absd(q2 - q0) + SMALL);
-  R aLt = D::div(R(2) * fmax2(R(1), r_L) * (R(1) + fmax2(R(0), D::div(tt1, r_R_p1 + SMALL))), R(1) + tt);
-  R aRt = D::div(R(2) * fmax2(R(1), r_R_p1) * (R(1) + fmax2(R(0), D::div(tt, r_L + SMALL))), R(1) + tt1);
-  R aL = fmax2(R(1), fmin2(R(2), aLt));
-  R aR = fmax2(R(1), fmin2(R(2), aRt));
-  L = q0 + R(0.5) * fmax2(R(0), fmin2(fmin2(aL * r_L, aL), beta_L)) * dm;
-  Rv = q1 - R(0.5) * fmax2(R(0), fmin2(fmin2(aR * r_R_p1, aR), beta_R)) * d3;
+  // D::min / D::max: FastOps takes the hardware min/max.  On an in-range tile
+  // no operand is NaN, and where a zero's sign could differ from the
+  // compare-and-select (max(0, x) for x = +-0, and the last min) it is dropped
+  // by 1 + (+-0) = 1 or by the outer compare-and-select max with 0, kept as is
+  R aLt = D::div(R(2) * D::max(R(1), r_L) * (R(1) + D::max(R(0), D::div(tt1, r_R_p1 + SMALL))), R(1) + tt);
+  R aRt = D::div(R(2) * D::max(R(1), r_R_p1) * (R(1) + D::max(R(0), D::div(tt, r_L + SMALL))), R(1) + tt1);
+  R aL = D::max(R(1), D::min(R(2), aLt));
+  R aR = D::max(R(1), D::min(R(2), aRt));
+  L = q0 + R(0.5) * fmax2(R(0), D::min(D::min(aL * r_L, aL), beta_L)) * dm;
+  Rv = q1 - R(0.5) * fmax2(R(0), D::min(D::min(aR * r_R_p1, aR), beta_R)) * d3;
 }
 
 // ---------------------------------------------------------------------------
