@@ -177,6 +177,7 @@ struct IeeeOps {
   template <typename T> static __device__ T div(T a, Rcp<T> r) { return a / r.b; }
   template <typename T> static __device__ T div(T a, T b) { return a / b; }
   template <typename T> static __device__ T sqrt_pos(T x) { return sqrtr(x); }
+  template <typename T> static __device__ T sqrt0(T x) { return sqrtr(x); }
   template <typename T> static __device__ T min(T a, T b) { return fmin2(a, b); }
   template <typename T> static __device__ T max(T a, T b) { return fmax2(a, b); }
 };
@@ -200,6 +201,9 @@ struct FastOps {
   // an in-range tile the same value as the reference's compare-and-select, at
   // the sites that use them (no -0 operand meets a +0 one there, or the sign of
   // that zero is dropped: SLAU's g enters only through 1 - g)
+  // sqrt of +0 or of an in-range x (sqrt_pos): SLAU's |velocity| from squares of
+  // reconstructed velocities, which are 0 or at least 2^-218 in magnitude
+  static __device__ double sqrt0(double x) { return x == 0.0 ? 0.0 : sqrt_pos(x); }
   static __device__ double min(double a, double b) { return __builtin_fmin(a, b); }
   static __device__ double max(double a, double b) { return __builtin_fmax(a, b); }
   static __device__ double sqrt_pos(double x) {
@@ -486,7 +490,8 @@ __device__ inline void flux_slau(const Gas<R> &gas, const EdgeState<R> &s, const
   };
   R V_n_bar = D::div(rho_L * absd(V_n_plus) + rho_R * absd(V_n_minus), rho_L + rho_R);
   R vel2 = sq(u_L) + sq(v_L) + sq(u_R) + sq(v_R);
-  R M_hat = D::min(R(1), over_c(sqrtr(R(0.5) * vel2)));
+  const R vmag = D::sqrt0(R(0.5) * vel2);
+  R M_hat = D::min(R(1), over_c(vmag));
   R Chi = sq(R(1) - M_hat);
   R M_bar = over_c(V_n_bar);
   R p_bar = R(0.5) * (p_L + p_R);
@@ -545,7 +550,7 @@ __device__ inline void flux_slau(const Gas<R> &gas, const EdgeState<R> &s, const
     p_tilde = (p_bar + R(0.5) * delta_Beta * p_L_minus_R) + (R(1) - Chi) * (Beta_plus + Beta_minus - R(1)) * p_bar;
   else
     p_tilde = p_bar + (R(0.5) * delta_Beta * p_L_minus_R) +
-              sqrtr(R(0.5) * vel2) * (Beta_plus + Beta_minus - R(1)) * rho_bar * c_bar;
+              vmag * (Beta_plus + Beta_minus - R(1)) * rho_bar * c_bar;
   R g = -(D::max(D::min(M_plus, R(0)), R(-1)) * D::min(D::max(M_minus, R(0)), R(1)));
   R base = R(0.5) * ((rho_L * V_n_plus) + (rho_R * V_n_minus) - absd(V_n_bar) * delta_rho) * (R(1) - g);
   R mass_flux;
