@@ -362,12 +362,13 @@ __device__ __forceinline__ bool tile_edge(const StageArgs &a, TileSmem<LIMC, R, 
 }
 
 // Phase 2 of a stage tile: every i- and j-edge of the tile once, its flux
-// written to LDS.  With hold_j_flux (one cell per thread): the left i-edge of
-// each thread's cell, then the i-edges right of the tile and the j-edges above
-// it (the first TY + TX threads), then the j-edge below each thread's cell,
-// which stays in `hold` (zero outside the domain); the first and last rounds
-// take a single edge direction, folded at compile time.  Otherwise the edges
-// in order, i-edges first, with the direction at run time.
+// written to LDS.  SPLIT layout (one cell per thread): the left i-edge of each
+// thread's cell, then the i-edges right of the tile and the j-edges above it
+// (the first TY + TX threads), then the j-edge below each thread's cell, which
+// with hold_j_flux stays in `hold` (zero outside the domain) and otherwise goes
+// to sfj; the first and last rounds take a single edge direction, folded at
+// compile time.  Otherwise the edges in order, i-edges first, with the
+// direction at run time.
 template <int FLUX, int LIMC, typename R, int TX, int TY, class D, class DF, bool GEOM, int NT, bool SPLIT>
 __device__ inline void edge_pass(const StageArgs &a, TileSmem<LIMC, R, TX, TY, NT, SPLIT> &sm, int i0, int j0,
                                  R (&hold)[4]) {
